@@ -1,0 +1,128 @@
+/*
+ * owt.h -- C ABI of the MI355X channel-flow time step.
+ *
+ * Replaces the reference solver's free-function surface (src/setup.hpp:84-114),
+ * called by its driver in the order of src/main.cpp:63-87:
+ *
+ *   load_params + allocate_var + init_mat + init_fft   -> owt_create
+ *   init_fields                                         -> owt_init_fields / owt_set_velocity
+ *   time_Integration (x NT+1) and its stdout diagnostics -> owt_step, owt_wall_shear, owt_diag
+ *   save_instant_field                                  -> owt_get_field (+ owt_checkpoint)
+ *   cleanup_fft + process exit                          -> owt_destroy
+ *
+ * Fields are exchanged in the reference's wave-space layout [NZ][NXH][NY]
+ * (src/setup.hpp:58-69, y fastest) as interleaved complex doubles (re, im).
+ * Every function returns 0 on success or a status code (OWT_E_*) and never
+ * lets an exception cross the ABI; owt_last_error() gives the message.
+ * A handle is one channel on one GPU with its own HIP stream; calls on one
+ * handle are not thread-safe, distinct handles are independent.
+ */
+#ifndef OWT_H
+#define OWT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    OWT_OK = 0,
+    OWT_E_CONFIG = 1,   /* case file missing, key missing or malformed (src/IO.cpp:36-39) */
+    OWT_E_ARG = 2,      /* invalid argument or unsupported size */
+    OWT_E_DEVICE = 3,   /* HIP runtime error, out of device memory, no device */
+    OWT_E_SINGULAR = 4, /* zero pivot in an LU factorization (src/linalg.hpp:641-644) */
+    OWT_E_IO = 5        /* file could not be written or read */
+};
+
+/* field ids for owt_get_field */
+enum {
+    OWT_U = 0, OWT_V = 1, OWT_W = 2,           /* velx, vely, velz */
+    OWT_OMX = 3, OWT_OMY = 4, OWT_OMZ = 5,     /* vorx, vory, vorz */
+    OWT_LMBX = 6, OWT_LMBY = 7, OWT_LMBZ = 8,  /* Lamb vector */
+    OWT_HV = 9, OWT_HG = 10                     /* nonlinear sources of the v and omega_y equations */
+};
+
+/* methods and precisions */
+enum { OWT_PLAIN = 0, OWT_BATCHED = 1 };
+enum { OWT_FP64 = 0, OWT_MIXED = 1 };
+
+typedef struct owt_handle owt_handle;
+
+typedef struct {
+    int device;    /* HIP device ordinal */
+    int precision; /* -1: from the case file ([owt] precision), else OWT_FP64 / OWT_MIXED */
+    int method;    /* -1: from the case file ([owt] method), else OWT_PLAIN / OWT_BATCHED */
+    int verbose;   /* print the reference's per-step friction line (src/solver.cpp:314) */
+} owt_opts;
+
+/* The reference's toml keys (src/IO.cpp:14-25) for callers without a case file. */
+typedef struct {
+    double nu, dt, dkx, dkz, alpha_mesh;
+    int NX, NY, NZ;
+    int precision, method;
+} owt_params;
+
+/* load_params + allocate_var + init_mat + init_fft (src/IO.cpp:8-74,
+ * src/solver.cpp:15-216, src/FFT.cpp:11-34). opts may be NULL. */
+int owt_create(const char *toml_path, const owt_opts *opts, owt_handle **out);
+int owt_create_params(const owt_params *params, const owt_opts *opts, owt_handle **out);
+/* cleanup_fft + frees (src/FFT.cpp:36-47) */
+void owt_destroy(owt_handle *h);
+/* message of the last failed call on h (or of the last failed owt_create* when h is NULL) */
+const char *owt_last_error(const owt_handle *h);
+
+/* sizes: NX, NY, NZ, NXH, NZH, NX2, NZ2 */
+int owt_get_sizes(const owt_handle *h, int *sizes7);
+
+/* init_fields (src/solver.cpp:329-426). mode 1: laminar profile plus the
+ * reference's fixed (2, +-2) wave; mode 2: seeded divergence-free perturbation
+ * of peak amplitude amp (seed, amp unused in mode 1). */
+int owt_init_fields(owt_handle *h, int mode, uint64_t seed, double amp);
+/* injects u, v, w and runs the rest of init_fields (src/solver.cpp:379-425) */
+int owt_set_velocity(owt_handle *h, const double *u, const double *v, const double *w);
+
+/* nsteps calls of time_Integration (src/solver.cpp:541-627) */
+int owt_step(owt_handle *h, int nsteps);
+/* the latest friction line of src/solver.cpp:314: fx_fric, bottom, upper */
+int owt_wall_shear(owt_handle *h, double *fx, double *bottom, double *upper);
+/* up to max_rows most recent (fx_fric, bottom, upper) rows, oldest first;
+ * row 0 is the evaluation of init_fields. Returns the row count (>= 0) or -code. */
+int owt_diag(owt_handle *h, double *out, int max_rows);
+/* number of completed steps (id_step) or -code */
+long owt_steps_done(owt_handle *h);
+
+/* current field in [NZ][NXH][NY] complex interleaved */
+int owt_get_field(owt_handle *h, int field, double *out);
+
+int owt_set_method(owt_handle *h, int method);
+int owt_set_precision(owt_handle *h, int precision);
+
+/* full state: all history levels, mean-flow and friction history, step counter */
+int owt_checkpoint(owt_handle *h, const char *path);
+int owt_restore(owt_handle *h, const char *path);
+
+/* Fourier2Physical_dealiased / Physical2Fourier_dealiased (src/FFT.cpp:49-194)
+ * on nf fields (FP64); physical layout [NZ2][NX2][NY] */
+int owt_f2p(owt_handle *h, const double *wave, int nf, double *phys);
+int owt_p2f(owt_handle *h, const double *phys, int nf, double *wave);
+/* Lamb vector of (u, v, w, omx, omy, omz) through the step's transform kernels
+ * (src/solver.cpp:262-291) in the handle's precision; three output fields */
+int owt_lamb(owt_handle *h, const double *six, double *three);
+
+/* host-side operators, column-major N x N: 0..3 DY1..DY4 */
+int owt_get_matrix(owt_handle *h, int which, double *out);
+
+/* timing: enqueue nsteps steps on the handle's stream between two HIP events
+ * and return the elapsed milliseconds (inputs stay resident on the device) */
+int owt_time_steps(owt_handle *h, int nsteps, double *ms);
+/* timing of one stage kernel launched reps times on the handle's stream:
+ * 0 y_rhs, 1 implicit solves, 2 horizontal, 3 z inverse, 4 x phase,
+ * 5 z forward, 6 nonlinear. Returns the mean milliseconds per launch. */
+int owt_time_kernel(owt_handle *h, int kernel, int reps, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OWT_H */

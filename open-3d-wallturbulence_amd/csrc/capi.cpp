@@ -1,0 +1,265 @@
+// extern "C" layer: status codes in place of the reference's exceptions and
+// aborts; nothing throws across the ABI.
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "../../include/owt.h"
+#include "channel.hpp"
+
+struct owt_handle {
+    std::unique_ptr<owt::Channel> ch;
+    std::string err;
+};
+
+namespace {
+
+std::mutex g_mu;
+std::string g_create_err;
+
+template <typename F>
+int guard(owt_handle *h, F &&f) {
+    if (!h) return OWT_E_ARG;
+    try {
+        f();
+        return OWT_OK;
+    } catch (const owt::Error &e) {
+        h->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        h->err = "host out of memory";
+        return OWT_E_DEVICE;
+    } catch (const std::exception &e) {
+        h->err = e.what();
+        return OWT_E_DEVICE;
+    }
+}
+
+int create(const owt::Params &p, const owt_opts *opts, owt_handle **out) {
+    owt::Params q = p;
+    int device = 0;
+    bool verbose = false;
+    if (opts) {
+        device = opts->device;
+        verbose = opts->verbose != 0;
+        if (opts->precision >= 0) q.precision = opts->precision;
+        if (opts->method >= 0) q.method = opts->method;
+    }
+    if (q.precision < 0 || q.precision > 1 || q.method < 0 || q.method > 1)
+        throw owt::Error(OWT_E_ARG, "precision and method must be 0 or 1");
+    auto h = std::make_unique<owt_handle>();
+    h->ch = std::make_unique<owt::Channel>(q, device, verbose);
+    *out = h.release();
+    return OWT_OK;
+}
+
+int create_guarded(const owt::Params *p, const char *path, const owt_opts *opts, owt_handle **out) {
+    if (!out) return OWT_E_ARG;
+    *out = nullptr;
+    try {
+        owt::Params q = path ? owt::load_params(path) : *p;
+        if (!path) owt::derive_sizes(q);
+        return create(q, opts, out);
+    } catch (const owt::Error &e) {
+        std::lock_guard<std::mutex> l(g_mu);
+        g_create_err = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        std::lock_guard<std::mutex> l(g_mu);
+        g_create_err = e.what();
+        return OWT_E_DEVICE;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int owt_create(const char *toml_path, const owt_opts *opts, owt_handle **out) {
+    if (!toml_path) return OWT_E_ARG;
+    return create_guarded(nullptr, toml_path, opts, out);
+}
+
+int owt_create_params(const owt_params *params, const owt_opts *opts, owt_handle **out) {
+    if (!params) return OWT_E_ARG;
+    owt::Params p;
+    p.nu = params->nu;
+    p.dt = params->dt;
+    p.dkx = params->dkx;
+    p.dkz = params->dkz;
+    p.alpha_mesh = params->alpha_mesh;
+    p.NX = params->NX;
+    p.NY = params->NY;
+    p.NZ = params->NZ;
+    p.precision = params->precision;
+    p.method = params->method;
+    return create_guarded(&p, nullptr, opts, out);
+}
+
+void owt_destroy(owt_handle *h) {
+    try {
+        delete h;
+    } catch (...) {
+    }
+}
+
+const char *owt_last_error(const owt_handle *h) {
+    if (h) return h->err.c_str();
+    std::lock_guard<std::mutex> l(g_mu);
+    return g_create_err.c_str();
+}
+
+int owt_get_sizes(const owt_handle *h, int *s) {
+    if (!h || !s) return OWT_E_ARG;
+    const owt::Params &p = h->ch->params();
+    s[0] = (int)p.NX;
+    s[1] = (int)p.NY;
+    s[2] = (int)p.NZ;
+    s[3] = (int)p.NXH;
+    s[4] = (int)p.NZH;
+    s[5] = (int)p.NX2;
+    s[6] = (int)p.NZ2;
+    return OWT_OK;
+}
+
+int owt_init_fields(owt_handle *h, int mode, uint64_t seed, double amp) {
+    return guard(h, [&] { h->ch->init_fields(mode, seed, amp); });
+}
+
+int owt_set_velocity(owt_handle *h, const double *u, const double *v, const double *w) {
+    if (!u || !v || !w) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->set_velocity(u, v, w); });
+}
+
+int owt_step(owt_handle *h, int nsteps) { return guard(h, [&] { h->ch->step(nsteps); }); }
+
+int owt_wall_shear(owt_handle *h, double *fx, double *bottom, double *upper) {
+    if (!fx || !bottom || !upper) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->wall_shear(fx, bottom, upper); });
+}
+
+int owt_diag(owt_handle *h, double *out, int max_rows) {
+    if (!out || max_rows < 0) return -OWT_E_ARG;
+    int rows = 0;
+    int rc = guard(h, [&] { rows = h->ch->diag(out, max_rows); });
+    return rc ? -rc : rows;
+}
+
+long owt_steps_done(owt_handle *h) {
+    long n = 0;
+    int rc = guard(h, [&] { n = h->ch->steps_done(); });
+    return rc ? -rc : n;
+}
+
+int owt_get_field(owt_handle *h, int field, double *out) {
+    if (!out) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->get_field(field, out); });
+}
+
+int owt_set_method(owt_handle *h, int method) { return guard(h, [&] { h->ch->set_method(method); }); }
+
+int owt_set_precision(owt_handle *h, int precision) { return guard(h, [&] { h->ch->set_precision(precision); }); }
+
+int owt_checkpoint(owt_handle *h, const char *path) {
+    if (!path) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->checkpoint(path); });
+}
+
+int owt_restore(owt_handle *h, const char *path) {
+    if (!path) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->restore(path); });
+}
+
+int owt_f2p(owt_handle *h, const double *wave, int nf, double *phys) {
+    if (!wave || !phys) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->f2p(wave, nf, phys); });
+}
+
+int owt_p2f(owt_handle *h, const double *phys, int nf, double *wave) {
+    if (!wave || !phys) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->p2f(phys, nf, wave); });
+}
+
+int owt_lamb(owt_handle *h, const double *six, double *three) {
+    if (!six || !three) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->lamb(six, three); });
+}
+
+int owt_get_matrix(owt_handle *h, int which, double *out) {
+    if (!out) return OWT_E_ARG;
+    return guard(h, [&] {
+        const owt::Operators &op = h->ch->ops();
+        const std::vector<double> *m[] = {&op.DY1, &op.DY2, &op.DY3, &op.DY4};
+        if (which < 0 || which > 3) throw owt::Error(OWT_E_ARG, "matrix id must be 0..3");
+        std::memcpy(out, m[which]->data(), m[which]->size() * sizeof(double));
+    });
+}
+
+int owt_time_steps(owt_handle *h, int nsteps, double *ms) {
+    if (!ms || nsteps < 1) return OWT_E_ARG;
+    return guard(h, [&] {
+        owt::Channel &c = *h->ch;
+        hipEvent_t a, b;
+        OWT_HIP(hipEventCreate(&a));
+        OWT_HIP(hipEventCreate(&b));
+        c.enqueue_step();  // builds and uploads the graph outside the timed region
+        c.sync();
+        OWT_HIP(hipEventRecord(a, c.stream()));
+        for (int k = 0; k < nsteps; ++k) c.enqueue_step();
+        OWT_HIP(hipEventRecord(b, c.stream()));
+        OWT_HIP(hipEventSynchronize(b));
+        float t = 0;
+        OWT_HIP(hipEventElapsedTime(&t, a, b));
+        *ms = t;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    });
+}
+
+int owt_time_kernel(owt_handle *h, int kernel, int reps, double *ms) {
+    if (!ms || reps < 1) return OWT_E_ARG;
+    return guard(h, [&] {
+        owt::Channel &c = *h->ch;
+        const owt::Chan &d = c.dev();
+        hipStream_t s = c.stream();
+        const bool plain = c.params().method == 0;
+        if (!plain) c.enqueue_step();  // factors exist before timing the solves
+        c.sync();
+        auto launch = [&] {
+            switch (kernel) {
+                case 0: owt::launch_y_rhs(d, s); break;
+                case 1:
+                    if (plain) owt::launch_lu_plain(d, s);
+                    else owt::launch_lu_apply(d, 0, s);
+                    break;
+                case 2: owt::launch_horizontal(d, s); break;
+                case 3: owt::launch_z_inverse(d, nullptr, -6, d.zbuf, s); break;
+                case 4: owt::launch_x_lamb(d, s); break;
+                case 5: owt::launch_z_forward(d, s); break;
+                case 6: owt::launch_nonlinear(d, s); break;
+                default: throw owt::Error(OWT_E_ARG, "kernel id must be 0..6");
+            }
+        };
+        const owt::Chan &dd = c.dev();
+        (void)dd;
+        launch();
+        OWT_HIP(hipGetLastError());
+        c.sync();
+        hipEvent_t a, b;
+        OWT_HIP(hipEventCreate(&a));
+        OWT_HIP(hipEventCreate(&b));
+        OWT_HIP(hipEventRecord(a, s));
+        for (int r = 0; r < reps; ++r) launch();
+        OWT_HIP(hipEventRecord(b, s));
+        OWT_HIP(hipEventSynchronize(b));
+        float t = 0;
+        OWT_HIP(hipEventElapsedTime(&t, a, b));
+        *ms = t / reps;
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,700 @@
+// Channel runtime: device memory, initial fields, the step sequence and its
+// hipGraph capture, field access and full-state checkpoints.
+#include "channel.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+
+namespace owt {
+
+FFTPlan make_plan(int L) {
+    FFTPlan p;
+    p.L = L;
+    int n = L;
+    while (n % 4 == 0) p.radix[p.nst++] = 4, n /= 4;
+    while (n % 3 == 0) p.radix[p.nst++] = 3, n /= 3;
+    while (n % 2 == 0) p.radix[p.nst++] = 2, n /= 2;
+    if (n != 1 || p.nst > kMaxStages) throw Error(2, "transform length " + std::to_string(L) + " is not 2^a 3^b");
+    return p;
+}
+
+namespace {
+
+std::vector<double2> twiddles(int L) {
+    std::vector<double2> t(L);
+    for (int m = 0; m < L; ++m) {
+        long double th = 2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)L;
+        t[m] = make_double2((double)cosl(th), -(double)sinl(th));
+    }
+    return t;
+}
+
+}  // namespace
+
+void *Channel::dmalloc(size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    OWT_HIP(hipMalloc(&p, bytes));
+    allocs_.push_back(p);
+    return p;
+}
+
+Channel::Channel(const Params &p, int device, bool verbose) : P_(p), device_(device), verbose_(verbose) {
+    int ndev = 0;
+    OWT_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) throw Error(3, "no HIP device " + std::to_string(device));
+    OWT_HIP(hipSetDevice(device));
+    OWT_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    op_ = build_operators((int)P_.NY, P_.alpha_mesh);
+    try {
+        alloc();
+        upload_operators();
+    } catch (...) {
+        free_all();
+        throw;
+    }
+}
+
+Channel::~Channel() {
+    (void)hipSetDevice(device_);
+    drop_graph();
+    free_all();
+}
+
+void Channel::free_all() {
+    if (s_) (void)hipStreamSynchronize(s_);
+    for (void *p : allocs_) (void)hipFree(p);
+    allocs_.clear();
+    if (s_) (void)hipStreamDestroy(s_);
+    s_ = nullptr;
+}
+
+void Channel::alloc() {
+    Chan &c = c_;
+    c.NX = (int)P_.NX;
+    c.NY = (int)P_.NY;
+    c.NZ = (int)P_.NZ;
+    c.NXH = (int)P_.NXH;
+    c.NZH = (int)P_.NZH;
+    c.NX2 = (int)P_.NX2;
+    c.NZ2 = (int)P_.NZ2;
+    c.npencil = c.NZ * c.NXH;
+    c.nsys = c.NXH * (c.NZH + 1);
+    c.W = (size_t)c.NZ * c.NXH * c.NY;
+    c.I = (size_t)c.NZ2 * c.NXH * c.NY;
+    c.nu = P_.nu;
+    c.dt = P_.dt;
+    c.Ly = op_.y[c.NY - 1] - op_.y[0];
+    c.precision = P_.precision;
+    const size_t es = P_.precision == 0 ? sizeof(double2) : sizeof(float2);
+    c.hist = (double2 *)dmalloc(12 * c.W * sizeof(double2));
+    c.cur = (double2 *)dmalloc(4 * c.W * sizeof(double2));
+    c.src = (double2 *)dmalloc(2 * c.W * sizeof(double2));
+    // transform buffers sized for FP64 so the precision can be switched in place
+    c.lmb = dmalloc(3 * c.W * sizeof(double2));
+    c.zbuf = dmalloc(6 * c.I * sizeof(double2));
+    c.xbuf = dmalloc(3 * c.I * sizeof(double2));
+    (void)es;
+    const size_t N = c.NY;
+    c.DY1 = (const double *)dmalloc(N * N * sizeof(double));
+    c.DY2 = (const double *)dmalloc(N * N * sizeof(double));
+    c.DY4 = (const double *)dmalloc(N * N * sizeof(double));
+    c.kx = (const double *)dmalloc(c.NXH * sizeof(double));
+    c.kz = (const double *)dmalloc(c.NZ * sizeof(double));
+    c.means = (double2 *)dmalloc(12 * N * sizeof(double2));
+    c.fric = (double *)dmalloc(8 * sizeof(double));
+    c.mrhs = (double2 *)dmalloc(2 * N * sizeof(double2));
+    c.msol = (double2 *)dmalloc(2 * N * sizeof(double2));
+    c.ctrl = (int *)dmalloc(16 * sizeof(int));
+    c.diag_cap = 1 << 16;
+    c.diag = (double *)dmalloc((size_t)c.diag_cap * 3 * sizeof(double));
+    c.twx = (const double2 *)dmalloc(c.NX2 * sizeof(double2));
+    c.twz = (const double2 *)dmalloc(c.NZ2 * sizeof(double2));
+    c.twxf = (const float2 *)dmalloc(c.NX2 * sizeof(float2));
+    c.twzf = (const float2 *)dmalloc(c.NZ2 * sizeof(float2));
+    c.px = make_plan(c.NX2);
+    c.pz = make_plan(c.NZ2);
+    c.LUv = c.LUg = nullptr;
+    c.permv = c.permg = nullptr;
+    OWT_HIP(hipMemsetAsync(c.hist, 0, 12 * c.W * sizeof(double2), s_));
+    OWT_HIP(hipMemsetAsync(c.cur, 0, 4 * c.W * sizeof(double2), s_));
+    OWT_HIP(hipMemsetAsync(c.lmb, 0, 3 * c.W * sizeof(double2), s_));
+    OWT_HIP(hipMemsetAsync(c.ctrl, 0, 16 * sizeof(int), s_));
+    OWT_HIP(hipMemsetAsync(c.diag, 0, (size_t)c.diag_cap * 3 * sizeof(double), s_));
+}
+
+void Channel::upload_operators() {
+    const size_t N = c_.NY;
+    OWT_HIP(hipMemcpyAsync((void *)c_.DY1, op_.DY1.data(), N * N * sizeof(double), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync((void *)c_.DY2, op_.DY2.data(), N * N * sizeof(double), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync((void *)c_.DY4, op_.DY4.data(), N * N * sizeof(double), hipMemcpyHostToDevice, s_));
+    // wavenumbers (src/solver.cpp:22-37) with PX = PZ = 1
+    std::vector<double> kx(c_.NXH), kz(c_.NZ);
+    for (int i = 0; i < c_.NXH; ++i) kx[i] = (double)i * P_.dkx;
+    for (int k = 0; k < c_.NZ; ++k) {
+        double t = (double)k;
+        if (t >= c_.NZH) t -= c_.NZ;
+        kz[k] = t * P_.dkz;
+    }
+    OWT_HIP(hipMemcpyAsync((void *)c_.kx, kx.data(), kx.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync((void *)c_.kz, kz.data(), kz.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+    auto tx = twiddles(c_.NX2), tz = twiddles(c_.NZ2);
+    std::vector<float2> txf(tx.size()), tzf(tz.size());
+    for (size_t m = 0; m < tx.size(); ++m) txf[m] = make_float2((float)tx[m].x, (float)tx[m].y);
+    for (size_t m = 0; m < tz.size(); ++m) tzf[m] = make_float2((float)tz[m].x, (float)tz[m].y);
+    OWT_HIP(hipMemcpyAsync((void *)c_.twx, tx.data(), tx.size() * sizeof(double2), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync((void *)c_.twz, tz.data(), tz.size() * sizeof(double2), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync((void *)c_.twxf, txf.data(), txf.size() * sizeof(float2), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync((void *)c_.twzf, tzf.data(), tzf.size() * sizeof(float2), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipStreamSynchronize(s_));
+}
+
+// The operators Av(k^2), Ag(k^2) do not change in time: factor each distinct
+// k^2 once (the reference refactors every pencil every step).
+void Channel::factor_operators() {
+    if (factored_) return;
+    const size_t N = c_.NY, ns = c_.nsys;
+    c_.LUv = (double *)dmalloc(ns * N * N * sizeof(double));
+    c_.LUg = (double *)dmalloc(ns * N * N * sizeof(double));
+    c_.permv = (int *)dmalloc(ns * N * sizeof(int));
+    c_.permg = (int *)dmalloc(ns * N * sizeof(int));
+    launch_lu_factor_store(c_, s_);
+    OWT_HIP(hipGetLastError());
+    OWT_HIP(hipStreamSynchronize(s_));
+    check_errors();
+    factored_ = true;
+}
+
+void Channel::check_errors() {
+    int err = 0;
+    OWT_HIP(hipMemcpy(&err, c_.ctrl + C_ERR, sizeof(int), hipMemcpyDeviceToHost));
+    if (err & E_SINGULAR) {
+        int zero = 0;
+        OWT_HIP(hipMemcpy(c_.ctrl + C_ERR, &zero, sizeof(int), hipMemcpyHostToDevice));
+        throw Error(4, "LU factorization failed. Matrix A may be singular.");
+    }
+}
+
+void Channel::drop_graph() {
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    gexec_ = nullptr;
+    graph_ = nullptr;
+}
+
+void Channel::set_method(int method) {
+    if (method != 0 && method != 1) throw Error(2, "method must be 0 (plain) or 1 (batched)");
+    P_.method = method;
+    drop_graph();
+}
+
+void Channel::set_precision(int precision) {
+    if (precision != 0 && precision != 1) throw Error(2, "precision must be 0 (fp64) or 1 (mixed)");
+    if (precision == P_.precision) return;
+    // the Lamb vector is kept in the transform precision: convert it in place
+    std::vector<double> l(3 * c_.W * 2);
+    if (P_.precision == 0) {
+        OWT_HIP(hipMemcpy(l.data(), c_.lmb, l.size() * sizeof(double), hipMemcpyDeviceToHost));
+        std::vector<float> f(l.begin(), l.end());
+        OWT_HIP(hipMemcpy(c_.lmb, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(l.size());
+        OWT_HIP(hipMemcpy(f.data(), c_.lmb, f.size() * sizeof(float), hipMemcpyDeviceToHost));
+        std::vector<double> d(f.begin(), f.end());
+        OWT_HIP(hipMemcpy(c_.lmb, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    P_.precision = precision;
+    c_.precision = precision;
+    drop_graph();
+}
+
+// ---------------------------------------------------------------------------
+// initial fields
+// ---------------------------------------------------------------------------
+namespace {
+
+uint64_t splitmix(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// complex N(0,1) keyed by (seed, field, i, m, p): SplitMix64 + Box-Muller
+void seeded_gauss(uint64_t seed, int field, int i, int m, int p, double &re, double &im) {
+    uint64_t h = splitmix(seed ^ 0x5851F42D4C957F2DULL);
+    h = splitmix(h ^ (uint64_t)(int64_t)field);
+    h = splitmix(h ^ (uint64_t)(int64_t)i);
+    h = splitmix(h ^ (uint64_t)(int64_t)m);
+    h = splitmix(h ^ (uint64_t)(int64_t)p);
+    const uint64_t h2 = splitmix(h);
+    const double u1 = (double)((h >> 11) + 1) * 0x1.0p-53;
+    const double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+    const double r = std::sqrt(-2.0 * std::log(u1));
+    const double th = 6.283185307179586476925286766559 * u2;
+    re = r * std::cos(th);
+    im = r * std::sin(th);
+}
+
+}  // namespace
+
+double Channel::peak_velocity_dev(const std::vector<double> &u, const std::vector<double> &v, const std::vector<double> &w) {
+    const size_t W = c_.W, I = c_.I, P = (size_t)c_.NZ2 * c_.NX2 * c_.NY;
+    std::vector<void *> tmp;
+    auto tm = [&](size_t b) {
+        void *p = nullptr;
+        OWT_HIP(hipMalloc(&p, b));
+        tmp.push_back(p);
+        return p;
+    };
+    double peak = 0.0;
+    try {
+        double2 *wave = (double2 *)tm(3 * W * sizeof(double2));
+        double2 *zp = (double2 *)tm(3 * I * sizeof(double2));
+        double *phys = (double *)tm(3 * P * sizeof(double));
+        unsigned long long *mx = (unsigned long long *)tm(sizeof(unsigned long long));
+        OWT_HIP(hipMemcpyAsync(wave, u.data(), W * sizeof(double2), hipMemcpyHostToDevice, s_));
+        OWT_HIP(hipMemcpyAsync(wave + W, v.data(), W * sizeof(double2), hipMemcpyHostToDevice, s_));
+        OWT_HIP(hipMemcpyAsync(wave + 2 * W, w.data(), W * sizeof(double2), hipMemcpyHostToDevice, s_));
+        OWT_HIP(hipMemsetAsync(mx, 0, sizeof(unsigned long long), s_));
+        Chan d = c_;
+        d.precision = 0;
+        const double2 *in[3] = {wave, wave + W, wave + 2 * W};
+        launch_z_inverse(d, in, 3, zp, s_);
+        launch_x_c2r(d, zp, 3, phys, s_);
+        launch_peak3(phys, P, mx, s_);
+        OWT_HIP(hipGetLastError());
+        unsigned long long bits = 0;
+        OWT_HIP(hipMemcpyAsync(&bits, mx, sizeof(bits), hipMemcpyDeviceToHost, s_));
+        OWT_HIP(hipStreamSynchronize(s_));
+        double m2;
+        std::memcpy(&m2, &bits, sizeof(m2));
+        peak = std::sqrt(m2);
+    } catch (...) {
+        for (void *p : tmp) (void)hipFree(p);
+        throw;
+    }
+    for (void *p : tmp) (void)hipFree(p);
+    return peak;
+}
+
+void Channel::init_fields(int mode, uint64_t seed, double amp) {
+    const int NZ = c_.NZ, NXH = c_.NXH, NY = c_.NY, NZH = c_.NZH;
+    const size_t W = c_.W;
+    std::vector<double> u(2 * W, 0.0), v(2 * W, 0.0), w(2 * W, 0.0);
+    auto at = [&](int k, int i) { return ((size_t)k * NXH + i) * NY; };
+    std::vector<double> kx(NXH), kz(NZ);
+    for (int i = 0; i < NXH; ++i) kx[i] = (double)i * P_.dkx;
+    for (int k = 0; k < NZ; ++k) kz[k] = ((double)k >= NZH ? (double)k - NZ : (double)k) * P_.dkz;
+    const std::vector<double> &y = op_.y;
+    auto laminar = [&](std::vector<double> &uu, bool add) {
+        for (int j = 0; j < NY; ++j) {
+            const double U = 1.5 * (1 - std::pow(y[j], 2));
+            uu[2 * j] = add ? uu[2 * j] + U : U;
+            if (!add) uu[2 * j + 1] = 0.0;
+        }
+    };
+    if (mode == 1) {
+        // laminar profile plus the fixed wave at (kx, kz) = (2, +-2) (src/solver.cpp:336-377)
+        laminar(u, false);
+        const double k1 = 2.0, k2 = 2.0, al = 1.0;
+        for (int k = 0; k < NZ; ++k)
+            for (int i = 0; i < NXH; ++i) {
+                const bool plus = kz[k] == k2 && kx[i] == k1, minus = kz[k] == -k2 && kx[i] == k1;
+                if (!plus && !minus) continue;
+                for (int j = 0; j < NY; ++j) {
+                    const double yy = y[j];
+                    const size_t o = 2 * (at(k, i) + j);
+                    const double fu = al / k1 * (yy - std::pow(yy, 3));
+                    const double fv = al * (0.5 * std::pow(yy, 2) - 0.25 * std::pow(yy, 4) - 0.25) * (0.25);
+                    const double fw = -2 * al / k2 * (yy - std::pow(yy, 3));
+                    // f * (-0.25 i) = (f * -0, f * -0.25); the real part adds a signed zero
+                    u[o] += fu * -0.0;
+                    u[o + 1] += fu * -0.25;
+                    v[o] += fv;
+                    w[o] += fw * (plus ? -0.0 : 0.0);
+                    w[o + 1] += fw * (plus ? -0.25 : 0.25);
+                }
+            }
+    } else if (mode == 2) {
+        // seeded divergence-free perturbation (recipe defined by this build)
+        std::vector<double> g(2 * W, 0.0);
+        for (int k = 0; k < NZ; ++k) {
+            if (k == NZH) continue;
+            const int m = k < NZH ? k : k - NZ;
+            for (int i = 0; i < NXH; ++i) {
+                const int r2 = i * i + m * m;
+                if (r2 < 1 || r2 > 64) continue;
+                if (i == 0 && m < 0) continue;
+                double cr[4], ci[4], dr[4], di[4];
+                for (int p = 0; p < 4; ++p) {
+                    seeded_gauss(seed, 0, i, m, p, cr[p], ci[p]);
+                    seeded_gauss(seed, 1, i, m, p, dr[p], di[p]);
+                }
+                const size_t o = at(k, i), oc = (i == 0) ? at(NZ - m, 0) : 0;
+                for (int j = 0; j < NY; ++j) {
+                    const double yy = y[j];
+                    const double T[4] = {1.0, yy, 2.0 * yy * yy - 1.0, (4.0 * yy * yy - 3.0) * yy};
+                    const double w1 = 1.0 - yy * yy;
+                    // complex * real products summed left to right as the oracle's std::complex does
+                    double svr = ((cr[0] * T[0] + cr[1] * T[1]) + cr[2] * T[2]) + cr[3] * T[3];
+                    double svi = ((ci[0] * T[0] + ci[1] * T[1]) + ci[2] * T[2]) + ci[3] * T[3];
+                    double sgr = ((dr[0] * T[0] + dr[1] * T[1]) + dr[2] * T[2]) + dr[3] * T[3];
+                    double sgi = ((di[0] * T[0] + di[1] * T[1]) + di[2] * T[2]) + di[3] * T[3];
+                    const double ww = w1 * w1;
+                    v[2 * (o + j)] = ww * svr;
+                    v[2 * (o + j) + 1] = ww * svi;
+                    g[2 * (o + j)] = w1 * sgr;
+                    g[2 * (o + j) + 1] = w1 * sgi;
+                    if (i == 0) {
+                        v[2 * (oc + j)] = ww * svr;
+                        v[2 * (oc + j) + 1] = -(ww * svi);
+                        g[2 * (oc + j)] = w1 * sgr;
+                        g[2 * (oc + j) + 1] = -(w1 * sgi);
+                    }
+                }
+            }
+        }
+        // u, w from v, omega_y with the discrete DY1 (src/solver.cpp:532-533)
+        std::vector<double> d1r(NY), d1i(NY);
+        for (int k = 0; k < NZ; ++k)
+            for (int i = 0; i < NXH; ++i) {
+                const double k2 = std::pow(kx[i], 2) + std::pow(kz[k], 2);
+                if (k2 < 1.e-8) continue;
+                const size_t o = at(k, i);
+                // only the pencils that received modes are non-zero
+                const int m = k < NZH ? k : k - NZ;
+                if (i * i + m * m > 64) continue;
+                std::fill(d1r.begin(), d1r.end(), 0.0);
+                std::fill(d1i.begin(), d1i.end(), 0.0);
+                for (int j = 0; j < NY; ++j) {
+                    const double xr = v[2 * (o + j)], xi = v[2 * (o + j) + 1];
+                    const double *col = op_.DY1.data() + (size_t)j * NY;
+                    for (int r = 0; r < NY; ++r) {
+                        d1r[r] = d1r[r] + col[r] * xr;
+                        d1i[r] = d1i[r] + col[r] * xi;
+                    }
+                }
+                const double ix = kx[i] / k2, iz = kz[k] / k2;
+                for (int j = 0; j < NY; ++j) {
+                    const double gr = g[2 * (o + j)], gi = g[2 * (o + j) + 1];
+                    // (0, ix) * d1 - (0, iz) * g ;  (0, ix) * g + (0, iz) * d1
+                    u[2 * (o + j)] = (0.0 * d1r[j] - ix * d1i[j]) - (0.0 * gr - iz * gi);
+                    u[2 * (o + j) + 1] = (0.0 * d1i[j] + ix * d1r[j]) - (0.0 * gi + iz * gr);
+                    w[2 * (o + j)] = (0.0 * gr - ix * gi) + (0.0 * d1r[j] - iz * d1i[j]);
+                    w[2 * (o + j) + 1] = (0.0 * gi + ix * gr) + (0.0 * d1i[j] + iz * d1r[j]);
+                }
+            }
+        const double peak = peak_velocity_dev(u, v, w);
+        if (peak > 0.0) {
+            const double f = amp / peak;
+            for (size_t e = 0; e < 2 * W; ++e) {
+                u[e] *= f;
+                v[e] *= f;
+                w[e] *= f;
+            }
+        }
+        laminar(u, true);
+    } else {
+        throw Error(2, "init mode must be 1 (reference) or 2 (seeded)");
+    }
+    set_velocity(u.data(), v.data(), w.data());
+}
+
+void Channel::set_velocity(const double *u, const double *v, const double *w) {
+    const size_t W = c_.W;
+    int ctrl[16] = {0};
+    OWT_HIP(hipMemcpyAsync(c_.ctrl, ctrl, sizeof(ctrl), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync(c_.cur, u, W * sizeof(double2), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync(c_.cur + W, w, W * sizeof(double2), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync(c_.hist + (size_t)(H_V * 3) * W, v, W * sizeof(double2), hipMemcpyHostToDevice, s_));
+    host_steps_ = 0;
+    init_post();
+}
+
+// src/solver.cpp:379-425
+void Channel::init_post() {
+    const size_t W = c_.W;
+    launch_vor_from_vel(c_, s_);
+    const double2 *six[6] = {c_.cur, c_.hist + (size_t)(H_V * 3) * W, c_.cur + W, c_.cur + 2 * W, c_.hist + (size_t)(H_G * 3) * W,
+                             c_.cur + 3 * W};
+    launch_z_inverse(c_, six, 6, c_.zbuf, s_);
+    launch_x_lamb(c_, s_);
+    launch_z_forward(c_, s_);
+    launch_nonlinear(c_, s_);
+    launch_init_history(c_, s_);
+    launch_init_mean(c_, s_);
+    OWT_HIP(hipGetLastError());
+    OWT_HIP(hipStreamSynchronize(s_));
+}
+
+// ---------------------------------------------------------------------------
+// time step
+// ---------------------------------------------------------------------------
+void Channel::enqueue_step_body() {
+    const bool plain = P_.method == 0;
+    launch_y_rhs(c_, s_);
+    if (plain) launch_lu_plain(c_, s_);
+    else launch_lu_apply(c_, 0, s_);
+    launch_horizontal(c_, s_);
+    launch_mean_rhs(c_, s_);
+    launch_mean_solve(c_, plain, s_);
+    launch_mean_post(c_, s_);
+    // the six transformed fields; v and omega_y are read from the history slot
+    // the solves just wrote, found on the device from the ring counter
+    launch_z_inverse(c_, nullptr, -6, c_.zbuf, s_);
+    launch_x_lamb(c_, s_);
+    launch_z_forward(c_, s_);
+    launch_nonlinear(c_, s_);
+    launch_advance(c_, s_);
+}
+
+void Channel::build_graph() {
+    drop_graph();
+    OWT_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_step_body();
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    OWT_HIP(hipStreamEndCapture(s_, &graph_));
+    OWT_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+}
+
+void Channel::enqueue_step() {
+    if (P_.method == 1) {
+        factor_operators();
+        if (!gexec_) build_graph();
+        OWT_HIP(hipGraphLaunch(gexec_, s_));
+    } else {
+        enqueue_step_body();
+        OWT_HIP(hipGetLastError());
+    }
+    ++host_steps_;
+}
+
+void Channel::sync() { OWT_HIP(hipStreamSynchronize(s_)); }
+
+void Channel::step(int n) {
+    if (n < 0) throw Error(2, "step count must be non-negative");
+    for (int k = 0; k < n; ++k) {
+        enqueue_step();
+        if (verbose_) {
+            sync();
+            double fx, b, u;
+            wall_shear(&fx, &b, &u);
+            std::printf("fx_fric, %g, bottom, %g, upper, %g\n", fx, b, u);
+        }
+    }
+    sync();
+    check_errors();
+}
+
+long Channel::steps_done() {
+    int ctrl[2];
+    OWT_HIP(hipMemcpy(ctrl, c_.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+    return ctrl[C_STEP];
+}
+
+// ---------------------------------------------------------------------------
+// access
+// ---------------------------------------------------------------------------
+void Channel::get_field(int f, double *out) {
+    if (f < 0 || f >= F_COUNT) throw Error(2, "unknown field id " + std::to_string(f));
+    sync();
+    int ctrl[2];
+    OWT_HIP(hipMemcpy(ctrl, c_.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+    const int r = ctrl[C_RING];
+    const size_t W = c_.W;
+    const double2 *src = nullptr;
+    switch (f) {
+        case F_U: src = c_.cur; break;
+        case F_V: src = c_.hist + (size_t)(H_V * 3 + r) * W; break;
+        case F_W: src = c_.cur + W; break;
+        case F_OMX: src = c_.cur + 2 * W; break;
+        case F_OMY: src = c_.hist + (size_t)(H_G * 3 + r) * W; break;
+        case F_OMZ: src = c_.cur + 3 * W; break;
+        case F_HV: src = c_.hist + (size_t)(H_HV * 3 + r) * W; break;
+        case F_HG: src = c_.hist + (size_t)(H_HG * 3 + r) * W; break;
+        default: break;
+    }
+    if (src) {
+        OWT_HIP(hipMemcpy(out, src, W * sizeof(double2), hipMemcpyDeviceToHost));
+        return;
+    }
+    const int q = f - F_LMBX;
+    if (c_.precision == 0) {
+        OWT_HIP(hipMemcpy(out, (const double2 *)c_.lmb + q * W, W * sizeof(double2), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(2 * W);
+        OWT_HIP(hipMemcpy(tmp.data(), (const float2 *)c_.lmb + q * W, W * sizeof(float2), hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < 2 * W; ++e) out[e] = tmp[e];
+    }
+}
+
+void Channel::wall_shear(double *fx, double *bottom, double *upper) {
+    sync();
+    int ctrl[2];
+    OWT_HIP(hipMemcpy(ctrl, c_.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+    double d[3];
+    OWT_HIP(hipMemcpy(d, c_.diag + (size_t)(ctrl[C_STEP] % c_.diag_cap) * 3, sizeof(d), hipMemcpyDeviceToHost));
+    *fx = d[0];
+    *bottom = d[1];
+    *upper = d[2];
+}
+
+int Channel::diag(double *out, int max_rows) {
+    sync();
+    const long n = steps_done() + 1;
+    long rows = n < max_rows ? n : max_rows;
+    if (rows > c_.diag_cap) rows = c_.diag_cap;
+    const long first = n - rows;
+    for (long r = 0; r < rows; ++r) {
+        const long idx = (first + r) % c_.diag_cap;
+        OWT_HIP(hipMemcpy(out + 3 * r, c_.diag + idx * 3, 3 * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    return (int)rows;
+}
+
+// ---------------------------------------------------------------------------
+// stand-alone transforms (FP64)
+// ---------------------------------------------------------------------------
+void Channel::f2p(const double *wave, int nf, double *phys) {
+    if (nf < 1 || nf > 6) throw Error(2, "f2p takes 1..6 fields");
+    const size_t W = c_.W, I = c_.I, P = (size_t)c_.NZ2 * c_.NX2 * c_.NY;
+    double2 *dw = nullptr, *dz = nullptr;
+    double *dp = nullptr;
+    OWT_HIP(hipMalloc(&dw, nf * W * sizeof(double2)));
+    OWT_HIP(hipMalloc(&dz, nf * I * sizeof(double2)));
+    OWT_HIP(hipMalloc(&dp, nf * P * sizeof(double)));
+    OWT_HIP(hipMemcpy(dw, wave, nf * W * sizeof(double2), hipMemcpyHostToDevice));
+    Chan d = c_;
+    d.precision = 0;
+    const double2 *in[6];
+    for (int f = 0; f < nf; ++f) in[f] = dw + f * W;
+    launch_z_inverse(d, in, nf, dz, s_);
+    launch_x_c2r(d, dz, nf, dp, s_);
+    OWT_HIP(hipGetLastError());
+    OWT_HIP(hipStreamSynchronize(s_));
+    OWT_HIP(hipMemcpy(phys, dp, nf * P * sizeof(double), hipMemcpyDeviceToHost));
+    (void)hipFree(dw);
+    (void)hipFree(dz);
+    (void)hipFree(dp);
+}
+
+void Channel::p2f(const double *phys, int nf, double *wave) {
+    if (nf < 1 || nf > 6) throw Error(2, "p2f takes 1..6 fields");
+    const size_t W = c_.W, I = c_.I, P = (size_t)c_.NZ2 * c_.NX2 * c_.NY;
+    double2 *dw = nullptr, *dz = nullptr;
+    double *dp = nullptr;
+    OWT_HIP(hipMalloc(&dw, nf * W * sizeof(double2)));
+    OWT_HIP(hipMalloc(&dz, nf * I * sizeof(double2)));
+    OWT_HIP(hipMalloc(&dp, nf * P * sizeof(double)));
+    OWT_HIP(hipMemcpy(dp, phys, nf * P * sizeof(double), hipMemcpyHostToDevice));
+    Chan d = c_;
+    d.precision = 0;
+    launch_x_r2c(d, dp, nf, dz, s_);
+    launch_z_forward_to(d, dz, nf, dw, s_);
+    OWT_HIP(hipGetLastError());
+    OWT_HIP(hipStreamSynchronize(s_));
+    OWT_HIP(hipMemcpy(wave, dw, nf * W * sizeof(double2), hipMemcpyDeviceToHost));
+    (void)hipFree(dw);
+    (void)hipFree(dz);
+    (void)hipFree(dp);
+}
+
+// Lamb vector of six given wave fields through the step's own transform kernels
+void Channel::lamb(const double *six, double *three) {
+    const size_t W = c_.W;
+    double2 *dw = nullptr;
+    OWT_HIP(hipMalloc(&dw, 6 * W * sizeof(double2)));
+    OWT_HIP(hipMemcpy(dw, six, 6 * W * sizeof(double2), hipMemcpyHostToDevice));
+    const double2 *in[6];
+    for (int f = 0; f < 6; ++f) in[f] = dw + f * W;
+    launch_z_inverse(c_, in, 6, c_.zbuf, s_);
+    launch_x_lamb(c_, s_);
+    launch_z_forward(c_, s_);
+    OWT_HIP(hipGetLastError());
+    OWT_HIP(hipStreamSynchronize(s_));
+    (void)hipFree(dw);
+    for (int q = 0; q < 3; ++q) get_field(F_LMBX + q, three + 2 * W * q);
+}
+
+// ---------------------------------------------------------------------------
+// full-state checkpoint: every history level, the mean-flow and friction
+// history and the step counter (the reference's snapshots hold u, v, w only)
+// ---------------------------------------------------------------------------
+namespace {
+struct CkptHeader {
+    char magic[8];
+    int32_t version, NX, NY, NZ, precision, pad;
+    double nu, dt, dkx, dkz, alpha;
+};
+}  // namespace
+
+void Channel::checkpoint(const std::string &path) {
+    sync();
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw Error(5, "cannot open " + path + " for writing");
+    CkptHeader h{};
+    std::memcpy(h.magic, "OWTCKPT1", 8);
+    h.version = 1;
+    h.NX = c_.NX;
+    h.NY = c_.NY;
+    h.NZ = c_.NZ;
+    h.precision = c_.precision;
+    h.nu = P_.nu;
+    h.dt = P_.dt;
+    h.dkx = P_.dkx;
+    h.dkz = P_.dkz;
+    h.alpha = P_.alpha_mesh;
+    f.write((const char *)&h, sizeof(h));
+    auto dump = [&](const void *d, size_t bytes) {
+        std::vector<char> buf(bytes);
+        OWT_HIP(hipMemcpy(buf.data(), d, bytes, hipMemcpyDeviceToHost));
+        f.write(buf.data(), (std::streamsize)bytes);
+    };
+    const size_t W = c_.W, N = c_.NY;
+    dump(c_.hist, 12 * W * sizeof(double2));
+    dump(c_.cur, 4 * W * sizeof(double2));
+    dump(c_.lmb, 3 * W * (c_.precision == 0 ? sizeof(double2) : sizeof(float2)));
+    dump(c_.means, 12 * N * sizeof(double2));
+    dump(c_.fric, 8 * sizeof(double));
+    dump(c_.ctrl, 16 * sizeof(int));
+    dump(c_.diag, (size_t)c_.diag_cap * 3 * sizeof(double));
+    if (!f) throw Error(5, "write failed: " + path);
+}
+
+void Channel::restore(const std::string &path) {
+    sync();
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw Error(5, "cannot open " + path);
+    CkptHeader h{};
+    f.read((char *)&h, sizeof(h));
+    if (!f || std::memcmp(h.magic, "OWTCKPT1", 8) != 0) throw Error(5, path + " is not a checkpoint");
+    if (h.NX != c_.NX || h.NY != c_.NY || h.NZ != c_.NZ || h.nu != P_.nu || h.dt != P_.dt || h.dkx != P_.dkx || h.dkz != P_.dkz ||
+        h.alpha != P_.alpha_mesh)
+        throw Error(5, "checkpoint " + path + " was written for a different case");
+    if (h.precision != c_.precision) set_precision(h.precision);
+    auto load = [&](void *d, size_t bytes) {
+        std::vector<char> buf(bytes);
+        f.read(buf.data(), (std::streamsize)bytes);
+        if (!f) throw Error(5, "truncated checkpoint " + path);
+        OWT_HIP(hipMemcpy(d, buf.data(), bytes, hipMemcpyHostToDevice));
+    };
+    const size_t W = c_.W, N = c_.NY;
+    load(c_.hist, 12 * W * sizeof(double2));
+    load(c_.cur, 4 * W * sizeof(double2));
+    load(c_.lmb, 3 * W * (c_.precision == 0 ? sizeof(double2) : sizeof(float2)));
+    load(c_.means, 12 * N * sizeof(double2));
+    load(c_.fric, 8 * sizeof(double));
+    load(c_.ctrl, 16 * sizeof(int));
+    load(c_.diag, (size_t)c_.diag_cap * 3 * sizeof(double));
+}
+
+}  // namespace owt

@@ -1,0 +1,80 @@
+// One channel resident on one GPU: the drop-in for the reference's global
+// solver state (src/setup.hpp:21-83) and its setup/step call sequence
+// (src/main.cpp:63-87).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "owt_internal.hpp"
+
+namespace owt {
+
+// field ids of the C ABI (owt.h)
+enum FieldId { F_U = 0, F_V, F_W, F_OMX, F_OMY, F_OMZ, F_LMBX, F_LMBY, F_LMBZ, F_HV, F_HG, F_COUNT };
+
+class Channel {
+  public:
+    Channel(const Params &p, int device, bool verbose);
+    ~Channel();
+    Channel(const Channel &) = delete;
+    Channel &operator=(const Channel &) = delete;
+
+    const Params &params() const { return P_; }
+    const Operators &ops() const { return op_; }
+    const Chan &dev() const { return c_; }
+    hipStream_t stream() const { return s_; }
+
+    // init_fields (src/solver.cpp:329-426); mode 1 reference wave, 2 seeded recipe
+    void init_fields(int mode, uint64_t seed, double amp);
+    // set u, v, w ([NZ][NXH][NY] complex interleaved) and run the init post-processing
+    void set_velocity(const double *u, const double *v, const double *w);
+    // n calls of time_Integration (src/solver.cpp:541-627)
+    void step(int n);
+    void get_field(int f, double *out);
+    void wall_shear(double *fx, double *bottom, double *upper);
+    int diag(double *out, int max_rows);
+    long steps_done();
+    void set_method(int method);
+    void set_precision(int precision);
+    void checkpoint(const std::string &path);
+    void restore(const std::string &path);
+
+    // stand-alone transforms, FP64 (stage-level parity)
+    void f2p(const double *wave, int nf, double *phys);
+    void p2f(const double *phys, int nf, double *wave);
+    void lamb(const double *six, double *three);
+
+    // device-side timing helpers (bench): run one step on the stream without syncing
+    void enqueue_step();
+    void sync();
+
+  private:
+    void alloc();
+    void free_all();
+    void upload_operators();
+    void factor_operators();
+    void init_post();
+    void enqueue_step_body();
+    void build_graph();
+    void drop_graph();
+    void check_errors();
+    void *dmalloc(size_t bytes);
+    double peak_velocity_dev(const std::vector<double> &u, const std::vector<double> &v, const std::vector<double> &w);
+
+    Params P_;
+    Operators op_;
+    Chan c_{};
+    int device_ = 0;
+    bool verbose_ = false;
+    hipStream_t s_ = nullptr;
+    std::vector<void *> allocs_;
+    bool factored_ = false;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t gexec_ = nullptr;
+    long host_steps_ = 0;
+};
+
+FFTPlan make_plan(int L);
+
+}  // namespace owt

@@ -1,0 +1,165 @@
+// Case-file loader: the reference's toml keys (src/IO.cpp:8-74) read by a small
+// flat-key parser (the reference vendors toml11; only `key = value` lines,
+// comments, strings and one `[owt]` table of extension keys are needed here).
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <sstream>
+
+#include "owt_internal.hpp"
+
+namespace owt {
+
+namespace {
+
+std::string trim(const std::string &s) {
+    size_t a = 0, b = s.size();
+    while (a < b && std::isspace((unsigned char)s[a])) ++a;
+    while (b > a && std::isspace((unsigned char)s[b - 1])) --b;
+    return s.substr(a, b - a);
+}
+
+// strip a trailing comment that is not inside a string
+std::string strip_comment(const std::string &s) {
+    bool in_str = false;
+    for (size_t i = 0; i < s.size(); ++i) {
+        if (s[i] == '"') in_str = !in_str;
+        if (s[i] == '#' && !in_str) return s.substr(0, i);
+    }
+    return s;
+}
+
+struct Value {
+    std::string raw;
+    bool is_string = false;
+};
+
+std::map<std::string, Value> parse_flat(const std::string &path) {
+    std::ifstream f(path);
+    if (!f) throw Error(1, "Error loading configuration file:cannot open " + path);
+    std::map<std::string, Value> kv;
+    std::string line, table;
+    int lineno = 0;
+    while (std::getline(f, line)) {
+        ++lineno;
+        std::string t = trim(strip_comment(line));
+        if (t.empty()) continue;
+        if (t.front() == '[') {
+            if (t.back() != ']') throw Error(1, "Error loading configuration file:bad table header at line " + std::to_string(lineno));
+            table = trim(t.substr(1, t.size() - 2));
+            continue;
+        }
+        size_t eq = t.find('=');
+        if (eq == std::string::npos) throw Error(1, "Error loading configuration file:expected key = value at line " + std::to_string(lineno));
+        std::string key = trim(t.substr(0, eq)), val = trim(t.substr(eq + 1));
+        if (!table.empty()) key = table + "." + key;
+        Value v;
+        if (!val.empty() && val.front() == '"') {
+            if (val.size() < 2 || val.back() != '"') throw Error(1, "Error loading configuration file:unterminated string at line " + std::to_string(lineno));
+            v.raw = val.substr(1, val.size() - 2);
+            v.is_string = true;
+        } else {
+            v.raw = val;
+        }
+        kv[key] = v;
+    }
+    return kv;
+}
+
+const Value &need(const std::map<std::string, Value> &kv, const std::string &k) {
+    auto it = kv.find(k);
+    if (it == kv.end()) throw Error(1, "Error loading configuration file:key \"" + k + "\" not found");
+    return it->second;
+}
+
+double as_double(const Value &v, const std::string &k) {
+    std::string s = v.raw;
+    s.erase(std::remove(s.begin(), s.end(), '_'), s.end());
+    char *end = nullptr;
+    double d = std::strtod(s.c_str(), &end);
+    if (v.is_string || s.empty() || *end != '\0' || !std::isfinite(d))
+        throw Error(1, "Error loading configuration file:key \"" + k + "\" is not a number");
+    return d;
+}
+
+long as_int(const Value &v, const std::string &k) {
+    std::string s = v.raw;
+    s.erase(std::remove(s.begin(), s.end(), '_'), s.end());
+    char *end = nullptr;
+    long n = std::strtol(s.c_str(), &end, 10);
+    if (v.is_string || s.empty() || *end != '\0')
+        throw Error(1, "Error loading configuration file:key \"" + k + "\" is not an integer");
+    if (n < 0) throw Error(1, "Error loading configuration file:key \"" + k + "\" must be non-negative");
+    return n;
+}
+
+}  // namespace
+
+void derive_sizes(Params &p) {
+    if (p.NX < 4 || p.NZ < 4 || p.NX % 2 || p.NZ % 2)
+        throw Error(2, "NX and NZ must be even and at least 4");
+    if (p.NY < 7) throw Error(2, "NY must be at least 7 for the 3rd/4th derivative schemes");
+    if (p.NY > 256) throw Error(2, "NY above 256 is not supported by the wall-normal kernels");
+    if (!(p.dt > 0) || !(p.nu > 0)) throw Error(2, "nu and dt must be positive");
+    p.NXH = p.NX / 2;
+    p.NZH = p.NZ / 2;
+    p.NX2 = p.NX / 2 * 3;
+    p.NZ2 = p.NZ / 2 * 3;
+    auto smooth = [](long n) {
+        while (n % 2 == 0) n /= 2;
+        while (n % 3 == 0) n /= 3;
+        return n == 1;
+    };
+    if (!smooth(p.NX2) || !smooth(p.NZ2) || p.NX2 > 1152 || p.NZ2 > 1152)
+        throw Error(2, "3/2-padded lengths NX2, NZ2 must be of the form 2^a 3^b and at most 1152");
+}
+
+// load_params (src/IO.cpp:8-34). Missing or malformed keys are an error here,
+// where the reference prints the message and continues.
+Params load_params(const std::string &path) {
+    auto kv = parse_flat(path);
+    Params p;
+    p.dt = as_double(need(kv, "dt"), "dt");
+    p.dkx = as_double(need(kv, "dkx"), "dkx");
+    p.dkz = as_double(need(kv, "dkz"), "dkz");
+    p.NX = as_int(need(kv, "NX"), "NX");
+    p.NY = as_int(need(kv, "NY"), "NY");
+    p.NZ = as_int(need(kv, "NZ"), "NZ");
+    p.alpha_mesh = as_double(need(kv, "alpha_mesh"), "alpha_mesh");
+    p.NT = as_int(need(kv, "NT"), "NT");
+    p.PX = as_int(need(kv, "PX"), "PX");
+    p.PZ = as_int(need(kv, "PZ"), "PZ");
+    p.save_interval = as_int(need(kv, "save_interval"), "save_interval");
+    p.nu = as_double(need(kv, "nu"), "nu");
+    if (p.save_interval == 0) throw Error(1, "Error loading configuration file:save_interval must be positive");
+    // extension table
+    auto it = kv.find("owt.precision");
+    if (it != kv.end()) {
+        if (it->second.raw == "fp64") p.precision = 0;
+        else if (it->second.raw == "mixed") p.precision = 1;
+        else throw Error(1, "Error loading configuration file:owt.precision must be \"fp64\" or \"mixed\"");
+    }
+    it = kv.find("owt.method");
+    if (it != kv.end()) {
+        if (it->second.raw == "plain") p.method = 0;
+        else if (it->second.raw == "batched") p.method = 1;
+        else throw Error(1, "Error loading configuration file:owt.method must be \"plain\" or \"batched\"");
+    }
+    it = kv.find("owt.init");
+    if (it != kv.end()) {
+        if (it->second.raw == "reference") p.init_mode = 1;
+        else if (it->second.raw == "seeded") p.init_mode = 2;
+        else throw Error(1, "Error loading configuration file:owt.init must be \"reference\" or \"seeded\"");
+    }
+    it = kv.find("owt.seed");
+    if (it != kv.end()) p.seed = (uint64_t)as_int(it->second, "owt.seed");
+    it = kv.find("owt.amp");
+    if (it != kv.end()) p.amp = as_double(it->second, "owt.amp");
+    derive_sizes(p);
+    return p;
+}
+
+}  // namespace owt

@@ -1,0 +1,93 @@
+// Device-side data structures and launch entry points of the HIP kernels.
+// One Chan describes one channel resident in HBM; it is passed to every kernel
+// by value. Wave-space fields use the reference layout [NZ][NXH][NY] (complex,
+// y fastest, src/setup.hpp:58-69); the z-padded intermediate is [NZ2][NXH][NY].
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace owt {
+
+constexpr int kMaxStages = 8;
+constexpr int kMaxNY = 256;
+
+// 1-D transform plan: length L = prod(radix[s]), radices in {2, 3, 4}.
+struct FFTPlan {
+    int L = 0;
+    int nst = 0;
+    int radix[kMaxStages] = {0};
+};
+
+// history field indices inside Chan::hist ([field][slot] of W elements each)
+enum HistField { H_V = 0, H_G = 1, H_HV = 2, H_HG = 3 };
+// mean-flow history indices inside Chan::means ([quantity][slot][NY])
+enum MeanField { M_LMBX = 0, M_LMBZ = 1, M_VELX = 2, M_VELZ = 3 };
+// control words inside Chan::ctrl
+enum Ctrl { C_RING = 0, C_STEP = 1, C_ERR = 2 };
+// error bits
+enum ErrBits { E_SINGULAR = 1 };
+
+struct Chan {
+    int NX, NY, NZ, NXH, NZH, NX2, NZ2;
+    int npencil;  // NZ * NXH
+    int nsys;     // NXH * (NZH + 1): distinct k^2 shared by +-kz
+    size_t W;     // NZ * NXH * NY complex elements
+    size_t I;     // NZ2 * NXH * NY complex elements
+    double nu, dt, Ly;
+    int precision;  // 0 fp64 transforms, 1 fp32 transforms
+    double2 *hist;  // 12 W: v, omega_y, Hv, Hg at 3 levels
+    double2 *cur;   // 4 W: u, w, omega_x, omega_z
+    void *lmb;      // 3 W of double2 or float2
+    double2 *src;   // 2 W: right-hand sides of the v and omega_y systems
+    void *zbuf;     // 6 I of double2 or float2 (z-inverse output)
+    void *xbuf;     // 3 I of double2 or float2 (x-phase output)
+    const double *DY1, *DY2, *DY4;  // N x N column-major
+    const double *kx, *kz;
+    double2 *means;  // 4 x 3 x NY
+    double *fric;    // fx[3], fz[3]
+    double2 *mrhs;   // 2 x NY mean-flow right-hand sides
+    double2 *msol;   // 2 x NY mean-flow solutions
+    int *ctrl;
+    double *diag;    // diag_cap x 3: fx_fric, bottom, upper per friction evaluation
+    int diag_cap;
+    double *LUv, *LUg;   // nsys x N x N factors (rows in pivot order), column-major
+    int *permv, *permg;  // nsys x N pivot rows
+    const double2 *twx, *twz;  // e^{-2 pi i m / L}, fp64
+    const float2 *twxf, *twzf; // same, fp32
+    FFTPlan px, pz;
+};
+
+// ---- FFT phase launches (fft_kernels.hip) ----
+// z inverse: nf wave fields -> nf z-padded fields; fields given as pointer table
+void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out, hipStream_t s);
+// fused x phase: 6 z-padded fields -> Lamb vector (3 z-padded fields)
+void launch_x_lamb(const Chan &c, hipStream_t s);
+// z forward: 3 z-padded fields -> 3 wave fields (c.lmb)
+void launch_z_forward(const Chan &c, hipStream_t s);
+// stand-alone x transforms used by the physical-space utilities:
+// c2r: nf z-padded fields -> physical [NZ2][NX2][NY] doubles; r2c: inverse
+void launch_x_c2r(const Chan &c, const void *in, int nf, double *phys, hipStream_t s);
+void launch_x_r2c(const Chan &c, const double *phys, int nf, void *out, hipStream_t s);
+// z forward of nf fields into given double2 wave outputs
+void launch_z_forward_to(const Chan &c, const void *in, int nf, double2 *out, hipStream_t s);
+// max over the physical grid of u^2+v^2+w^2 (three consecutive fields), stored as bits
+void launch_peak3(const double *phys, size_t n, unsigned long long *out, hipStream_t s);
+
+// ---- wall-normal phase launches (y_kernels.hip) ----
+void launch_y_rhs(const Chan &c, hipStream_t s);
+void launch_lu_plain(const Chan &c, hipStream_t s);           // per-step factor + solve (reference method)
+void launch_lu_factor_store(const Chan &c, hipStream_t s);    // factor once per distinct k^2
+void launch_lu_apply(const Chan &c, int grid, hipStream_t s); // persistent solve with stored factors
+void launch_horizontal(const Chan &c, hipStream_t s);
+void launch_mean_rhs(const Chan &c, hipStream_t s);
+void launch_mean_solve(const Chan &c, bool plain, hipStream_t s);
+void launch_mean_post(const Chan &c, hipStream_t s);
+void launch_nonlinear(const Chan &c, hipStream_t s);
+void launch_advance(const Chan &c, hipStream_t s);
+// initialisation helpers (src/solver.cpp:379-425)
+void launch_vor_from_vel(const Chan &c, hipStream_t s);
+void launch_init_history(const Chan &c, hipStream_t s);
+void launch_init_mean(const Chan &c, hipStream_t s);
+
+}  // namespace owt

@@ -1,0 +1,524 @@
+// Dealiased horizontal transforms of the nonlinear term (reference
+// src/FFT.cpp:49-194 and the Lamb product src/solver.cpp:284-286).
+//
+// Every kernel works on tiles of TJ consecutive wall-normal points y, which is
+// the contiguous axis of all fields, so each global access is a run of TJ
+// complex values. A tile of lines is transformed in LDS by a Stockham
+// autosort FFT with radix-4/3/2 stages; each thread keeps its butterfly
+// inputs in registers between the read and write barriers of a stage.
+//
+//   z_inverse : wave [NZ][NXH][NY] -> zero-padded, z-Nyquist dropped,
+//               backward FFT of length NZ2 -> [NZ2][NXH][NY]
+//   x_lamb    : six z-padded fields -> three Hermitian-extended c2r transforms
+//               (two real fields per complex transform) -> u x omega in
+//               registers -> two forward transforms (lx + i ly, lz) ->
+//               kx < NXH kept, / NX2 -> three z-padded fields
+//   z_forward : forward FFT of length NZ2, low and high bands kept, / NZ2
+// The compute type T is double (plain method) or float (mixed precision).
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "device.hpp"
+
+namespace owt {
+
+template <typename T>
+struct CT;
+template <>
+struct CT<double> {
+    using t = double2;
+};
+template <>
+struct CT<float> {
+    using t = float2;
+};
+
+template <typename C>
+__device__ __forceinline__ C mk(decltype(C::x) a, decltype(C::x) b) {
+    C r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) { return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+
+template <typename C, int R>
+__device__ __forceinline__ void dft(C (&x)[R], int sign) {
+    using T = decltype(C::x);
+    if constexpr (R == 2) {
+        C a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    } else if constexpr (R == 3) {
+        const T sn = (T)sign * (T)0.86602540378443864676372317075294;
+        C t = cadd(x[1], x[2]), d = csub(x[1], x[2]);
+        C m = mk<C>(x[0].x - (T)0.5 * t.x, x[0].y - (T)0.5 * t.y);
+        x[0] = cadd(x[0], t);
+        x[1] = mk<C>(m.x - sn * d.y, m.y + sn * d.x);
+        x[2] = mk<C>(m.x + sn * d.y, m.y - sn * d.x);
+    } else {
+        C a = cadd(x[0], x[2]), b = csub(x[0], x[2]), c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
+        const T s = (T)sign;
+        x[0] = cadd(a, c);
+        x[2] = csub(a, c);
+        x[1] = mk<C>(b.x - s * d.y, b.y + s * d.x);
+        x[3] = mk<C>(b.x + s * d.y, b.y - s * d.x);
+    }
+}
+
+// One Stockham stage of radix R over TJ lines of length L held in buf[n*TJ + jj].
+// Ns = product of the radices of the earlier stages. tw[m] = e^{-2 pi i m/L}.
+template <typename C, int R, int TJ, int TL, int MAXB>
+__device__ __forceinline__ void stage(C *buf, const C *tw, int L, int Ns, int sign, int jj, int tl) {
+    const int Lr = L / R;
+    const int tstep = L / (Ns * R);
+    C out[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tl + b * TL;
+        if (j < Lr) {
+            const int k = j % Ns;
+            C x[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) x[q] = buf[(j + q * Lr) * TJ + jj];
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                C w = tw[q * k * tstep];
+                if (sign > 0) w.y = -w.y;
+                x[q] = cmul(x[q], w);
+            }
+            dft<C, R>(x, sign);
+#pragma unroll
+            for (int q = 0; q < R; ++q) out[b][q] = x[q];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tl + b * TL;
+        if (j < Lr) {
+            const int k = j % Ns;
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[(base + q * Ns) * TJ + jj] = out[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+// Full transform of the TJ lines in buf (natural order in and out).
+// Callers synchronise before the call; the call ends synchronised.
+template <typename C, int TJ, int TL, int LMAX>
+__device__ void fft_lines(C *buf, const C *tw, const FFTPlan &P, int sign, int jj, int tl) {
+    int Ns = 1;
+    for (int s = 0; s < P.nst; ++s) {
+        const int R = P.radix[s];
+        if (R == 4)
+            stage<C, 4, TJ, TL, (LMAX / 4 + TL - 1) / TL>(buf, tw, P.L, Ns, sign, jj, tl);
+        else if (R == 3)
+            stage<C, 3, TJ, TL, (LMAX / 3 + TL - 1) / TL>(buf, tw, P.L, Ns, sign, jj, tl);
+        else
+            stage<C, 2, TJ, TL, (LMAX / 2 + TL - 1) / TL>(buf, tw, P.L, Ns, sign, jj, tl);
+        Ns *= R;
+    }
+}
+
+template <typename C>
+__device__ __forceinline__ C load_c(const double2 *p) {
+    double2 v = *p;
+    return mk<C>((decltype(C::x))v.x, (decltype(C::x))v.y);
+}
+
+template <typename C>
+__device__ __forceinline__ const C *tw_of(const Chan &c, bool xdir);
+template <>
+__device__ __forceinline__ const double2 *tw_of<double2>(const Chan &c, bool xdir) { return xdir ? c.twx : c.twz; }
+template <>
+__device__ __forceinline__ const float2 *tw_of<float2>(const Chan &c, bool xdir) { return xdir ? c.twxf : c.twzf; }
+
+// ---------------------------------------------------------------------------
+// z inverse (src/FFT.cpp:68-95): per (kx index i, y tile, field)
+// ---------------------------------------------------------------------------
+// Input fields of the z inverse. With ring set, fields 1 and 4 are v and
+// omega_y in the history slot written by this step's solves, located from the
+// device-side ring counter so that one captured graph serves every step.
+struct PtrTable {
+    const double2 *p[6];
+    int ring;
+};
+
+template <typename T, int TJ, int TL, int LMAX>
+__global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typename CT<T>::t *out) {
+    using C = typename CT<T>::t;
+    extern __shared__ unsigned char smem[];
+    C *tw = reinterpret_cast<C *>(smem);
+    C *buf = tw + LMAX;
+    const int L = c.NZ2, NZ = c.NZ, NZH = c.NZH, NXH = c.NXH, NY = c.NY;
+    const int i = blockIdx.x, f = blockIdx.z;
+    const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
+    const int j = blockIdx.y * TJ + jj;
+    const bool valid = j < NY;
+    const C *twg = tw_of<C>(c, false);
+    for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = twg[n];
+    const double2 *src = in.p[f];
+    if (in.ring && (f == 1 || f == 4)) {
+        const int ns = (c.ctrl[C_RING] + 1) % 3;
+        src = c.hist + (size_t)((f == 1 ? H_V : H_G) * 3 + ns) * c.W;
+    }
+    for (int n = tl; n < L; n += TL) {
+        // slot n <- mode n (n < NZH); slots NZH..NZ are padding and the dropped z-Nyquist
+        const int mode = n < NZH ? n : (n <= NZ ? -1 : n - NZH);
+        C v = mk<C>(0, 0);
+        if (mode >= 0 && valid) v = load_c<C>(src + ((size_t)mode * NXH + i) * NY + j);
+        buf[n * TJ + jj] = v;
+    }
+    __syncthreads();
+    fft_lines<C, TJ, TL, LMAX>(buf, tw, c.pz, +1, jj, tl);
+    C *dst = out + (size_t)f * c.I;
+    if (valid)
+        for (int n = tl; n < L; n += TL) dst[((size_t)n * NXH + i) * NY + j] = buf[n * TJ + jj];
+}
+
+// ---------------------------------------------------------------------------
+// z forward (src/FFT.cpp:171-190): per (i, y tile, field)
+// ---------------------------------------------------------------------------
+template <typename T, int TJ, int TL, int LMAX, typename OutC>
+__global__ __launch_bounds__(TJ *TL) void k_z_forward(Chan c, const typename CT<T>::t *in, OutC *out) {
+    using C = typename CT<T>::t;
+    extern __shared__ unsigned char smem[];
+    C *tw = reinterpret_cast<C *>(smem);
+    C *buf = tw + LMAX;
+    const int L = c.NZ2, NZ = c.NZ, NZH = c.NZH, NXH = c.NXH, NY = c.NY;
+    const int i = blockIdx.x, f = blockIdx.z;
+    const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
+    const int j = blockIdx.y * TJ + jj;
+    const bool valid = j < NY;
+    const C *twg = tw_of<C>(c, false);
+    for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = twg[n];
+    const C *src = in + (size_t)f * c.I;
+    for (int n = tl; n < L; n += TL) buf[n * TJ + jj] = valid ? src[((size_t)n * NXH + i) * NY + j] : mk<C>(0, 0);
+    __syncthreads();
+    fft_lines<C, TJ, TL, LMAX>(buf, tw, c.pz, -1, jj, tl);
+    OutC *dst = out + (size_t)f * c.W;
+    const T sc = (T)L;
+    if (valid)
+        for (int k = tl; k < NZ; k += TL) {
+            // mode k <- bin k (k < NZH); mode k >= NZH <- bin k + NZH (negative frequencies)
+            const int bin = k < NZH ? k : k + NZH;
+            C v = buf[bin * TJ + jj];
+            OutC o;
+            o.x = v.x / sc;
+            o.y = v.y / sc;
+            dst[((size_t)k * NXH + i) * NY + j] = o;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Fused x phase (src/FFT.cpp:101-127, src/solver.cpp:284-286, src/FFT.cpp:147-164)
+// per (z plane k2, y tile). Each thread owns the x positions n = tl + e*TL.
+// ---------------------------------------------------------------------------
+template <typename C>
+__device__ __forceinline__ void load_pair_spectrum(C *buf, const C *A, const C *B, int L, int NXH, size_t stride_i,
+                                                   bool valid, int jj, int tl, int TJ, int TL) {
+    // Z = Aext + i Bext with Hermitian extensions; the kx = 0 entries enter
+    // through their real parts (the reference keeps Re of the backward transform)
+    for (int m = tl; m < NXH; m += TL) {
+        C a = mk<C>(0, 0), b = mk<C>(0, 0);
+        if (valid) {
+            a = A[(size_t)m * stride_i];
+            b = B[(size_t)m * stride_i];
+        }
+        if (m == 0) {
+            buf[jj] = mk<C>(a.x, b.x);
+        } else {
+            buf[m * TJ + jj] = mk<C>(a.x - b.y, a.y + b.x);
+            buf[(L - m) * TJ + jj] = mk<C>(a.x + b.y, b.x - a.y);
+        }
+    }
+    for (int n = NXH + tl; n <= L - NXH; n += TL) buf[n * TJ + jj] = mk<C>(0, 0);
+}
+
+template <typename T, int TJ, int TL, int LMAX>
+__global__ __launch_bounds__(TJ *TL) void k_x_lamb(Chan c, const typename CT<T>::t *in, typename CT<T>::t *out) {
+    using C = typename CT<T>::t;
+    constexpr int EPT = (LMAX + TL - 1) / TL;
+    extern __shared__ unsigned char smem[];
+    C *tw = reinterpret_cast<C *>(smem);
+    C *buf = tw + LMAX;
+    const int L = c.NX2, NXH = c.NXH, NY = c.NY;
+    const int k2 = blockIdx.x;
+    const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
+    const int j = blockIdx.y * TJ + jj;
+    const bool valid = j < NY;
+    const C *twg = tw_of<C>(c, true);
+    for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = twg[n];
+    const size_t plane = (size_t)k2 * NXH * NY + j;
+    C a[3][EPT];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const C *A = in + (size_t)(2 * q) * c.I + plane;
+        const C *B = in + (size_t)(2 * q + 1) * c.I + plane;
+        load_pair_spectrum<C>(buf, A, B, L, NXH, (size_t)NY, valid, jj, tl, TJ, TL);
+        __syncthreads();
+        fft_lines<C, TJ, TL, LMAX>(buf, tw, c.px, +1, jj, tl);
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int n = tl + e * TL;
+            if (n < L) a[q][e] = buf[n * TJ + jj];
+        }
+        __syncthreads();
+    }
+    // Lamb vector u x omega (src/solver.cpp:284-286):
+    // pair 0 = (u, v), pair 1 = (w, omx), pair 2 = (omy, omz)
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const T u = a[0][e].x, v = a[0][e].y, w = a[1][e].x, ox = a[1][e].y, oy = a[2][e].x, oz = a[2][e].y;
+        const T lx = v * oz - w * oy;
+        const T ly = w * ox - u * oz;
+        const T lz = u * oy - v * ox;
+        a[0][e] = mk<C>(lx, ly);
+        a[1][e] = mk<C>(lz, (T)0);
+    }
+    const T sc = (T)L;
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int n = tl + e * TL;
+            if (n < L) buf[n * TJ + jj] = a[pass][e];
+        }
+        __syncthreads();
+        fft_lines<C, TJ, TL, LMAX>(buf, tw, c.px, -1, jj, tl);
+        if (valid) {
+            for (int n = tl; n < NXH; n += TL) {
+                const C zn = buf[n * TJ + jj];
+                if (pass == 0) {
+                    const C zm = buf[((L - n) % L) * TJ + jj];
+                    // spectra of the two real sequences packed as lx + i ly
+                    C X = mk<C>((zn.x + zm.x) * (T)0.5 / sc, (zn.y - zm.y) * (T)0.5 / sc);
+                    C Y = mk<C>((zn.y + zm.y) * (T)0.5 / sc, (zm.x - zn.x) * (T)0.5 / sc);
+                    out[0 * c.I + plane + (size_t)n * NY] = X;
+                    out[1 * c.I + plane + (size_t)n * NY] = Y;
+                } else {
+                    out[2 * c.I + plane + (size_t)n * NY] = mk<C>(zn.x / sc, zn.y / sc);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// physical-space utilities (FP64): c2r of nf z-padded fields -> [NZ2][NX2][NY]
+template <int TJ, int TL, int LMAX>
+__global__ __launch_bounds__(TJ *TL) void k_x_c2r(Chan c, const double2 *in, int nf, double *phys) {
+    using C = double2;
+    extern __shared__ unsigned char smem[];
+    C *tw = reinterpret_cast<C *>(smem);
+    C *buf = tw + LMAX;
+    const int L = c.NX2, NXH = c.NXH, NY = c.NY;
+    const int k2 = blockIdx.x, pr = blockIdx.z;
+    const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
+    const int j = blockIdx.y * TJ + jj;
+    const bool valid = j < NY;
+    for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = c.twx[n];
+    const size_t plane = (size_t)k2 * NXH * NY + j;
+    const int f0 = 2 * pr, f1 = 2 * pr + 1;
+    const bool has2 = f1 < nf;
+    // the second field of an odd count is taken as zero: reuse the first pointer and zero it
+    const C *A = in + (size_t)f0 * c.I + plane;
+    const C *B = in + (size_t)(has2 ? f1 : f0) * c.I + plane;
+    for (int m = tl; m < NXH; m += TL) {
+        C a = mk<C>(0, 0), b = mk<C>(0, 0);
+        if (valid) {
+            a = A[(size_t)m * NY];
+            if (has2) b = B[(size_t)m * NY];
+        }
+        if (m == 0) buf[jj] = mk<C>(a.x, b.x);
+        else {
+            buf[m * TJ + jj] = mk<C>(a.x - b.y, a.y + b.x);
+            buf[(L - m) * TJ + jj] = mk<C>(a.x + b.y, b.x - a.y);
+        }
+    }
+    for (int n = NXH + tl; n <= L - NXH; n += TL) buf[n * TJ + jj] = mk<C>(0, 0);
+    __syncthreads();
+    fft_lines<C, TJ, TL, LMAX>(buf, tw, c.px, +1, jj, tl);
+    const size_t P = (size_t)c.NZ2 * L * NY;
+    if (valid)
+        for (int n = tl; n < L; n += TL) {
+            const C z = buf[n * TJ + jj];
+            const size_t o = ((size_t)k2 * L + n) * NY + j;
+            phys[(size_t)f0 * P + o] = z.x;
+            if (has2) phys[(size_t)f1 * P + o] = z.y;
+        }
+}
+
+// r2c of nf physical fields -> nf z-padded fields (kx < NXH, / NX2)
+template <int TJ, int TL, int LMAX>
+__global__ __launch_bounds__(TJ *TL) void k_x_r2c(Chan c, const double *phys, int nf, double2 *out) {
+    using C = double2;
+    extern __shared__ unsigned char smem[];
+    C *tw = reinterpret_cast<C *>(smem);
+    C *buf = tw + LMAX;
+    const int L = c.NX2, NXH = c.NXH, NY = c.NY;
+    const int k2 = blockIdx.x, pr = blockIdx.z;
+    const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
+    const int j = blockIdx.y * TJ + jj;
+    const bool valid = j < NY;
+    for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = c.twx[n];
+    const int f0 = 2 * pr, f1 = 2 * pr + 1;
+    const bool has2 = f1 < nf;
+    const size_t P = (size_t)c.NZ2 * L * NY;
+    for (int n = tl; n < L; n += TL) {
+        double x = 0, y = 0;
+        if (valid) {
+            const size_t o = ((size_t)k2 * L + n) * NY + j;
+            x = phys[(size_t)f0 * P + o];
+            if (has2) y = phys[(size_t)f1 * P + o];
+        }
+        buf[n * TJ + jj] = mk<C>(x, y);
+    }
+    __syncthreads();
+    fft_lines<C, TJ, TL, LMAX>(buf, tw, c.px, -1, jj, tl);
+    const size_t plane = (size_t)k2 * NXH * NY + j;
+    const double sc = (double)L;
+    if (valid)
+        for (int n = tl; n < NXH; n += TL) {
+            const C zn = buf[n * TJ + jj], zm = buf[((L - n) % L) * TJ + jj];
+            if (has2) {
+                out[(size_t)f0 * c.I + plane + (size_t)n * NY] = mk<C>((zn.x + zm.x) * 0.5 / sc, (zn.y - zm.y) * 0.5 / sc);
+                out[(size_t)f1 * c.I + plane + (size_t)n * NY] = mk<C>((zn.y + zm.y) * 0.5 / sc, (zm.x - zn.x) * 0.5 / sc);
+            } else {
+                out[(size_t)f0 * c.I + plane + (size_t)n * NY] = mk<C>(zn.x / sc, zn.y / sc);
+            }
+        }
+}
+
+__global__ void k_peak3(const double *p, size_t n, unsigned long long *out) {
+    double m = 0.0;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const double u = p[e], v = p[n + e], w = p[2 * n + e];
+        const double s = u * u + v * v + w * w;
+        m = s > m ? s : m;
+    }
+    // non-negative doubles order like their bit patterns
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers: tile shape chosen by the padded length
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T, int TJ, int TL, int LMAX>
+size_t smem_bytes() {
+    return sizeof(typename CT<T>::t) * (size_t)LMAX * (TJ + 1);
+}
+
+#define OWT_LEN_DISPATCH(L, BODY)                                                   \
+    do {                                                                            \
+        if ((L) <= 96) {                                                            \
+            constexpr int TJ = 16, TL = 16, LMAX = 96;                              \
+            BODY;                                                                   \
+        } else if ((L) <= 192) {                                                    \
+            constexpr int TJ = 8, TL = 32, LMAX = 192;                              \
+            BODY;                                                                   \
+        } else if ((L) <= 576) {                                                    \
+            constexpr int TJ = 4, TL = 64, LMAX = 576;                              \
+            BODY;                                                                   \
+        } else {                                                                    \
+            constexpr int TJ = 2, TL = 128, LMAX = 1152;                            \
+            BODY;                                                                   \
+        }                                                                           \
+    } while (0)
+
+}  // namespace
+
+void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out, hipStream_t s) {
+    PtrTable t{};
+    if (!in) {
+        // step mode: u, v, w, omega_x, omega_y, omega_z of the current step
+        nf = 6;
+        t.ring = 1;
+        t.p[0] = c.cur;
+        t.p[2] = c.cur + c.W;
+        t.p[3] = c.cur + 2 * c.W;
+        t.p[5] = c.cur + 3 * c.W;
+    } else {
+        for (int f = 0; f < nf; ++f) t.p[f] = in[f];
+    }
+    OWT_LEN_DISPATCH(c.NZ2, {
+        dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, nf);
+        if (c.precision == 0)
+            hipLaunchKernelGGL((k_z_inverse<double, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, t,
+                               (double2 *)out);
+        else
+            hipLaunchKernelGGL((k_z_inverse<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c, t,
+                               (float2 *)out);
+    });
+}
+
+void launch_x_lamb(const Chan &c, hipStream_t s) {
+    OWT_LEN_DISPATCH(c.NX2, {
+        dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ);
+        if (c.precision == 0)
+            hipLaunchKernelGGL((k_x_lamb<double, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c,
+                               (const double2 *)c.zbuf, (double2 *)c.xbuf);
+        else
+            hipLaunchKernelGGL((k_x_lamb<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
+                               (const float2 *)c.zbuf, (float2 *)c.xbuf);
+    });
+}
+
+void launch_z_forward(const Chan &c, hipStream_t s) {
+    OWT_LEN_DISPATCH(c.NZ2, {
+        dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, 3);
+        if (c.precision == 0)
+            hipLaunchKernelGGL((k_z_forward<double, TJ, TL, LMAX, double2>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s,
+                               c, (const double2 *)c.xbuf, (double2 *)c.lmb);
+        else
+            hipLaunchKernelGGL((k_z_forward<float, TJ, TL, LMAX, float2>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
+                               (const float2 *)c.xbuf, (float2 *)c.lmb);
+    });
+}
+
+void launch_z_forward_to(const Chan &c, const void *in, int nf, double2 *out, hipStream_t s) {
+    Chan d = c;
+    d.precision = 0;
+    OWT_LEN_DISPATCH(c.NZ2, {
+        dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, nf);
+        hipLaunchKernelGGL((k_z_forward<double, TJ, TL, LMAX, double2>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, d,
+                           (const double2 *)in, out);
+    });
+}
+
+void launch_x_c2r(const Chan &c, const void *in, int nf, double *phys, hipStream_t s) {
+    OWT_LEN_DISPATCH(c.NX2, {
+        dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ, (nf + 1) / 2);
+        hipLaunchKernelGGL((k_x_c2r<TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c,
+                           (const double2 *)in, nf, phys);
+    });
+}
+
+void launch_x_r2c(const Chan &c, const double *phys, int nf, void *out, hipStream_t s) {
+    OWT_LEN_DISPATCH(c.NX2, {
+        dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ, (nf + 1) / 2);
+        hipLaunchKernelGGL((k_x_r2c<TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, phys, nf,
+                           (double2 *)out);
+    });
+}
+
+void launch_peak3(const double *phys, size_t n, unsigned long long *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_peak3, dim3(1024), dim3(256), 0, s, phys, n, out);
+}
+
+}  // namespace owt

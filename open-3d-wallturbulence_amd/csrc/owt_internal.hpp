@@ -1,0 +1,57 @@
+// Internal declarations shared by the host side and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace owt {
+
+// Raised by host code; the C ABI turns it into a status code + message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define OWT_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::owt::Error(3, std::string("HIP error: ") + hipGetErrorString(e_) + " at " + \
+                                      __FILE__ + ":" + std::to_string(__LINE__));              \
+    } while (0)
+
+// Reference toml keys (src/IO.cpp:14-25) and derived sizes (src/IO.cpp:27-34).
+struct Params {
+    double nu = 0, dt = 0, dkx = 0, dkz = 0, alpha_mesh = 0;
+    long NX = 0, NY = 0, NZ = 0, NT = 0, PX = 1, PZ = 1, save_interval = 1;
+    long NXH = 0, NZH = 0, NX2 = 0, NZ2 = 0;
+    // extension keys (namespaced in the toml as owt.*)
+    int precision = 0;  // 0 = FP64 everywhere, 1 = FP32 transform phase
+    int method = 1;     // 0 = plain (per-step LU, eager), 1 = batched (factor once, graph)
+    uint64_t seed = 0;
+    double amp = 0.15;
+    int init_mode = 1;  // 1 = reference laminar + wave, 2 = seeded recipe
+};
+
+Params load_params(const std::string &path);
+void derive_sizes(Params &p);
+
+// Host-side wall-normal discretisation (src/solver.cpp:15-216, src/compdiff.hpp).
+struct Operators {
+    int N = 0;
+    double ds = 0;
+    std::vector<double> s, y, dyds1, dyds2, dyds3, dyds4;
+    std::vector<double> c11, c21, c22, c31, c32, c33, c41, c42, c43, c44;
+    // column-major N x N
+    std::vector<double> A[4], B[4], D[4], DY1, DY2, DY3, DY4;
+};
+Operators build_operators(int NY, double alpha_mesh);
+
+// host dense LU with partial pivoting (dgetrf semantics), column-major
+int host_lu_factor(int n, double *A, int *ipiv);
+void host_lu_solve(int n, const double *LU, const int *ipiv, double *b);
+
+}  // namespace owt

@@ -1,0 +1,1004 @@
+// owt_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the Open-3D-WallTurbulence time step, used as the parity
+// oracle for the HIP implementation. Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load it; the product never links or calls it.
+//
+// Each function cites the reference file:line it restates. Flat arrays replace
+// the reference's nested containers; the arithmetic (operation order, reciprocal
+// versus division, pivoting rule) follows the reference so that the oracle
+// reproduces what the reference computes on the same inputs:
+//   * dense operators D_n = A_n^{-1} B_n and DY_n          src/solver.cpp:171-216
+//   * compact schemes incl. the B2/B3 top-row overwrites   src/compdiff.hpp:11-357
+//   * LU with partial pivoting (dgetrf/dgetrs semantics)   src/linalg.hpp:493-540, 615-689
+//   * mat-vec in column-axpy order                         src/linalg.hpp:434-453
+//   * 3/2-dealiased transforms                             src/FFT.cpp:49-194
+//   * time step, mean flow, friction, Lamb vector          src/solver.cpp:240-627
+// FFTW is replaced by a plain recursive mixed-radix DFT (same sign and scaling
+// conventions: backward unnormalised e^{+i}, forward e^{-i}).
+//
+// Pinning (what checks this file, in tests/): brute-force DFT; the dealiasing
+// identity against a direct spectral convolution; LU solve against LAPACK
+// (numpy.linalg.solve = dgetrf+dgetrs); polynomial exactness of the compact
+// rows; the B3 top-row signature; derivatives of analytic functions on the
+// stretched grid; laminar Poiseuille steadiness and fx_fric = 3 nu; Hermitian
+// and divergence invariants; third-order time accuracy.
+#include <cfloat>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using cd = std::complex<double>;
+
+namespace {
+
+// ----------------------------------------------------------------------------
+// Dense column-major matrix (src/linalg.hpp:223-236)
+// ----------------------------------------------------------------------------
+struct Mat {
+    int n = 0;
+    std::vector<double> a;
+    Mat() {}
+    explicit Mat(int n_) : n(n_), a((size_t)n_ * n_, 0.0) {}
+    double &operator()(int i, int j) { return a[(size_t)i + (size_t)j * n]; }
+    double operator()(int i, int j) const { return a[(size_t)i + (size_t)j * n]; }
+};
+
+// LU with partial pivoting, LAPACK dgetrf/dgetrf2 semantics (src/linalg.hpp:508-520):
+// pivot = first index of max |a| (idamax), full-row interchanges, multipliers
+// scaled by the reciprocal of the pivot when |pivot| >= DBL_MIN, rank-1 updates
+// accumulated in ascending k for every element (the order reference BLAS dgemm
+// and dtrsm produce). Returns LAPACK info (0 = ok, k+1 = exact zero pivot).
+int lu_factor(int n, double *A, int *ipiv) {
+    int info = 0;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double amax = std::fabs(A[k + (size_t)k * n]);
+        for (int i = k + 1; i < n; ++i) {
+            double v = std::fabs(A[i + (size_t)k * n]);
+            if (v > amax) { amax = v; p = i; }
+        }
+        ipiv[k] = p;
+        if (A[p + (size_t)k * n] == 0.0) {
+            if (info == 0) info = k + 1;
+            continue;
+        }
+        if (p != k)
+            for (int j = 0; j < n; ++j) std::swap(A[k + (size_t)j * n], A[p + (size_t)j * n]);
+        double piv = A[k + (size_t)k * n];
+        if (std::fabs(piv) >= DBL_MIN) {
+            double r = 1.0 / piv;
+            for (int i = k + 1; i < n; ++i) A[i + (size_t)k * n] *= r;
+        } else {
+            for (int i = k + 1; i < n; ++i) A[i + (size_t)k * n] /= piv;
+        }
+        for (int j = k + 1; j < n; ++j) {
+            double t = A[k + (size_t)j * n];
+            if (t == 0.0) continue;
+            double *cj = A + (size_t)j * n;
+            const double *ck = A + (size_t)k * n;
+            for (int i = k + 1; i < n; ++i) cj[i] -= ck[i] * t;
+        }
+    }
+    return info;
+}
+
+// dgetrs 'N' on one real right-hand side (src/linalg.hpp:522-532, 652-662):
+// dlaswp, then unit-lower and non-unit-upper dtrsm in column order.
+void lu_solve(int n, const double *LU, const int *ipiv, double *b) {
+    for (int k = 0; k < n; ++k)
+        if (ipiv[k] != k) std::swap(b[k], b[ipiv[k]]);
+    for (int k = 0; k < n; ++k) {
+        double bk = b[k];
+        if (bk == 0.0) continue;
+        const double *ck = LU + (size_t)k * n;
+        for (int i = k + 1; i < n; ++i) b[i] -= bk * ck[i];
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        if (b[k] == 0.0) continue;
+        b[k] = b[k] / LU[k + (size_t)k * n];
+        double bk = b[k];
+        const double *ck = LU + (size_t)k * n;
+        for (int i = 0; i < k; ++i) b[i] -= bk * ck[i];
+    }
+}
+
+// Mat::solve(Vec<Complex>) (src/linalg.hpp:615-689): dgetrf on a copy, zgetrs
+// with the real factors promoted to complex; with a zero imaginary part in the
+// factors the complex solve is two independent real solves.
+void dense_solve_complex(const Mat &A, cd *x) {
+    int n = A.n;
+    std::vector<double> LU(A.a);
+    std::vector<int> ipiv(n);
+    if (lu_factor(n, LU.data(), ipiv.data()) != 0)
+        throw std::runtime_error("LU factorization failed. Matrix A may be singular.");
+    std::vector<double> re(n), im(n);
+    for (int i = 0; i < n; ++i) { re[i] = x[i].real(); im[i] = x[i].imag(); }
+    lu_solve(n, LU.data(), ipiv.data(), re.data());
+    lu_solve(n, LU.data(), ipiv.data(), im.data());
+    for (int i = 0; i < n; ++i) x[i] = cd(re[i], im[i]);
+}
+
+// Mat::solve(Mat) (src/linalg.hpp:493-540): A^{-1} B column by column.
+Mat dense_solve_mat(const Mat &A, const Mat &B) {
+    int n = A.n;
+    std::vector<double> LU(A.a);
+    std::vector<int> ipiv(n);
+    if (lu_factor(n, LU.data(), ipiv.data()) != 0)
+        throw std::runtime_error("LU factorization failed. Matrix A may be singular.");
+    Mat X = B;
+    for (int j = 0; j < n; ++j) lu_solve(n, LU.data(), ipiv.data(), X.a.data() + (size_t)j * n);
+    return X;
+}
+
+// Mat * Vec<Complex> in column-axpy order (src/linalg.hpp:434-453).
+void matvec(const Mat &M, const cd *x, cd *y) {
+    int n = M.n;
+    for (int i = 0; i < n; ++i) y[i] = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double xr = x[j].real(), xi = x[j].imag();
+        const double *c = M.a.data() + (size_t)j * n;
+        for (int i = 0; i < n; ++i) y[i] = cd(y[i].real() + c[i] * xr, y[i].imag() + c[i] * xi);
+    }
+}
+
+// Vec<double> * Mat: row scaling (src/linalg.hpp:455-470).
+Mat rowscale(const std::vector<double> &v, const Mat &m) {
+    Mat r(m.n);
+    for (int j = 0; j < m.n; ++j)
+        for (int i = 0; i < m.n; ++i) r(i, j) = v[i] * m(i, j);
+    return r;
+}
+Mat madd(const Mat &a, const Mat &b) {
+    Mat r(a.n);
+    for (size_t e = 0; e < a.a.size(); ++e) r.a[e] = a.a[e] + b.a[e];
+    return r;
+}
+
+// ----------------------------------------------------------------------------
+// Compact finite-difference schemes (src/compdiff.hpp:11-357), including the
+// B2 and B3 top-row overwrites at :164-165 and :257-258.
+// ----------------------------------------------------------------------------
+void op_A1B1(int N, double h, Mat &A, Mat &B) {
+    const double al2 = 4.0 / 9.0, be2 = 1.0 / 36.0, a2 = 40.0 / 27.0, b2 = 25.0 / 54.0;
+    for (int i = 2; i < N - 2; ++i) {
+        A(i, i - 2) = be2; A(i, i - 1) = al2; A(i, i) = 1.0; A(i, i + 1) = al2; A(i, i + 2) = be2;
+        B(i, i - 2) = -b2 / (4 * h); B(i, i - 1) = -a2 / (2 * h);
+        B(i, i + 1) = a2 / (2 * h); B(i, i + 2) = b2 / (4 * h);
+    }
+    const double al1 = 0.25, a1 = 1.5;
+    for (int i : {1, N - 2}) {
+        A(i, i - 1) = al1; A(i, i) = 1.0; A(i, i + 1) = al1;
+        B(i, i - 1) = -a1 / (2 * h); B(i, i + 1) = a1 / (2 * h);
+    }
+    const double al0 = 3.0, a0 = -17.0 / 6.0, b0 = 1.5, c0 = 1.5, d0 = -1.0 / 6.0;
+    A(0, 0) = 1.0; A(0, 1) = al0;
+    A(N - 1, N - 1) = 1.0; A(N - 1, N - 2) = al0;
+    B(0, 0) = a0 / h; B(0, 1) = b0 / h; B(0, 2) = c0 / h; B(0, 3) = d0 / h;
+    B(N - 1, N - 1) = -a0 / h; B(N - 1, N - 2) = -b0 / h; B(N - 1, N - 3) = -c0 / h; B(N - 1, N - 4) = -d0 / h;
+}
+
+void op_A2B2(int N, double h, Mat &A, Mat &B) {
+    double h2 = std::pow(h, 2);
+    const double al2 = 344.0 / 1179.0, be2 = 23.0 / 2358.0, a2 = 320.0 / 393.0, b2 = 310.0 / 393.0;
+    for (int i = 2; i < N - 2; ++i) {
+        A(i, i - 2) = be2; A(i, i - 1) = al2; A(i, i) = 1.0; A(i, i + 1) = al2; A(i, i + 2) = be2;
+        B(i, i - 2) = b2 / (4 * h2); B(i, i - 1) = a2 / h2;
+        B(i, i) = -2 * b2 / (4 * h2) - 2 * a2 / h2;
+        B(i, i + 1) = a2 / h2; B(i, i + 2) = b2 / (4 * h2);
+    }
+    // near-wall rows centred one node inward (src/compdiff.hpp:113-137)
+    const double al1 = 2.0 / 11.0, a1 = 12.0 / 11.0, b1 = 3.0 / 11.0;
+    for (int i : {1, N - 2}) {
+        int c = (i == 1) ? i + 1 : i - 1;
+        A(i, c - 1) = al1; A(i, c) = 1.0; A(i, c + 1) = al1;
+        B(i, c - 2) = b1 / (4 * h2); B(i, c - 1) = a1 / h2;
+        B(i, c) = -2 * b1 / (4 * h2) - 2 * a1 / h2;
+        B(i, c + 1) = a1 / h2; B(i, c + 2) = b1 / (4 * h2);
+    }
+    const double al0 = 10.0, be0 = 1.0, a0 = 12.0, b0 = -24.0, c0 = 12.0, d0 = 0.0, e0 = 0.0;
+    A(0, 0) = 1.0; A(0, 1) = al0; A(0, 2) = be0;
+    A(N - 1, N - 1) = 1.0; A(N - 1, N - 2) = al0; A(N - 1, N - 3) = be0;
+    B(0, 0) = a0 / h2; B(0, 1) = b0 / h2; B(0, 2) = c0 / h2; B(0, 3) = d0 / h2; B(0, 4) = e0 / h2;
+    B(N - 1, N - 1) = a0 / h2; B(N - 1, N - 2) = b0 / h2; B(N - 1, N - 3) = c0 / h2;
+    B(N - 1, N - 4) = d0 / h2;
+    B(N - 1, N - 4) = e0 / h2;  // overwrite as in src/compdiff.hpp:165 (both zero)
+}
+
+void op_A3B3(int N, double h, Mat &A, Mat &B) {
+    double h3 = std::pow(h, 3);
+    const double al3 = 147.0 / 332.0, be3 = 1.0 / 166.0, a3 = 160.0 / 83.0, b3 = -5.0 / 166.0;
+    for (int i = 3; i < N - 3; ++i) {
+        A(i, i - 2) = be3; A(i, i - 1) = al3; A(i, i) = 1.0; A(i, i + 1) = al3; A(i, i + 2) = be3;
+        B(i, i - 3) = -b3 / (8 * h3); B(i, i - 2) = -a3 / (2 * h3);
+        B(i, i - 1) = a3 / h3 + 3 * b3 / (8 * h3);
+        B(i, i + 1) = -a3 / h3 - 3 * b3 / (8 * h3);
+        B(i, i + 2) = a3 / (2 * h3); B(i, i + 3) = b3 / (8 * h3);
+    }
+    const double al2 = 4.0 / 9.0, be2 = 1.0 / 126.0, a2 = 40.0 / 21.0;
+    for (int i : {2, N - 3}) {
+        A(i, i - 2) = be2; A(i, i - 1) = al2; A(i, i) = 1.0; A(i, i + 1) = al2; A(i, i + 2) = be2;
+        B(i, i - 2) = -a2 / (2 * h3); B(i, i - 1) = a2 / h3; B(i, i + 1) = -a2 / h3; B(i, i + 2) = a2 / (2 * h3);
+    }
+    const double al1 = 0.5, a1 = 2.0;
+    A(1, 1) = al1; A(1, 2) = 1.0; A(1, 3) = al1;
+    A(N - 2, N - 4) = al1; A(N - 2, N - 3) = 1.0; A(N - 2, N - 2) = al1;
+    B(1, 0) = -a1 / (2 * h3); B(1, 1) = a1 / h3; B(1, 3) = -a1 / h3; B(1, 4) = a1 / (2 * h3);
+    B(N - 2, N - 5) = -a1 / (2 * h3); B(N - 2, N - 4) = a1 / h3; B(N - 2, N - 2) = -a1 / h3; B(N - 2, N - 1) = a1 / (2 * h3);
+    const double al0 = -7.0, a0 = 8.0, b0 = -26.0, c0 = 30.0, d0 = -14.0, e0 = 2.0;
+    A(0, 0) = 1.0; A(0, 1) = al0;
+    A(N - 1, N - 1) = 1.0; A(N - 1, N - 2) = al0;
+    B(0, 0) = a0 / h3; B(0, 1) = b0 / h3; B(0, 2) = c0 / h3; B(0, 3) = d0 / h3; B(0, 4) = e0 / h3;
+    B(N - 1, N - 1) = -a0 / h3; B(N - 1, N - 2) = -b0 / h3; B(N - 1, N - 3) = -c0 / h3;
+    B(N - 1, N - 4) = -d0 / h3;
+    B(N - 1, N - 4) = -e0 / h3;  // overwrite as in src/compdiff.hpp:258; (N-1, N-5) stays 0
+}
+
+void op_A4B4(int N, double h, Mat &A, Mat &B) {
+    double h4 = std::pow(h, 4);
+    const double al3 = 7.0 / 26.0, a3 = 19.0 / 13.0, b3 = 1.0 / 13.0;
+    for (int i = 3; i < N - 3; ++i) {
+        A(i, i - 1) = al3; A(i, i) = 1.0; A(i, i + 1) = al3;
+        B(i, i - 3) = b3 / (6 * h4); B(i, i - 2) = a3 / h4;
+        B(i, i - 1) = -4 * a3 / h4 - 9 * b3 / (6 * h4);
+        B(i, i) = 6 * a3 / h4 + 16 * b3 / (6 * h4);
+        B(i, i + 1) = -4 * a3 / h4 - 9 * b3 / (6 * h4);
+        B(i, i + 2) = a3 / h4; B(i, i + 3) = b3 / (6 * h4);
+    }
+    const double al2 = 0.25, a2 = 1.5;
+    for (int i : {2, N - 3}) {
+        A(i, i - 1) = al2; A(i, i) = 1.0; A(i, i + 1) = al2;
+        B(i, i - 2) = a2 / h4; B(i, i - 1) = -4 * a2 / h4; B(i, i) = 6 * a2 / h4;
+        B(i, i + 1) = -4 * a2 / h4; B(i, i + 2) = a2 / h4;
+    }
+    const double al1 = -2.0, be1 = 7.0, a1 = 6.0, b1 = -24.0, c1 = 36.0, d1 = -24.0, e1 = 6.0;
+    A(1, 0) = 1.0; A(1, 1) = al1; A(1, 2) = be1;
+    B(1, 0) = a1 / h4; B(1, 1) = b1 / h4; B(1, 2) = c1 / h4; B(1, 3) = d1 / h4; B(1, 4) = e1 / h4;
+    A(N - 2, N - 1) = 1.0; A(N - 2, N - 2) = al1; A(N - 2, N - 3) = be1;
+    B(N - 2, N - 1) = a1 / h4; B(N - 2, N - 2) = b1 / h4; B(N - 2, N - 3) = c1 / h4;
+    B(N - 2, N - 4) = d1 / h4; B(N - 2, N - 5) = e1 / h4;
+    const double al0 = -2.0, a0 = -1.0, b0 = 4.0, c0 = -6.0, d0 = 4.0, e0 = -1.0;
+    A(0, 0) = 1.0; A(0, 1) = al0;
+    B(0, 0) = a0 / h4; B(0, 1) = b0 / h4; B(0, 2) = c0 / h4; B(0, 3) = d0 / h4; B(0, 4) = e0 / h4;
+    A(N - 1, N - 1) = 1.0; A(N - 1, N - 2) = al0;
+    B(N - 1, N - 1) = a0 / h4; B(N - 1, N - 2) = b0 / h4; B(N - 1, N - 3) = c0 / h4;
+    B(N - 1, N - 4) = d0 / h4; B(N - 1, N - 5) = e0 / h4;
+}
+
+// ----------------------------------------------------------------------------
+// 1-D complex DFT, recursive mixed radix (stands in for FFTW; src/FFT.cpp:24-27).
+// sign = +1: backward (e^{+2 pi i jk/n}), sign = -1: forward. Unnormalised.
+// ----------------------------------------------------------------------------
+struct FFTPlan {
+    int n = 0;
+    std::vector<cd> w_fwd, w_bwd;  // e^{-+2 pi i k/n}
+    explicit FFTPlan(int n_ = 0) : n(n_), w_fwd(n_), w_bwd(n_) {
+        for (int k = 0; k < n_; ++k) {
+            long double th = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n_;
+            double c = (double)cosl(th), s = (double)sinl(th);
+            w_fwd[k] = cd(c, -s);
+            w_bwd[k] = cd(c, s);
+        }
+    }
+};
+
+void fft_rec(int n, const cd *x, int stride, cd *out, const std::vector<cd> &W, int N0, cd *tmp) {
+    if (n == 1) { out[0] = x[0]; return; }
+    int p = (n % 4 == 0) ? 4 : (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : n;
+    int m = n / p;
+    if (p == n) {  // plain DFT for a prime length
+        for (int k = 0; k < n; ++k) {
+            cd s = 0.0;
+            for (int j = 0; j < n; ++j) s += x[(size_t)j * stride] * W[(size_t)((long)j * k % n) * (N0 / n)];
+            out[k] = s;
+        }
+        return;
+    }
+    // sub-transforms of the p decimated sequences, stored contiguously in tmp
+    for (int r = 0; r < p; ++r) fft_rec(m, x + (size_t)r * stride, stride * p, tmp + (size_t)r * m, W, N0, out);
+    int step = N0 / n;
+    for (int k = 0; k < m; ++k) {
+        cd y[4];
+        for (int r = 0; r < p; ++r) y[r] = tmp[(size_t)r * m + k] * W[(size_t)r * k * step];
+        for (int q = 0; q < p; ++q) {
+            cd s = 0.0;
+            for (int r = 0; r < p; ++r) s += y[r] * W[(size_t)((r * q) % p) * (N0 / p)];
+            out[k + q * m] = s;
+        }
+    }
+}
+
+void fft(const FFTPlan &P, int sign, const cd *in, cd *out) {
+    std::vector<cd> tmp(P.n);
+    fft_rec(P.n, in, 1, out, sign > 0 ? P.w_bwd : P.w_fwd, P.n, tmp.data());
+}
+
+// SplitMix64 and the seeded Gaussian draws of the seeded initial condition
+// (a recipe defined by this build; the reference has no seeded start).
+uint64_t splitmix(uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+cd seeded_gauss(uint64_t seed, int field, int i, int m, int p) {
+    uint64_t h = splitmix(seed ^ 0x5851F42D4C957F2DULL);
+    h = splitmix(h ^ (uint64_t)(int64_t)field);
+    h = splitmix(h ^ (uint64_t)(int64_t)i);
+    h = splitmix(h ^ (uint64_t)(int64_t)m);
+    h = splitmix(h ^ (uint64_t)(int64_t)p);
+    uint64_t h2 = splitmix(h);
+    double u1 = (double)((h >> 11) + 1) * 0x1.0p-53;
+    double u2 = (double)(h2 >> 11) * 0x1.0p-53;
+    double r = std::sqrt(-2.0 * std::log(u1));
+    double th = 6.283185307179586476925286766559 * u2;
+    return cd(r * std::cos(th), r * std::sin(th));
+}
+
+// ----------------------------------------------------------------------------
+// Solver state: src/setup.hpp:21-83 with PX = PZ = 1.
+// ----------------------------------------------------------------------------
+struct Oracle {
+    double nu, dt, dkx, dkz, alpha_mesh;
+    int NX, NY, NZ, NXH, NZH, NX2, NZ2;
+    size_t W;  // complex elements per wave field, [NZ][NXH][NY]
+    std::vector<double> kx, kz, s, y, dyds1, dyds2, dyds3, dyds4;
+    std::vector<double> c11, c21, c22, c31, c32, c33, c41, c42, c43, c44;
+    double ds;
+    Mat A[4], B[4], D[4], DY1, DY2, DY3, DY4;
+    FFTPlan px, pz;
+
+    std::vector<cd> velx, vely, velz, vorx, vory, vorz, lmbx, lmby, lmbz, Hv, Hg;
+    std::vector<cd> vely_h[3], vory_h[3], Hv_h[3], Hg_h[3];
+    std::vector<cd> lmbx_mean[3], lmbz_mean[3], velx_mean[3], velz_mean[3];
+    double fx_fric = 0, fz_fric = 0, fx_h[3] = {0, 0, 0}, fz_h[3] = {0, 0, 0};
+    double last_bottom = 0, last_upper = 0;
+    long id_step = 0;
+    std::vector<double> diag;  // (fx_fric, bottom, upper) per friction evaluation
+
+    size_t at(int k, int i) const { return ((size_t)k * NXH + i) * NY; }
+
+    Oracle(double nu_, double dt_, double dkx_, double dkz_, double alpha_, int NX_, int NY_, int NZ_)
+        : nu(nu_), dt(dt_), dkx(dkx_), dkz(dkz_), alpha_mesh(alpha_), NX(NX_), NY(NY_), NZ(NZ_) {
+        // derived sizes, src/IO.cpp:27-34
+        NXH = NX / 2; NZH = NZ / 2; NX2 = NX / 2 * 3; NZ2 = NZ / 2 * 3;
+        W = (size_t)NZ * NXH * NY;
+        allocate_var();
+        init_mat();
+        px = FFTPlan(NX2);
+        pz = FFTPlan(NZ2);
+    }
+
+    // src/solver.cpp:15-169
+    void allocate_var() {
+        kx.resize(NXH); kz.resize(NZ);
+        for (int i = 0; i < NXH; ++i) kx[i] = (double)i * dkx;
+        for (int k = 0; k < NZ; ++k) {
+            double t = (double)k;
+            if (t >= NZH) t -= NZ;
+            kz[k] = t * dkz;
+        }
+        s.resize(NY); y.resize(NY); dyds1.resize(NY); dyds2.resize(NY); dyds3.resize(NY); dyds4.resize(NY);
+        ds = 2.0 / (NY - 1.0);
+        double ap = alpha_mesh, ap2 = std::pow(ap, 2), ap3 = std::pow(ap, 3), ap4 = std::pow(ap, 4);
+        auto coth = [](double x) { return 1.0 / std::tanh(x); };
+        auto sech = [](double x) { return 1.0 / std::cosh(x); };
+        for (int i = 0; i < NY; ++i) {
+            s[i] = (double)(2 * (size_t)i) / (NY - 1.0) - 1.0;
+            double as = ap * s[i];
+            y[i] = std::tanh(as) / std::tanh(ap);
+            dyds1[i] = ap * coth(ap) * std::pow(sech(as), 2);
+            dyds2[i] = -2 * ap2 * coth(ap) * std::pow(sech(as), 2) * std::tanh(as);
+            dyds3[i] = 2 * ap3 * (-2 + std::cosh(2 * as)) * coth(ap) * std::pow(sech(as), 4);
+            dyds4[i] = -2 * ap4 * coth(ap) * std::pow(sech(as), 5) * (-11 * std::sinh(as) + std::sinh(3 * as));
+        }
+        auto P = [&](int e, int i) { return std::pow(dyds1[i], (double)e); };
+        c11.resize(NY); c21.resize(NY); c22.resize(NY); c31.resize(NY); c32.resize(NY); c33.resize(NY);
+        c41.resize(NY); c42.resize(NY); c43.resize(NY); c44.resize(NY);
+        for (int i = 0; i < NY; ++i) {
+            double d2 = dyds2[i], d3 = dyds3[i], d4 = dyds4[i];
+            c11[i] = P(-1, i);
+            c21[i] = (-d2) * P(-3, i);
+            c22[i] = P(-2, i);
+            c31[i] = (-d3) * P(-4, i) + (3.0 * std::pow(d2, 2.0)) * P(-5, i);
+            c32[i] = (-3.0 * d2) * P(-4, i);
+            c33[i] = P(-3, i);
+            c41[i] = ((-d4) * P(-5, i) + ((10.0 * d3) * d2) * P(-6, i)) - (15.0 * std::pow(d2, 3.0)) * P(-7, i);
+            c42[i] = (-4.0 * d3) * P(-5, i) + (15.0 * std::pow(d2, 2.0)) * P(-6, i);
+            c43[i] = (-6.0 * d2) * P(-5, i);
+            c44[i] = P(-4, i);
+        }
+        auto z = [&]() { return std::vector<cd>(W, cd(0.0, 0.0)); };
+        velx = z(); vely = z(); velz = z(); vorx = z(); vory = z(); vorz = z();
+        lmbx = z(); lmby = z(); lmbz = z(); Hv = z(); Hg = z();
+        for (int l = 0; l < 3; ++l) {
+            vely_h[l] = z(); vory_h[l] = z(); Hv_h[l] = z(); Hg_h[l] = z();
+            lmbx_mean[l].assign(NY, 0.0); lmbz_mean[l].assign(NY, 0.0);
+            velx_mean[l].assign(NY, 0.0); velz_mean[l].assign(NY, 0.0);
+        }
+    }
+
+    // src/solver.cpp:171-216
+    void init_mat() {
+        for (int n = 0; n < 4; ++n) { A[n] = Mat(NY); B[n] = Mat(NY); }
+        op_A1B1(NY, ds, A[0], B[0]);
+        op_A2B2(NY, ds, A[1], B[1]);
+        op_A3B3(NY, ds, A[2], B[2]);
+        op_A4B4(NY, ds, A[3], B[3]);
+        for (int n = 0; n < 4; ++n) D[n] = dense_solve_mat(A[n], B[n]);
+        DY1 = rowscale(c11, D[0]);
+        DY2 = madd(rowscale(c21, D[0]), rowscale(c22, D[1]));
+        DY3 = madd(madd(rowscale(c31, D[0]), rowscale(c32, D[1])), rowscale(c33, D[2]));
+        DY4 = madd(madd(madd(rowscale(c41, D[0]), rowscale(c42, D[1])), rowscale(c43, D[2])), rowscale(c44, D[3]));
+    }
+
+    // ---------------- transforms, src/FFT.cpp:49-194 ----------------
+    // wave [NZ][NXH][NY] complex -> physical [NZ2][NX2][NY] real
+    void f2p(const cd *u, double *up) const {
+        std::vector<cd> I((size_t)NZ2 * NXH * NY);
+#pragma omp parallel
+        {
+            std::vector<cd> a(NZ2), b(NZ2), tmp(NZ2 > NX2 ? NZ2 : NX2);
+#pragma omp for collapse(2) schedule(static)
+            for (int i = 0; i < NXH; ++i)
+                for (int j = 0; j < NY; ++j) {
+                    for (int k = 0; k < NZ2; ++k) a[k] = 0.0;
+                    for (int k = 0; k < NZH; ++k) {
+                        a[k] = u[at(k, i) + j];
+                        a[k + NZ] = u[at(k + NZH, i) + j];
+                    }
+                    a[NZ] = 0.0;  // z-Nyquist dropped (src/FFT.cpp:86-87)
+                    fft_rec(NZ2, a.data(), 1, b.data(), pz.w_bwd, NZ2, tmp.data());
+                    for (int k = 0; k < NZ2; ++k) I[((size_t)k * NXH + i) * NY + j] = b[k];
+                }
+            std::vector<cd> c(NX2), d(NX2);
+#pragma omp for collapse(2) schedule(static)
+            for (int j = 0; j < NY; ++j)
+                for (int k = 0; k < NZ2; ++k) {
+                    for (int i = 0; i < NX2; ++i) c[i] = 0.0;
+                    for (int i = 0; i < NXH; ++i) {
+                        c[i] = I[((size_t)k * NXH + i) * NY + j];
+                        if (i > 0) c[i + NX] = std::conj(I[((size_t)k * NXH + (NXH - i)) * NY + j]);
+                    }
+                    fft_rec(NX2, c.data(), 1, d.data(), px.w_bwd, NX2, tmp.data());
+                    for (int i = 0; i < NX2; ++i) up[((size_t)k * NX2 + i) * NY + j] = d[i].real();
+                }
+        }
+    }
+
+    // physical [NZ2][NX2][NY] real -> wave [NZ][NXH][NY] complex
+    void p2f(const double *up, cd *u) const {
+        std::vector<cd> I((size_t)NZ2 * NXH * NY);
+#pragma omp parallel
+        {
+            std::vector<cd> c(NX2), d(NX2), tmp(NZ2 > NX2 ? NZ2 : NX2);
+#pragma omp for collapse(2) schedule(static)
+            for (int k = 0; k < NZ2; ++k)
+                for (int j = 0; j < NY; ++j) {
+                    for (int i = 0; i < NX2; ++i) c[i] = cd(up[((size_t)k * NX2 + i) * NY + j], 0.0);
+                    fft_rec(NX2, c.data(), 1, d.data(), px.w_fwd, NX2, tmp.data());
+                    for (int i = 0; i < NXH; ++i) I[((size_t)k * NXH + i) * NY + j] = d[i] / (double)NX2;
+                }
+            std::vector<cd> a(NZ2), b(NZ2);
+#pragma omp for collapse(2) schedule(static)
+            for (int i = 0; i < NXH; ++i)
+                for (int j = 0; j < NY; ++j) {
+                    for (int k = 0; k < NZ2; ++k) a[k] = I[((size_t)k * NXH + i) * NY + j];
+                    fft_rec(NZ2, a.data(), 1, b.data(), pz.w_fwd, NZ2, tmp.data());
+                    for (int k = 0; k < NZH; ++k) {
+                        u[at(k, i) + j] = b[k] / (double)NZ2;
+                        u[at(k + NZH, i) + j] = b[k + NZ] / (double)NZ2;
+                    }
+                }
+        }
+    }
+
+    // src/solver.cpp:218-238
+    void get_vor_from_vel() {
+#pragma omp parallel
+        {
+            std::vector<cd> t(NY);
+#pragma omp for collapse(2)
+            for (int k = 0; k < NZ; ++k)
+                for (int i = 0; i < NXH; ++i) {
+                    double kxx = kx[i], kzz = kz[k];
+                    size_t o = at(k, i);
+                    cd ikz = cd(0.0, 1.0) * kzz, ikx = cd(0.0, 1.0) * kxx;
+                    matvec(DY1, &velz[o], t.data());
+                    for (int j = 0; j < NY; ++j) vorx[o + j] = t[j] - ikz * vely[o + j];
+                    for (int j = 0; j < NY; ++j) vory[o + j] = ikz * velx[o + j] - ikx * velz[o + j];
+                    matvec(DY1, &velx[o], t.data());
+                    for (int j = 0; j < NY; ++j) vorz[o + j] = ikx * vely[o + j] - t[j];
+                }
+        }
+    }
+
+    // src/solver.cpp:240-260
+    void nonlinear_source() {
+#pragma omp parallel
+        {
+            std::vector<cd> t(NY), d(NY);
+#pragma omp for collapse(2)
+            for (int k = 0; k < NZ; ++k)
+                for (int i = 0; i < NXH; ++i) {
+                    double kzz = kz[k], kxx = kx[i];
+                    double k2 = std::pow(kxx, 2) + std::pow(kzz, 2);
+                    size_t o = at(k, i);
+                    cd ikx = cd(0.0, 1.0) * kxx, ikz = cd(0.0, 1.0) * kzz;
+                    for (int j = 0; j < NY; ++j) t[j] = ikx * lmbx[o + j] + ikz * lmbz[o + j];
+                    matvec(DY1, t.data(), d.data());
+                    for (int j = 0; j < NY; ++j) Hv[o + j] = (-k2) * lmby[o + j] - d[j];
+                    for (int j = 0; j < NY; ++j) Hg[o + j] = ikz * lmbx[o + j] - ikx * lmbz[o + j];
+                }
+        }
+    }
+
+    // src/solver.cpp:262-304
+    void lamb_vec() {
+        size_t P = (size_t)NZ2 * NX2 * NY;
+        std::vector<double> up(P), vp(P), wp(P), oxp(P), oyp(P), ozp(P);
+        f2p(velx.data(), up.data()); f2p(vely.data(), vp.data()); f2p(velz.data(), wp.data());
+        f2p(vorx.data(), oxp.data()); f2p(vory.data(), oyp.data()); f2p(vorz.data(), ozp.data());
+        std::vector<double> lx(P), ly(P), lz(P);
+#pragma omp parallel for schedule(static)
+        for (size_t e = 0; e < P; ++e) {
+            lx[e] = vp[e] * ozp[e] - wp[e] * oyp[e];
+            ly[e] = wp[e] * oxp[e] - up[e] * ozp[e];
+            lz[e] = up[e] * oyp[e] - vp[e] * oxp[e];
+        }
+        p2f(lx.data(), lmbx.data()); p2f(ly.data(), lmby.data()); p2f(lz.data(), lmbz.data());
+        lmbx_mean[2] = lmbx_mean[1]; lmbx_mean[1] = lmbx_mean[0];
+        lmbx_mean[0].assign(lmbx.begin(), lmbx.begin() + NY);
+        lmbz_mean[2] = lmbz_mean[1]; lmbz_mean[1] = lmbz_mean[0];
+        lmbz_mean[0].assign(lmbz.begin(), lmbz.begin() + NY);
+    }
+
+    // src/solver.cpp:306-327
+    void solve_friction_force() {
+        std::vector<cd> dudy(NY), dwdy(NY);
+        matvec(DY1, &velx[0], dudy.data());
+        matvec(DY1, &velz[0], dwdy.data());
+        double Ly = y[NY - 1] - y[0];
+        fx_fric = -nu / Ly * (dudy[NY - 1].real() - dudy[0].real());
+        fz_fric = -nu / Ly * (dwdy[NY - 1].real() - dwdy[0].real());
+        last_bottom = 2 * dudy[0].real() * nu / Ly;
+        last_upper = 2 * dudy[NY - 1].real() * nu / Ly;
+        diag.push_back(fx_fric); diag.push_back(last_bottom); diag.push_back(last_upper);
+        fx_h[2] = fx_h[1]; fx_h[1] = fx_h[0]; fx_h[0] = fx_fric;
+        fz_h[2] = fz_h[1]; fz_h[1] = fz_h[0]; fz_h[0] = fz_fric;
+    }
+
+    // src/solver.cpp:428-514
+    void solve_mean() {
+        const double g0 = 11.0 / 6.0, a0 = 3.0, a1 = -1.5, a2 = 1.0 / 3.0, b0 = 3.0, b1 = -3.0, b2 = 1.0;
+        Mat Am(NY);
+        for (size_t e = 0; e < Am.a.size(); ++e) Am.a[e] = (-nu) * DY2.a[e];
+        for (int i = 0; i < NY; ++i) Am(i, i) = Am(i, i) + g0 / dt;
+        std::vector<cd> su(NY), sw(NY);
+        for (int j = 0; j < NY; ++j) {
+            cd t0 = b0 * (lmbx_mean[0][j] + fx_h[0]);
+            cd t1 = b1 * (lmbx_mean[1][j] + fx_h[1]);
+            cd t2 = b2 * (lmbx_mean[2][j] + fx_h[2]);
+            cd va = (a0 * velx_mean[0][j] + a1 * velx_mean[1][j]) + a2 * velx_mean[2][j];
+            su[j] = ((t0 + t1) + t2) + cd(va.real() / dt, va.imag() / dt);
+            t0 = b0 * (lmbz_mean[0][j] + fz_h[0]);
+            t1 = b1 * (lmbz_mean[1][j] + fz_h[1]);
+            t2 = b2 * (lmbz_mean[2][j] + fz_h[2]);
+            va = (a0 * velz_mean[0][j] + a1 * velz_mean[1][j]) + a2 * velz_mean[2][j];
+            sw[j] = ((t0 + t1) + t2) + cd(va.real() / dt, va.imag() / dt);
+        }
+        for (int j = 0; j < NY; ++j) { Am(0, j) = 0.0; Am(NY - 1, j) = 0.0; }
+        Am(0, 0) = 1.0; Am(NY - 1, NY - 1) = 1.0;
+        su[0] = 0.0; su[NY - 1] = 0.0; sw[0] = 0.0; sw[NY - 1] = 0.0;
+        dense_solve_complex(Am, su.data());
+        dense_solve_complex(Am, sw.data());
+        for (int j = 0; j < NY; ++j) { velx[j] = su[j]; velz[j] = sw[j]; }
+        velx_mean[2] = velx_mean[1]; velx_mean[1] = velx_mean[0]; velx_mean[0] = su;
+        velz_mean[2] = velz_mean[1]; velz_mean[1] = velz_mean[0]; velz_mean[0] = sw;
+        solve_friction_force();
+        std::vector<cd> t(NY);
+        matvec(DY1, &velz[0], t.data());
+        for (int j = 0; j < NY; ++j) vorx[j] = t[j];
+        matvec(DY1, &velx[0], t.data());
+        for (int j = 0; j < NY; ++j) vorz[j] = -t[j];
+    }
+
+    // src/solver.cpp:516-539
+    void solve_horizontal() {
+#pragma omp parallel
+        {
+            std::vector<cd> d1v(NY), d2v(NY), d1g(NY);
+#pragma omp for collapse(2) schedule(dynamic, 4)
+            for (int k = 0; k < NZ; ++k)
+                for (int i = 0; i < NXH; ++i) {
+                    double kxx = kx[i], kzz = kz[k];
+                    double k2 = std::pow(kxx, 2) + std::pow(kzz, 2);
+                    if (k2 < 1.e-8) continue;  // mean pencil: solve_mean, below
+                    size_t o = at(k, i);
+                    matvec(DY1, &vely[o], d1v.data());
+                    matvec(DY2, &vely[o], d2v.data());
+                    matvec(DY1, &vory[o], d1g.data());
+                    cd ix = cd(0.0, 1.0) * kxx / k2, iz = cd(0.0, 1.0) * kzz / k2, ir = cd(0.0, 1.0) / k2;
+                    for (int j = 0; j < NY; ++j) {
+                        velx[o + j] = ix * d1v[j] - iz * vory[o + j];
+                        velz[o + j] = ix * vory[o + j] + iz * d1v[j];
+                        vorx[o + j] = ir * ((kxx * d1g[j] + kzz * d2v[j]) - (kzz * k2) * vely[o + j]);
+                        vorz[o + j] = ir * ((kzz * d1g[j] - kxx * d2v[j]) + (kxx * k2) * vely[o + j]);
+                    }
+                }
+        }
+        for (int k = 0; k < NZ; ++k)
+            for (int i = 0; i < NXH; ++i)
+                if (std::pow(kx[i], 2) + std::pow(kz[k], 2) < 1.e-8) solve_mean();
+    }
+
+    // src/solver.cpp:541-627
+    void time_integration() {
+        const double g0 = 11.0 / 6.0, a0 = 3.0, a1 = -1.5, a2 = 1.0 / 3.0, b0 = 3.0, b1 = -3.0, b2 = 1.0;
+        int errflag = 0;
+#pragma omp parallel
+        {
+            std::vector<cd> sv(NY), sg(NY), sa(NY), d2(NY);
+            Mat Av(NY), Ag(NY);
+#pragma omp for collapse(2) schedule(dynamic, 4)
+            for (int k = 0; k < NZ; ++k)
+                for (int i = 0; i < NXH; ++i) {
+                    double k2 = std::pow(kx[i], 2) + std::pow(kz[k], 2);
+                    double k4 = std::pow(k2, 2);
+                    size_t o = at(k, i);
+                    for (int j = 0; j < NY; ++j) {
+                        sv[j] = (Hv_h[0][o + j] * b0 + Hv_h[1][o + j] * b1) + Hv_h[2][o + j] * b2;
+                        sa[j] = (vely_h[0][o + j] * a0 + vely_h[1][o + j] * a1) + vely_h[2][o + j] * a2;
+                        sv[j] = sv[j] - (k2 / dt) * sa[j];
+                    }
+                    matvec(DY2, sa.data(), d2.data());
+                    for (int j = 0; j < NY; ++j) sv[j] = sv[j] + cd(d2[j].real() / dt, d2[j].imag() / dt);
+                    for (int j = 0; j < NY; ++j) {
+                        cd hg = (Hg_h[0][o + j] * b0 + Hg_h[1][o + j] * b1) + Hg_h[2][o + j] * b2;
+                        cd va = (vory_h[0][o + j] * a0 + vory_h[1][o + j] * a1) + vory_h[2][o + j] * a2;
+                        sg[j] = hg + cd(va.real() / dt, va.imag() / dt);
+                    }
+                    double sa1 = g0 / dt + 2 * nu * k2, sa2 = nu * k4 + g0 * k2 / dt;
+                    double sg1 = nu * k2 + g0 / dt;
+                    for (size_t e = 0; e < Av.a.size(); ++e) {
+                        Av.a[e] = (-nu) * DY4.a[e] + sa1 * DY2.a[e];
+                        Ag.a[e] = (-nu) * DY2.a[e];
+                    }
+                    for (int j = 0; j < NY; ++j) { Av(j, j) = Av(j, j) - sa2; Ag(j, j) = Ag(j, j) + sg1; }
+                    for (int j = 0; j < NY; ++j) {
+                        Av(0, j) = 0.0; Av(NY - 1, j) = 0.0;
+                        Av(1, j) = DY1(0, j); Av(NY - 2, j) = DY1(NY - 1, j);
+                        Ag(0, j) = 0.0; Ag(NY - 1, j) = 0.0;
+                    }
+                    Av(0, 0) = 1.0; Av(NY - 1, NY - 1) = 1.0;
+                    Ag(0, 0) = 1.0; Ag(NY - 1, NY - 1) = 1.0;
+                    sv[0] = sv[1] = sv[NY - 2] = sv[NY - 1] = 0.0;
+                    sg[0] = sg[NY - 1] = 0.0;
+                    try {
+                        dense_solve_complex(Av, sv.data());
+                        dense_solve_complex(Ag, sg.data());
+                    } catch (...) {
+#pragma omp atomic write
+                        errflag = 1;
+                    }
+                    for (int j = 0; j < NY; ++j) { vely[o + j] = sv[j]; vory[o + j] = sg[j]; }
+                }
+        }
+        if (errflag) throw std::runtime_error("LU factorization failed. Matrix A may be singular.");
+        solve_horizontal();
+        lamb_vec();
+        nonlinear_source();
+        rotate_history();
+    }
+
+    void rotate_history() {
+        std::swap(vely_h[2], vely_h[1]); std::swap(vely_h[1], vely_h[0]); vely_h[0] = vely;
+        std::swap(vory_h[2], vory_h[1]); std::swap(vory_h[1], vory_h[0]); vory_h[0] = vory;
+        std::swap(Hv_h[2], Hv_h[1]); std::swap(Hv_h[1], Hv_h[0]); Hv_h[0] = Hv;
+        std::swap(Hg_h[2], Hg_h[1]); std::swap(Hg_h[1], Hg_h[0]); Hg_h[0] = Hg;
+    }
+
+    // post-processing of init_fields after the velocity is set (src/solver.cpp:379-425)
+    void init_post() {
+        get_vor_from_vel();
+        lamb_vec();
+        nonlinear_source();
+        for (int l = 0; l < 3; ++l) { vely_h[l] = vely; vory_h[l] = vory; Hv_h[l] = Hv; Hg_h[l] = Hg; }
+        solve_friction_force();
+        for (int l = 0; l < 3; ++l) {
+            fx_h[l] = fx_fric; fz_h[l] = fz_fric;
+            velx_mean[l].assign(velx.begin(), velx.begin() + NY);
+            velz_mean[l].assign(velz.begin(), velz.begin() + NY);
+            lmbx_mean[l].assign(lmbx.begin(), lmbx.begin() + NY);
+            lmbz_mean[l].assign(lmbz.begin(), lmbz.begin() + NY);
+        }
+    }
+
+    void zero_velocity() {
+        std::fill(velx.begin(), velx.end(), cd(0.0, 0.0));
+        std::fill(vely.begin(), vely.end(), cd(0.0, 0.0));
+        std::fill(velz.begin(), velz.end(), cd(0.0, 0.0));
+    }
+
+    // src/solver.cpp:329-377: laminar profile plus the fixed (2, +-2) wave
+    void init_reference() {
+        zero_velocity();
+        for (int j = 0; j < NY; ++j) velx[j] = cd(1.5 * (1 - std::pow(y[j], 2)), 0.0);
+        const double k1 = 2.0, k2 = 2.0, al = 1.0;
+        const cd I(0.0, 1.0);
+        for (int k = 0; k < NZ; ++k)
+            for (int i = 0; i < NXH; ++i)
+                for (int j = 0; j < NY; ++j) {
+                    double yy = y[j];
+                    size_t o = at(k, i) + j;
+                    if (kz[k] == k2 && kx[i] == k1) {
+                        velx[o] += al / k1 * (yy - std::pow(yy, 3)) * (-0.25 * I);
+                        vely[o] += al * (0.5 * std::pow(yy, 2) - 0.25 * std::pow(yy, 4) - 0.25) * (0.25);
+                        velz[o] += -2 * al / k2 * (yy - std::pow(yy, 3)) * (-0.25 * I);
+                    }
+                    if (kz[k] == -k2 && kx[i] == k1) {
+                        velx[o] += al / k1 * (yy - std::pow(yy, 3)) * (-0.25 * I);
+                        vely[o] += al * (0.5 * std::pow(yy, 2) - 0.25 * std::pow(yy, 4) - 0.25) * (0.25);
+                        velz[o] += -2 * al / k2 * (yy - std::pow(yy, 3)) * (0.25 * I);
+                    }
+                }
+    }
+
+    // Seeded divergence-free perturbation (recipe defined by this build):
+    // v = (1-y^2)^2 sum_p c_p T_p(y), g = (1-y^2) sum_p d_p T_p(y) on pencils with
+    // 1 <= |(i, m)| <= 8 in index space (z-Nyquist excluded), Hermitian in the
+    // kx = 0 plane, u and w from the horizontal-recovery formulas with the
+    // discrete DY1, scaled to peak |u'| = amp on the 3/2 grid, laminar added.
+    void init_seeded(uint64_t seed, double amp) {
+        zero_velocity();
+        std::vector<cd> g(W, cd(0.0, 0.0));
+        for (int k = 0; k < NZ; ++k) {
+            if (k == NZH) continue;
+            int m = k < NZH ? k : k - NZ;
+            for (int i = 0; i < NXH; ++i) {
+                int r2 = i * i + m * m;
+                if (r2 < 1 || r2 > 64) continue;
+                if (i == 0 && m < 0) continue;
+                cd c[4], d[4];
+                for (int p = 0; p < 4; ++p) { c[p] = seeded_gauss(seed, 0, i, m, p); d[p] = seeded_gauss(seed, 1, i, m, p); }
+                size_t o = at(k, i);
+                size_t oc = (i == 0) ? at(NZ - m, 0) : 0;
+                for (int j = 0; j < NY; ++j) {
+                    double yy = y[j];
+                    double T[4] = {1.0, yy, 2.0 * yy * yy - 1.0, (4.0 * yy * yy - 3.0) * yy};
+                    double w1 = 1.0 - yy * yy;
+                    cd sv = ((c[0] * T[0] + c[1] * T[1]) + c[2] * T[2]) + c[3] * T[3];
+                    cd sg = ((d[0] * T[0] + d[1] * T[1]) + d[2] * T[2]) + d[3] * T[3];
+                    vely[o + j] = (w1 * w1) * sv;
+                    g[o + j] = w1 * sg;
+                    if (i == 0) { vely[oc + j] = std::conj(vely[o + j]); g[oc + j] = std::conj(g[o + j]); }
+                }
+            }
+        }
+        std::vector<cd> d1(NY);
+        for (int k = 0; k < NZ; ++k)
+            for (int i = 0; i < NXH; ++i) {
+                double kxx = kx[i], kzz = kz[k];
+                double k2 = std::pow(kxx, 2) + std::pow(kzz, 2);
+                if (k2 < 1.e-8) continue;
+                size_t o = at(k, i);
+                matvec(DY1, &vely[o], d1.data());
+                cd ix = cd(0.0, 1.0) * kxx / k2, iz = cd(0.0, 1.0) * kzz / k2;
+                for (int j = 0; j < NY; ++j) {
+                    velx[o + j] = ix * d1[j] - iz * g[o + j];
+                    velz[o + j] = ix * g[o + j] + iz * d1[j];
+                }
+            }
+        double peak = peak_velocity();
+        if (peak > 0.0) {
+            double f = amp / peak;
+            for (size_t e = 0; e < W; ++e) { velx[e] *= f; vely[e] *= f; velz[e] *= f; }
+        }
+        for (int j = 0; j < NY; ++j) velx[j] += cd(1.5 * (1 - std::pow(y[j], 2)), 0.0);
+    }
+
+    double peak_velocity() const {
+        size_t P = (size_t)NZ2 * NX2 * NY;
+        std::vector<double> up(P), vp(P), wp(P);
+        f2p(velx.data(), up.data()); f2p(vely.data(), vp.data()); f2p(velz.data(), wp.data());
+        double mx = 0.0;
+        for (size_t e = 0; e < P; ++e) {
+            double m2 = up[e] * up[e] + vp[e] * vp[e] + wp[e] * wp[e];
+            if (m2 > mx) mx = m2;
+        }
+        return std::sqrt(mx);
+    }
+};
+
+thread_local std::string g_err;
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// C ABI for the test harness (ctypes).
+// ----------------------------------------------------------------------------
+extern "C" {
+
+const char *orc_last_error() { return g_err.c_str(); }
+
+void *orc_create(double nu, double dt, double dkx, double dkz, double alpha, int NX, int NY, int NZ) {
+    try {
+        if (NY < 7 || NX < 4 || NZ < 4 || NX % 2 || NZ % 2) throw std::invalid_argument("bad grid size");
+        return new Oracle(nu, dt, dkx, dkz, alpha, NX, NY, NZ);
+    } catch (std::exception &e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void orc_destroy(void *h) { delete (Oracle *)h; }
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+// mode 1: reference laminar + fixed wave; mode 2: seeded recipe.
+int orc_init(void *h, int mode, uint64_t seed, double amp) {
+    Oracle *o = (Oracle *)h;
+    try {
+        if (mode == 1) o->init_reference();
+        else if (mode == 2) o->init_seeded(seed, amp);
+        else throw std::invalid_argument("init mode must be 1 or 2");
+        o->init_post();
+        return 0;
+    } catch (std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// u, v, w: [NZ][NXH][NY] complex interleaved (re, im)
+int orc_set_velocity(void *h, const double *u, const double *v, const double *w) {
+    Oracle *o = (Oracle *)h;
+    try {
+        std::memcpy((void *)o->velx.data(), u, o->W * sizeof(cd));
+        std::memcpy((void *)o->vely.data(), v, o->W * sizeof(cd));
+        std::memcpy((void *)o->velz.data(), w, o->W * sizeof(cd));
+        o->init_post();
+        return 0;
+    } catch (std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int orc_step(void *h, int n) {
+    Oracle *o = (Oracle *)h;
+    try {
+        for (int s = 0; s < n; ++s) {
+            o->time_integration();
+            o->id_step += 1;
+        }
+        return 0;
+    } catch (std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// field ids: 0 u, 1 v, 2 w, 3 omx, 4 omy, 5 omz, 6 lmbx, 7 lmby, 8 lmbz, 9 Hv, 10 Hg
+int orc_get_field(void *h, int f, double *out) {
+    Oracle *o = (Oracle *)h;
+    const std::vector<cd> *src[] = {&o->velx, &o->vely, &o->velz, &o->vorx, &o->vory, &o->vorz,
+                                    &o->lmbx, &o->lmby, &o->lmbz, &o->Hv, &o->Hg};
+    if (f < 0 || f > 10) return -1;
+    std::memcpy(out, src[f]->data(), o->W * sizeof(cd));
+    return 0;
+}
+
+// matrices: 0..3 DY1..DY4, 4..7 A1..A4, 8..11 B1..B4, 12..15 D1..D4; column-major
+int orc_get_matrix(void *h, int which, double *out) {
+    Oracle *o = (Oracle *)h;
+    const Mat *m = nullptr;
+    if (which >= 0 && which < 4) m = (which == 0) ? &o->DY1 : (which == 1) ? &o->DY2 : (which == 2) ? &o->DY3 : &o->DY4;
+    else if (which < 8) m = &o->A[which - 4];
+    else if (which < 12) m = &o->B[which - 8];
+    else if (which < 16) m = &o->D[which - 12];
+    if (!m) return -1;
+    std::memcpy(out, m->a.data(), m->a.size() * sizeof(double));
+    return 0;
+}
+
+// vectors: 0 y, 1 s, 2..5 dyds1..4, 6 c11, 7 c21, 8 c22, 9 c41, 10 c42, 11 c43, 12 c44, 13 kx, 14 kz
+int orc_get_vector(void *h, int which, double *out) {
+    Oracle *o = (Oracle *)h;
+    const std::vector<double> *v[] = {&o->y, &o->s, &o->dyds1, &o->dyds2, &o->dyds3, &o->dyds4, &o->c11, &o->c21,
+                                      &o->c22, &o->c41, &o->c42, &o->c43, &o->c44, &o->kx, &o->kz};
+    if (which < 0 || which > 14) return -1;
+    std::memcpy(out, v[which]->data(), v[which]->size() * sizeof(double));
+    return 0;
+}
+
+// diagnostics of the most recent friction evaluation and the whole series
+int orc_wall_shear(void *h, double *fx, double *bottom, double *upper) {
+    Oracle *o = (Oracle *)h;
+    *fx = o->fx_fric; *bottom = o->last_bottom; *upper = o->last_upper;
+    return 0;
+}
+int orc_diag_count(void *h) { return (int)(((Oracle *)h)->diag.size() / 3); }
+int orc_diag(void *h, double *out) {
+    Oracle *o = (Oracle *)h;
+    std::memcpy(out, o->diag.data(), o->diag.size() * sizeof(double));
+    return 0;
+}
+
+// stand-alone stages for stage-by-stage parity
+int orc_f2p(void *h, const double *wave, double *phys) {
+    ((Oracle *)h)->f2p((const cd *)wave, phys);
+    return 0;
+}
+int orc_p2f(void *h, const double *phys, double *wave) {
+    ((Oracle *)h)->p2f(phys, (cd *)wave);
+    return 0;
+}
+// Lamb vector of given (u, v, w, omx, omy, omz) wave fields, without touching the mean history
+int orc_lamb(void *h, const double *in6, double *out3) {
+    Oracle *o = (Oracle *)h;
+    size_t W = o->W, P = (size_t)o->NZ2 * o->NX2 * o->NY;
+    std::vector<double> f[6];
+    for (int q = 0; q < 6; ++q) {
+        f[q].resize(P);
+        o->f2p((const cd *)(in6 + 2 * W * q), f[q].data());
+    }
+    std::vector<double> l(P);
+    for (size_t e = 0; e < P; ++e) l[e] = f[1][e] * f[5][e] - f[2][e] * f[4][e];
+    o->p2f(l.data(), (cd *)(out3));
+    for (size_t e = 0; e < P; ++e) l[e] = f[2][e] * f[3][e] - f[0][e] * f[5][e];
+    o->p2f(l.data(), (cd *)(out3 + 2 * W));
+    for (size_t e = 0; e < P; ++e) l[e] = f[0][e] * f[4][e] - f[1][e] * f[3][e];
+    o->p2f(l.data(), (cd *)(out3 + 4 * W));
+    return 0;
+}
+
+int orc_peak_velocity(void *h, double *out) {
+    *out = ((Oracle *)h)->peak_velocity();
+    return 0;
+}
+
+// 1-D DFT (sign +1 backward, -1 forward), unnormalised
+int orc_fft(int n, int sign, const double *in, double *out) {
+    FFTPlan P(n);
+    fft(P, sign, (const cd *)in, (cd *)out);
+    return 0;
+}
+
+// dense solve with LAPACK dgetrf/dgetrs semantics; A column-major, nrhs columns in b
+int orc_lu_solve(int n, const double *A, int nrhs, double *b) {
+    std::vector<double> LU(A, A + (size_t)n * n);
+    std::vector<int> ipiv(n);
+    int info = lu_factor(n, LU.data(), ipiv.data());
+    if (info) return info;
+    for (int r = 0; r < nrhs; ++r) lu_solve(n, LU.data(), ipiv.data(), b + (size_t)r * n);
+    return 0;
+}
+
+// LU factors (LAPACK layout: unit-lower L and U in place, 0-based ipiv)
+int orc_lu_factor(int n, double *A, int *ipiv) { return lu_factor(n, A, ipiv); }
+
+// seeded Gaussian draw, exposed to pin the recipe's random stream
+void orc_seeded_gauss(uint64_t seed, int field, int i, int m, int p, double *out) {
+    cd g = seeded_gauss(seed, field, i, m, p);
+    out[0] = g.real();
+    out[1] = g.imag();
+}
+
+}  // extern "C"

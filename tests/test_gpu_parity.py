@@ -1,0 +1,206 @@
+"""Parity of the HIP time step (through the C ABI) with the CPU oracle.
+
+Tolerances (relative L2 over a whole field, ||a - b|| / ||b||):
+  transforms, init, one Lamb evaluation      1e-12 (roundoff of different FFTs)
+  FP64 step after 10 steps                   1e-10 (the required parity bar)
+  batched vs plain method                    1e-13 (same factors, same arithmetic)
+  mixed precision vs FP64 after 20 steps     5e-6 on fields, 1e-6 on wall shear
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RE180 = dict(nu=1 / 2800, dt=1e-3, dkx=0.5, dkz=1.0, alpha_mesh=1.5)
+FIELDS = ("u", "v", "w", "omx", "omy", "omz", "lmbx", "lmby", "lmbz", "Hv", "Hg")
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def params(NX, NY, NZ, **kw):
+    p = dict(RE180, NX=NX, NY=NY, NZ=NZ, precision=0, method=1)
+    p.update(kw)
+    return p
+
+
+def gpu(owt_mod, p, **kw):
+    print(f"[gpu] channel {p['NX']}x{p['NY']}x{p['NZ']} {kw}", flush=True)
+    return owt_mod.Channel(params=p, **kw)
+
+
+def orc(oracle_mod, p):
+    return oracle_mod.Oracle(p["nu"], p["dt"], p["dkx"], p["dkz"], p["alpha_mesh"], p["NX"], p["NY"], p["NZ"])
+
+
+@pytest.mark.parametrize("shape", [(16, 17, 16), (64, 65, 64), (12, 33, 24)])
+def test_transforms_match_oracle(owt_mod, oracle_mod, shape):
+    p = params(*shape)
+    ch, o = gpu(owt_mod, p), orc(oracle_mod, p)
+    rng = np.random.default_rng(1)
+    F = rng.standard_normal((3,) + o.wave_shape) + 1j * rng.standard_normal((3,) + o.wave_shape)
+    P = ch.f2p(F)
+    for q in range(3):
+        assert rel(P[q], o.f2p(F[q])) < 1e-13
+    back = ch.p2f(P)
+    for q in range(3):
+        assert rel(back[q], o.p2f(P[q])) < 1e-13
+
+
+@pytest.mark.parametrize("shape", [(16, 17, 16), (64, 65, 64)])
+def test_lamb_matches_oracle(owt_mod, oracle_mod, shape):
+    p = params(*shape)
+    ch, o = gpu(owt_mod, p), orc(oracle_mod, p)
+    rng = np.random.default_rng(2)
+    six = rng.standard_normal((6,) + o.wave_shape) + 1j * rng.standard_normal((6,) + o.wave_shape)
+    ref = o.lamb(six)
+    got = ch.lamb(six)
+    for q in range(3):
+        assert rel(got[q], ref[q]) < 1e-13
+    ch.set_precision(1)
+    got32 = ch.lamb(six)
+    for q in range(3):
+        assert rel(got32[q], ref[q]) < 2e-6
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_init_fields_match_oracle(owt_mod, oracle_mod, mode):
+    p = params(64, 65, 64)
+    ch, o = gpu(owt_mod, p), orc(oracle_mod, p)
+    ch.init_fields(mode, seed=0, amp=0.15)
+    o.init(mode, seed=0, amp=0.15)
+    for name in FIELDS:
+        assert rel(ch.field(name), o.field(name)) < 1e-12, name
+    np.testing.assert_allclose(ch.wall_shear(), o.wall_shear(), rtol=1e-13)
+
+
+@pytest.fixture(scope="module")
+def ten_steps_64(owt_mod, oracle_mod):
+    """64x65x64, seeded start, 10 steps: oracle, plain GPU method, batched GPU method."""
+    p = params(64, 65, 64)
+    o = orc(oracle_mod, p)
+    o.init(2, seed=0, amp=0.15)
+    o.step(10)
+    ref = {n: o.field(n) for n in FIELDS}
+    out = {"oracle": ref, "oracle_diag": o.diag()}
+    for method in (0, 1):
+        ch = gpu(owt_mod, p, method=method)
+        ch.init_fields(2, seed=0, amp=0.15)
+        ch.step(10)
+        out[method] = {n: ch.field(n) for n in FIELDS}
+        out[(method, "diag")] = ch.diag()
+        out[(method, "steps")] = ch.steps_done()
+        ch.close()
+    return out
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_ten_step_parity_64(ten_steps_64, method):
+    got, ref = ten_steps_64[method], ten_steps_64["oracle"]
+    assert ten_steps_64[(method, "steps")] == 10
+    for name in FIELDS:
+        assert rel(got[name], ref[name]) < 1e-10, (name, rel(got[name], ref[name]))
+    d, dref = ten_steps_64[(method, "diag")], ten_steps_64["oracle_diag"]
+    assert d.shape == dref.shape == (11, 3)
+    np.testing.assert_allclose(d, dref, rtol=1e-10, atol=0)
+
+
+def test_batched_equals_plain(ten_steps_64):
+    for name in FIELDS:
+        assert rel(ten_steps_64[1][name], ten_steps_64[0][name]) < 1e-13, name
+
+
+def test_reference_initial_wave_parity(owt_mod, oracle_mod):
+    """mode 1: laminar + the reference's (2, +-2) wave, 10 steps on a small grid with N = 33"""
+    p = params(16, 33, 16, dkx=0.5, dkz=1.0)
+    o = orc(oracle_mod, p)
+    o.init(1)
+    o.step(10)
+    for method in (0, 1):
+        ch = gpu(owt_mod, p, method=method)
+        ch.init_fields(1)
+        ch.step(10)
+        for name in FIELDS:
+            assert rel(ch.field(name), o.field(name)) < 1e-10, (method, name)
+
+
+def test_ten_step_parity_128(owt_mod, oracle_mod):
+    """Re_tau 180, box 4pi x 2 x 4pi/3, 128x129x128: first 10 steps against the oracle."""
+    p = params(128, 129, 128, dkz=1.5)
+    o = orc(oracle_mod, p)
+    o.init(2, seed=0, amp=0.15)
+    o.step(10)
+    ch = gpu(owt_mod, p, method=1)
+    ch.init_fields(2, seed=0, amp=0.15)
+    ch.step(10)
+    for name in ("u", "v", "w", "omx", "omy", "omz"):
+        assert rel(ch.field(name), o.field(name)) < 1e-10, name
+    np.testing.assert_allclose(ch.diag(), o.diag(), rtol=1e-10)
+
+
+def test_mixed_precision_close_to_fp64(owt_mod):
+    p = params(64, 65, 64)
+    a = gpu(owt_mod, p, method=1, precision=0)
+    b = gpu(owt_mod, p, method=1, precision=1)
+    for ch in (a, b):
+        ch.init_fields(2, seed=0, amp=0.15)
+        ch.step(20)
+    for name in ("u", "v", "w"):
+        assert rel(b.field(name), a.field(name)) < 5e-6, name
+    da, db = a.diag(), b.diag()
+    assert np.abs(db[:, 0] - da[:, 0]).max() <= 1e-6 * np.abs(da[:, 0]).max()
+
+
+def test_checkpoint_restore_is_exact(owt_mod, tmp_path):
+    p = params(32, 33, 32)
+    ch = gpu(owt_mod, p)
+    ch.init_fields(2, seed=4, amp=0.2)
+    ch.step(3)
+    path = str(tmp_path / "state.ckpt")
+    ch.checkpoint(path)
+    ch.step(4)
+    ref = {n: ch.field(n) for n in FIELDS}
+    ch.restore(path)
+    assert ch.steps_done() == 3
+    ch.step(4)
+    for n in FIELDS:
+        assert np.array_equal(ch.field(n), ref[n]), n
+
+
+def test_laminar_flow_stays_put_on_gpu(owt_mod, oracle_mod):
+    p = params(16, 65, 16)
+    ch, o = gpu(owt_mod, p), orc(oracle_mod, p)
+    y = o.vector("y")
+    u = np.zeros(o.wave_shape, dtype=complex)
+    u[0, 0] = 1.5 * (1 - y ** 2)
+    z = np.zeros_like(u)
+    ch.set_velocity(u, z, z)
+    ch.step(20)
+    assert np.abs(ch.field("u")[0, 0] - u[0, 0]).max() < 1e-6
+    assert np.all(np.abs(ch.diag()[:, 0] - 3 * p["nu"]) < 5e-5 * 3 * p["nu"])
+
+
+def test_step_keeps_hermitian_plane_and_divergence(owt_mod):
+    p = params(32, 33, 32)
+    ch = gpu(owt_mod, p)
+    ch.init_fields(2, seed=9, amp=0.3)
+    ch.step(5)
+    DY1 = ch.matrix("DY1")
+    kx = np.arange(ch.NXH) * p["dkx"]
+    kz = np.array([k if k < ch.NZH else k - ch.NZ for k in range(ch.NZ)]) * p["dkz"]
+    u, v, w = ch.field("u"), ch.field("v"), ch.field("w")
+    for name in ("u", "v", "w", "omy", "Hv", "Hg"):
+        f = ch.field(name)
+        for m in range(1, ch.NZH):
+            assert np.abs(f[m, 0] - np.conj(f[ch.NZ - m, 0])).max() < 1e-12 * np.abs(f).max(), name
+    for k in range(ch.NZ):
+        for i in range(ch.NXH):
+            if kx[i] ** 2 + kz[k] ** 2 < 1e-8:
+                continue
+            d1v = DY1 @ v[k, i]
+            div = 1j * kx[i] * u[k, i] + d1v + 1j * kz[k] * w[k, i]
+            assert np.abs(div).max() <= 1e-12 * max(np.abs(d1v).max(), 1e-30)

@@ -1,0 +1,347 @@
+"""Pins of the CPU oracle (oracle/owt_oracle.cpp) against what the mathematics fixes.
+
+The reference ships no tests, golden vectors or fixtures, and it cannot be built
+in this image (its path needs FFTW, LAPACKE/CBLAS headers and parallel HDF5).
+So each part of the oracle is checked here against an independent fact:
+
+  transforms      brute-force DFT; dealiasing identity against a direct
+                  spectral convolution with the reference's conventions
+                  (z-Nyquist dropped, kx = 0 enters through Re, 3/2 padding)
+  dense LU        LAPACK itself (numpy.linalg.solve / scipy lu_factor are
+                  dgesv / dgetrf, the routines src/linalg.hpp calls)
+  compact rows    exactness degree on local Taylor monomials (Lele's schemes),
+                  the B3 top-row overwrite signature (src/compdiff.hpp:257-258)
+  metric, DY_n    derivatives of tanh(a s)/tanh(a) and of smooth functions
+  step            laminar Poiseuille balance fx_fric = 3 nu, Hermitian symmetry
+                  of the kx = 0 plane, discrete divergence, seeded amplitude
+"""
+from math import factorial
+
+import numpy as np
+import pytest
+
+RE180 = dict(nu=1 / 2800, dt=1e-3, dkx=0.5, dkz=1.0, alpha_mesh=1.5)
+
+
+def make(O, NX, NY, NZ, **kw):
+    p = dict(RE180)
+    p.update(kw)
+    return O.Oracle(p["nu"], p["dt"], p["dkx"], p["dkz"], p["alpha_mesh"], NX, NY, NZ)
+
+
+# ---------------------------------------------------------------------------
+# transforms
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 6, 8, 9, 12, 16, 18, 24, 36, 48, 72, 96, 144, 192, 288, 576])
+def test_fft_matches_bruteforce_dft(oracle_mod, n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    j = np.arange(n)
+    for sign in (-1, 1):
+        F = np.exp(sign * 2j * np.pi * (np.outer(j, j) % n) / n)
+        ref = F @ x
+        got = oracle_mod.fft(x, sign)
+        assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+
+
+def effective_spectrum(F, NXH, NZH, NZ):
+    """Full 2-D spectrum S[i + NXH - 1, m + NZH - 1] of the physical field the
+    reference's Fourier2Physical_dealiased produces from wave field F[k][i][y]."""
+    NY = F.shape[2]
+    S = np.zeros((2 * NXH - 1, 2 * NZH - 1, NY), dtype=complex)
+    def k_of(m):
+        return m if m >= 0 else m + NZ
+    for m in range(-NZH + 1, NZH):
+        for i in range(1, NXH):
+            S[i + NXH - 1, m + NZH - 1] = F[k_of(m), i]
+            S[-i + NXH - 1, -m + NZH - 1] = np.conj(F[k_of(m), i])
+        S[NXH - 1, m + NZH - 1] = 0.5 * (F[k_of(m), 0] + np.conj(F[k_of(-m), 0]))
+    return S
+
+
+def conv2(A, B):
+    a0, a1 = A.shape[:2]
+    b0, b1 = B.shape[:2]
+    H = np.zeros((a0 + b0 - 1, a1 + b1 - 1, A.shape[2]), dtype=complex)
+    for p in range(a0):
+        for q in range(a1):
+            H[p:p + b0, q:q + b1] += A[p, q] * B
+    return H
+
+
+@pytest.mark.parametrize("NX,NY,NZ", [(8, 7, 8), (12, 9, 16), (16, 7, 12)])
+def test_lamb_dealiasing_identity(oracle_mod, NX, NY, NZ):
+    """u x omega through the 3/2-padded transforms equals the direct, truncated
+    spectral convolution: exact to roundoff for every retained mode."""
+    o = make(oracle_mod, NX, NY, NZ)
+    NXH, NZH = NX // 2, NZ // 2
+    rng = np.random.default_rng(NX * 100 + NZ)
+    six = rng.standard_normal((6, NZ, NXH, NY)) + 1j * rng.standard_normal((6, NZ, NXH, NY))
+    got = o.lamb(six)
+    S = [effective_spectrum(f, NXH, NZH, NZ) for f in six]
+    u, v, w, ox, oy, oz = S
+    H = [conv2(v, oz) - conv2(w, oy), conv2(w, ox) - conv2(u, oz), conv2(u, oy) - conv2(v, ox)]
+    ci, cm = 2 * NXH - 2, 2 * NZH - 2  # index of (i, m) = (0, 0) in the convolution
+    for q in range(3):
+        ref = np.zeros((NZ, NXH, NY), dtype=complex)
+        for k in range(NZ):
+            m = k if k < NZH else k - NZ
+            for i in range(NXH):
+                ref[k, i] = H[q][ci + i, cm + m]
+        err = np.abs(got[q] - ref).max() / np.abs(ref).max()
+        assert err < 1e-13, (q, err)
+
+
+def test_f2p_p2f_roundtrip_and_realness(oracle_mod):
+    o = make(oracle_mod, 16, 9, 16)
+    rng = np.random.default_rng(5)
+    F = rng.standard_normal(o.wave_shape) + 1j * rng.standard_normal(o.wave_shape)
+    P = o.f2p(F)
+    back = o.p2f(P)
+    # modes that survive the round trip: z-Nyquist dropped, kx = 0 Hermitian-projected
+    S = effective_spectrum(F, o.NXH, o.NZH, o.NZ)
+    expect = np.zeros_like(F)
+    for k in range(o.NZ):
+        m = k if k < o.NZH else k - o.NZ
+        if k == o.NZH:
+            continue
+        for i in range(o.NXH):
+            expect[k, i] = S[i + o.NXH - 1, m + o.NZH - 1]
+    assert np.abs(back - expect).max() < 1e-13 * np.abs(expect).max()
+
+
+# ---------------------------------------------------------------------------
+# dense LU (src/linalg.hpp:493-540, 615-689)
+# ---------------------------------------------------------------------------
+def test_lu_matches_lapack(oracle_mod):
+    import scipy.linalg as sla
+    rng = np.random.default_rng(0)
+    for n in (5, 17, 65, 129):
+        A = rng.standard_normal((n, n))
+        b = rng.standard_normal((n, 3))
+        x = oracle_mod.lu_solve(A, b)
+        ref = np.linalg.solve(A, b)
+        assert np.abs(x - ref).max() <= 1e-11 * np.abs(ref).max()
+        lu, piv, info = oracle_mod.lu_factor(A)
+        lu_ref, piv_ref = sla.lu_factor(A)
+        assert info == 0
+        assert np.array_equal(piv, piv_ref)  # same pivot rows as dgetrf
+        assert np.abs(lu - lu_ref).max() <= 1e-12 * np.abs(lu_ref).max()
+
+
+def test_lu_singular_reports_info(oracle_mod):
+    A = np.ones((4, 4))
+    with pytest.raises(np.linalg.LinAlgError):
+        oracle_mod.lu_solve(A, np.ones(4))
+
+
+def assemble_Av_Ag(o, k2):
+    """Av, Ag with boundary rows as written in src/solver.cpp:577-591."""
+    DY1, DY2, DY4 = o.matrix("DY1"), o.matrix("DY2"), o.matrix("DY4")
+    N, nu, dt, g0 = o.NY, o.nu, o.dt, 11 / 6
+    Av = -nu * DY4 + (g0 / dt + 2 * nu * k2) * DY2 - (nu * k2 ** 2 + g0 * k2 / dt) * np.eye(N)
+    Ag = -nu * DY2 + (nu * k2 + g0 / dt) * np.eye(N)
+    for M in (Av, Ag):
+        M[0] = 0
+        M[0, 0] = 1
+        M[-1] = 0
+        M[-1, -1] = 1
+    Av[1] = DY1[0]
+    Av[-2] = DY1[-1]
+    return Av, Ag
+
+
+def test_lu_on_step_operators_matches_lapack(oracle_mod):
+    o = make(oracle_mod, 8, 65, 8)
+    rng = np.random.default_rng(3)
+    for k2 in (0.0, 0.25, 17.0, 400.0):
+        for M in assemble_Av_Ag(o, k2):
+            b = rng.standard_normal(o.NY)
+            x = oracle_mod.lu_solve(M, b)
+            ref = np.linalg.solve(M, b)
+            # both are dgetrf/dgetrs; residual-level agreement on operators with cond ~ 1e8
+            assert np.linalg.norm(x - ref) <= 1e-9 * np.linalg.norm(ref)
+
+
+# ---------------------------------------------------------------------------
+# compact schemes and metrics (src/compdiff.hpp, src/solver.cpp:57-216)
+# ---------------------------------------------------------------------------
+def exactness_degrees(A, B, s, n):
+    h = s[1] - s[0]
+    Bh = B * h ** n
+    out = []
+    for i in range(len(s)):
+        t = (s - s[i]) / h
+        d = -1
+        for p in range(0, 14):
+            f = t ** p
+            fn = np.zeros_like(s) if p < n else factorial(p) / factorial(p - n) * t ** (p - n)
+            r = Bh[i] @ f - A[i] @ fn
+            scale = np.abs(Bh[i]) @ np.abs(f) + np.abs(A[i]) @ np.abs(fn)
+            if abs(r) > 1e-11 * scale:
+                break
+            d = p
+        out.append(d)
+    return out
+
+
+def test_compact_rows_exactness(oracle_mod):
+    """Degree p up to which row i satisfies B f = A f^(n) exactly for f = ((s - s_i)/h)^p.
+    Interior rows: Lele's pentadiagonal 8th-order schemes for d/ds, d2/ds2, d3/ds3
+    (degree 8 + n - 1) and the tridiagonal 6th-order scheme for d4/ds4 (degree 9).
+    Near-wall rows: the degrees their coefficient sets give (the reference's comments
+    overstate the order of the d3 and d4 closures). The last row of the third
+    derivative is not exact even for constants: the overwrite at
+    src/compdiff.hpp:257-258 leaves -e0/h^3 at column N-4 and nothing at N-5."""
+    o = make(oracle_mod, 8, 33, 8)
+    s = o.vector("s")
+    expected = {
+        1: [4, 4] + [8] * 29 + [4, 4],
+        2: [5, 7] + [9] * 29 + [7, 5],
+        3: [5, 6, 8] + [10] * 27 + [8, 6, -1],
+        4: [5, 6, 7] + [9] * 27 + [7, 6, 5],
+    }
+    for n in range(1, 5):
+        got = exactness_degrees(o.matrix(f"A{n}"), o.matrix(f"B{n}"), s, n)
+        assert got == expected[n], (n, got)
+
+
+def test_b3_top_row_signature(oracle_mod):
+    o = make(oracle_mod, 8, 33, 8)
+    h = o.vector("s")[1] - o.vector("s")[0]
+    B3 = o.matrix("B3") * h ** 3
+    N = o.NY
+    # bottom closure (8, -26, 30, -14, 2) annihilates constants; the top row as the
+    # reference leaves it is (-8, 26, -30, -2, 0) read from the wall inward
+    assert abs(B3[0].sum()) < 1e-12
+    np.testing.assert_allclose(B3[N - 1, N - 5:][::-1], [-8, 26, -30, -2, 0], atol=1e-12)
+    assert abs(B3[N - 1].sum() + 14.0) < 1e-12
+    # every other row is consistent
+    for i in range(N - 1):
+        assert abs(B3[i].sum()) < 1e-9 * np.abs(B3[i]).sum()
+
+
+def test_metric_matches_tanh_derivatives(oracle_mod):
+    o = make(oracle_mod, 8, 65, 8, alpha_mesh=2.0)
+    s = o.vector("s")
+    a = 2.0
+    th = np.tanh(a * s)
+    sech2 = 1 - th ** 2
+    # derivatives of y = tanh(a s)/tanh(a), differentiated independently
+    d1 = a * sech2
+    d2 = -2 * a ** 2 * sech2 * th
+    d3 = -2 * a ** 3 * sech2 * (sech2 - 2 * th ** 2)
+    d4 = 8 * a ** 4 * sech2 * th * (2 * sech2 - th ** 2)
+    for name, ref in (("dyds1", d1), ("dyds2", d2), ("dyds3", d3), ("dyds4", d4)):
+        got = o.vector(name)
+        np.testing.assert_allclose(got, ref / np.tanh(a), rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    np.testing.assert_allclose(o.vector("y"), np.tanh(a * s) / np.tanh(a), rtol=1e-15, atol=1e-15)
+
+
+def test_dy_operators_converge_on_stretched_grid(oracle_mod):
+    """DY_n f -> f^(n) for a smooth f(y); the error falls at high order with N.
+    DY4 is checked on the lower half only: its top-wall rows carry the D3 overwrite."""
+    f = lambda y: np.sin(2 * y) + y ** 3
+    df = [lambda y: 2 * np.cos(2 * y) + 3 * y ** 2, lambda y: -4 * np.sin(2 * y) + 6 * y,
+          lambda y: -8 * np.cos(2 * y) + 6, lambda y: 16 * np.sin(2 * y)]
+    errs = {}
+    for N in (33, 65, 129):
+        o = make(oracle_mod, 8, N, 8)
+        y = o.vector("y")
+        for n in (1, 2, 4):
+            D = o.matrix(f"DY{n}")
+            rows = slice(0, N) if n < 4 else slice(0, N // 2)
+            e = np.abs((D @ f(y) - df[n - 1](y))[rows]).max() / np.abs(df[n - 1](y)).max()
+            errs[(n, N)] = e
+    # the low-order near-wall closures (degree 4..5) set the rate: better than
+    # third order everywhere, and the absolute error shrinks to the closures' level
+    assert errs[(1, 129)] < 1e-6 and errs[(2, 129)] < 1e-5 and errs[(4, 129)] < 0.5
+    for n in (1, 2, 4):
+        assert errs[(n, 129)] < errs[(n, 65)] / 8 and errs[(n, 65)] < errs[(n, 33)] / 8, (n, errs)
+    # the overwrite makes DY4 wrong near the top wall
+    o = make(oracle_mod, 8, 65, 8)
+    y = o.vector("y")
+    top = np.abs((o.matrix("DY4") @ f(y) - df[3](y))[-3:]).max() / np.abs(df[3](y)).max()
+    assert top > 100 * errs[(4, 65)]
+
+
+# ---------------------------------------------------------------------------
+# step invariants and closed forms
+# ---------------------------------------------------------------------------
+def laminar_fields(o):
+    y = o.vector("y")
+    u = np.zeros(o.wave_shape, dtype=complex)
+    u[0, 0] = 1.5 * (1 - y ** 2)
+    z = np.zeros(o.wave_shape, dtype=complex)
+    return u, z, z
+
+
+def test_laminar_poiseuille_balance(oracle_mod):
+    """U = 1.5(1 - y^2): friction force 3 nu balances nu U''; the flow stays put and
+    the nonlinear terms vanish (src/solver.cpp:306-327, 428-514)."""
+    o = make(oracle_mod, 16, 65, 16)
+    u, v, w = laminar_fields(o)
+    o.set_velocity(u, v, w)
+    fx, bottom, upper = o.wall_shear()
+    # DY1 wall rows are 4th order: on this grid the parabola is differentiated to ~1e-5
+    assert abs(fx - 3 * o.nu) < 5e-5 * 3 * o.nu
+    assert abs(bottom + upper) < 1e-12 and abs(bottom - 3 * o.nu) < 5e-5 * 3 * o.nu
+    o.step(20)
+    U = o.field("u")[0, 0]
+    assert np.abs(U - u[0, 0]).max() < 1e-6
+    # zero up to the transforms' roundoff on a field that is constant in x and z
+    for name in ("v", "w", "Hv", "Hg", "lmbx", "lmbz", "omx", "omy"):
+        assert np.abs(o.field(name)).max() < 1e-14, name
+    d = o.diag()
+    assert d.shape == (21, 3)
+    assert np.all(np.abs(d[:, 0] - 3 * o.nu) < 5e-5 * 3 * o.nu)
+
+
+def test_seeded_init_properties(oracle_mod):
+    o = make(oracle_mod, 32, 33, 32)
+    o.init(2, seed=7, amp=0.15)
+    u, v, w = o.field("u"), o.field("v"), o.field("w")
+    y = o.vector("y")
+    up = u.copy()
+    up[0, 0] -= 1.5 * (1 - y ** 2)
+    P = np.stack([o.f2p(up), o.f2p(v), o.f2p(w)])
+    assert abs(np.sqrt((P ** 2).sum(0)).max() - 0.15) < 1e-12
+    # v and omega_y vanish at the walls exactly; u and w carry the truncation error
+    # of DY1 at the wall rows until the first solve imposes DY1 v = 0 there
+    assert np.abs(v[:, :, 0]).max() == 0 and np.abs(v[:, :, -1]).max() == 0
+    assert np.abs(o.field("omy")[:, :, [0, -1]]).max() < 1e-15
+    for f in (u, w):
+        assert np.abs(f[1:, :, [0, -1]]).max() < 1e-3 * 0.15 and np.abs(f[0, 1:, [0, -1]]).max() < 1e-3 * 0.15
+    # Hermitian kx = 0 plane
+    for m in range(1, o.NZH):
+        for f in (u, v, w):
+            assert np.abs(f[m, 0] - np.conj(f[o.NZ - m, 0])).max() < 1e-15
+
+
+def test_step_preserves_hermitian_plane_and_divergence(oracle_mod):
+    o = make(oracle_mod, 16, 33, 16)
+    o.init(2, seed=3, amp=0.3)
+    o.step(3)
+    DY1 = o.matrix("DY1")
+    kx, kz = o.vector("kx"), o.vector("kz")
+    u, v, w = o.field("u"), o.field("v"), o.field("w")
+    for name in ("u", "v", "w", "omx", "omy", "omz", "Hv", "Hg"):
+        f = o.field(name)
+        scale = np.abs(f).max()
+        for m in range(1, o.NZH):
+            assert np.abs(f[m, 0] - np.conj(f[o.NZ - m, 0])).max() < 1e-12 * scale, name
+    # i kx u + DY1 v + i kz w = 0 on every k^2 > 0 pencil (src/solver.cpp:532-533)
+    for k in range(o.NZ):
+        for i in range(o.NXH):
+            if kx[i] ** 2 + kz[k] ** 2 < 1e-8:
+                continue
+            d1v = DY1 @ v[k, i]
+            div = 1j * kx[i] * u[k, i] + d1v + 1j * kz[k] * w[k, i]
+            assert np.abs(div).max() <= 1e-12 * max(np.abs(d1v).max(), 1e-30)
+
+
+def test_seeded_gauss_is_standard_normal(oracle_mod):
+    g = np.array([oracle_mod.seeded_gauss(1, 0, i, m, p) for i in range(20) for m in range(-10, 10) for p in range(4)])
+    z = np.concatenate([g.real, g.imag])
+    assert abs(z.mean()) < 0.06 and abs(z.var() - 1) < 0.08
+    assert len(np.unique(np.round(z, 12))) == len(z)
