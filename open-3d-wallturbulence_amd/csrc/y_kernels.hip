@@ -128,21 +128,21 @@ __global__ __launch_bounds__(256) void k_y_rhs(Chan c) {
 
 // ---------------------------------------------------------------------------
 // Dense LU with partial pivoting on one matrix per block.
-// 512 threads as a 32 x 16 grid; thread (tr, tc) keeps rows tr + 32a and
-// columns tc + 16b in registers (RPT x CPT). Rows are never moved: the pivot
-// of step k is recorded and its row retires; the rank of a row is its pivot
-// step, which is its row index in LAPACK's permuted factors. Ties in |a| go to
-// the row with the smaller position in LAPACK's interchanged order.
+// 32 x TC threads; thread (tr, tc) keeps rows tr + 32a and columns tc + TC*b
+// in registers (RPT x CPT). Rows are never moved: the pivot of step k is
+// recorded and its row retires; the rank of a row is its pivot step, which is
+// its row index in LAPACK's permuted factors. Ties in |a| go to the row with
+// the smaller position in LAPACK's interchanged order.
 // Right-hand sides ride along as extra columns (forward substitution for free),
 // followed by a column-oriented back substitution.
 // ---------------------------------------------------------------------------
 enum LuMode { LU_PLAIN = 0, LU_MEAN = 1, LU_STORE = 2 };
 
-constexpr int LTR = 32, LTC = 16, LNT = LTR * LTC;
+constexpr int LTR = 32;
 
-template <int RPT>
-__global__ __launch_bounds__(LNT) void k_lu(Chan c, int mode) {
-    constexpr int CPT = 2 * RPT + 1;
+template <int RPT, int CPT, int LTC>
+__global__ __launch_bounds__(LTR *LTC) void k_lu(Chan c, int mode) {
+    constexpr int LNT = LTR * LTC;
     __shared__ double colval[2][kMaxNY];
     __shared__ double colmag[2][kMaxNY];
     __shared__ double rowbuf[2][kMaxNY + 8];
@@ -313,16 +313,25 @@ __global__ __launch_bounds__(LNT) void k_lu(Chan c, int mode) {
         // 4. multipliers and rank-1 update of the active rows
         const bool recip = fabs(pv) >= DBL_MIN;
         const double rinv = 1.0 / pv;
+        double l[RPT];
 #pragma unroll
         for (int a = 0; a < RPT; ++a) {
-            if (!((active >> a) & 1u)) continue;
             const int r = tr + LTR * a;
-            const double l = recip ? colval[bsel][r] * rinv : colval[bsel][r] / pv;
+            l[a] = 0.0;
+            if ((active >> a) & 1u) l[a] = recip ? colval[bsel][r] * rinv : colval[bsel][r] / pv;
+        }
 #pragma unroll
-            for (int b = 0; b < CPT; ++b) {
-                const int col = tc + LTC * b;
-                if (col > k && col < ncols) M[a][b] = fma(-l, rowbuf[bsel][col], M[a][b]);
-                else if (col == k) M[a][b] = l;
+        for (int b = 0; b < CPT; ++b) {
+            const int col = tc + LTC * b;
+            if (col > k && col < ncols) {
+                const double u = rowbuf[bsel][col];
+#pragma unroll
+                for (int a = 0; a < RPT; ++a)
+                    if ((active >> a) & 1u) M[a][b] = fma(-l[a], u, M[a][b]);
+            } else if (col == k) {
+#pragma unroll
+                for (int a = 0; a < RPT; ++a)
+                    if ((active >> a) & 1u) M[a][b] = l[a];
             }
         }
     }
@@ -388,13 +397,13 @@ __global__ __launch_bounds__(LNT) void k_lu(Chan c, int mode) {
         }
         __syncthreads();
 #pragma unroll
-        for (int a = 0; a < RPT; ++a) {
-            if (rank[a] >= k) continue;
-            const int r = tr + LTR * a;
+        for (int b = 0; b < CPT; ++b) {
+            const int col = tc + LTC * b;
+            if (col >= N && col < ncols) {
+                const double x = xval[bsel][col - N];
 #pragma unroll
-            for (int b = 0; b < CPT; ++b) {
-                const int col = tc + LTC * b;
-                if (col >= N && col < ncols) M[a][b] = fma(-colval[bsel][r], xval[bsel][col - N], M[a][b]);
+                for (int a = 0; a < RPT; ++a)
+                    if (rank[a] < k) M[a][b] = fma(-colval[bsel][tr + LTR * a], x, M[a][b]);
             }
         }
     }
@@ -823,27 +832,28 @@ static int blocks_tp(const Chan &c) { return (c.npencil + kTP - 1) / kTP; }
 
 void launch_y_rhs(const Chan &c, hipStream_t s) { hipLaunchKernelGGL(k_y_rhs<kTP>, dim3(blocks_tp(c)), dim3(256), 0, s, c); }
 
-#define OWT_RPT_DISPATCH(N, BODY)                                                                   \
-    do {                                                                                            \
-        const int rpt_ = ((N) + 31) / 32;                                                           \
-        switch (rpt_) {                                                                             \
-            case 1: { constexpr int RPT = 1; BODY; } break;                                         \
-            case 2: { constexpr int RPT = 2; BODY; } break;                                         \
-            case 3: { constexpr int RPT = 3; BODY; } break;                                         \
-            case 4: { constexpr int RPT = 4; BODY; } break;                                         \
-            case 5: { constexpr int RPT = 5; BODY; } break;                                         \
-            case 6: { constexpr int RPT = 6; BODY; } break;                                         \
-            case 7: { constexpr int RPT = 7; BODY; } break;                                         \
-            default: { constexpr int RPT = 8; BODY; } break;                                        \
-        }                                                                                           \
+// (rows per thread, columns per thread, column groups) by NY: the tile holds
+// NY rows and NY + 4 columns (right-hand sides) in registers
+#define OWT_LU_DISPATCH(N, BODY)                                                                   \
+    do {                                                                                           \
+        const int n_ = (N);                                                                        \
+        if (n_ <= 32) { constexpr int RPT = 1, CPT = 3, TC = 16; BODY; }                           \
+        else if (n_ <= 64) { constexpr int RPT = 2, CPT = 5, TC = 16; BODY; }                      \
+        else if (n_ <= 96) { constexpr int RPT = 3, CPT = 7, TC = 16; BODY; }                      \
+        else if (n_ <= 128) { constexpr int RPT = 4, CPT = 9, TC = 16; BODY; }                     \
+        else if (n_ <= 160) { constexpr int RPT = 5, CPT = 11, TC = 16; BODY; }                    \
+        else if (n_ <= 192) { constexpr int RPT = 6, CPT = 13, TC = 16; BODY; }                    \
+        else if (n_ <= 212) { constexpr int RPT = 7, CPT = 9, TC = 24; BODY; }                     \
+        else if (n_ <= 224) { constexpr int RPT = 7, CPT = 10, TC = 24; BODY; }                    \
+        else { constexpr int RPT = 8, CPT = 9, TC = 32; BODY; }                                    \
     } while (0)
 
 void launch_lu_plain(const Chan &c, hipStream_t s) {
-    OWT_RPT_DISPATCH(c.NY, hipLaunchKernelGGL(k_lu<RPT>, dim3(c.npencil, 2), dim3(LNT), 0, s, c, (int)LU_PLAIN));
+    OWT_LU_DISPATCH(c.NY, hipLaunchKernelGGL((k_lu<RPT, CPT, TC>), dim3(c.npencil, 2), dim3(32 * TC), 0, s, c, (int)LU_PLAIN));
 }
 
 void launch_lu_factor_store(const Chan &c, hipStream_t s) {
-    OWT_RPT_DISPATCH(c.NY, hipLaunchKernelGGL(k_lu<RPT>, dim3(c.nsys, 2), dim3(LNT), 0, s, c, (int)LU_STORE));
+    OWT_LU_DISPATCH(c.NY, hipLaunchKernelGGL((k_lu<RPT, CPT, TC>), dim3(c.nsys, 2), dim3(32 * TC), 0, s, c, (int)LU_STORE));
 }
 
 #define OWT_NQ_DISPATCH(N, BODY)                                      \
@@ -868,7 +878,7 @@ void launch_mean_rhs(const Chan &c, hipStream_t s) { hipLaunchKernelGGL(k_mean_r
 
 void launch_mean_solve(const Chan &c, bool plain, hipStream_t s) {
     if (plain)
-        OWT_RPT_DISPATCH(c.NY, hipLaunchKernelGGL(k_lu<RPT>, dim3(1), dim3(LNT), 0, s, c, (int)LU_MEAN));
+        OWT_LU_DISPATCH(c.NY, hipLaunchKernelGGL((k_lu<RPT, CPT, TC>), dim3(1), dim3(32 * TC), 0, s, c, (int)LU_MEAN));
     else
         OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_lu_apply_mean<NQ>, dim3(1), dim3(64), 0, s, c));
 }
