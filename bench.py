@@ -22,27 +22,23 @@ sys.path.insert(0, os.path.join(ROOT, "open-3d-wallturbulence_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def stage_bytes(W, I, es):
+def stage_bytes(W, I, es, N, nfac):
     """Algorithmic HBM bytes of each stage per launch: one read of every input
-    field and one write of every output field (operators excluded).
-    W, I: complex elements of a wave field and of a z-padded field;
-    es: bytes per complex element in the transform phase (16 FP64, 8 FP32)."""
+    field, one write of every output field, and the stored LU factors the solves
+    stream (nfac factor indices x 2 operators x N^2 doubles). W, I: complex
+    elements of a wave field and of a z-padded field; es: bytes per complex
+    element in the transform phase (16 FP64, 8 FP32)."""
     c = 16
+    factors = 2 * nfac * N * N * 8
     return {
-        "y_rhs": (12 * W + 2 * W) * c,
-        "solve": (2 * W + 2 * W) * c,
+        "solve": (12 * W + 2 * W) * c + factors,
         "horizontal": (2 * W + 4 * W) * c,
+        "mean": 0,
         "z_inverse": 6 * W * c + 6 * I * es,
         "x_lamb": 6 * I * es + 3 * I * es,
         "z_forward": 3 * I * es + 3 * W * es,
         "nonlinear": 3 * W * es + 2 * W * c,
     }
-
-
-def step_bytes_per_point(precision):
-    # canonical count of the step's dataflow: 59 wave-field passes in FP64
-    # (472 B/gridpoint), 42.5 W-equivalents with the FP32 transform phase (340 B)
-    return 472.0 if precision == 0 else 340.0
 
 
 def cpu_baseline(threads, budget_s=20.0):
@@ -90,8 +86,12 @@ def main():
     ch = owt.Channel(params=cfg, precision=owt.MIXED, method=owt.BATCHED)
     ch.init_fields(2, seed=0, amp=amp)
     G = ch.gridpoints
+    N = ch.NY
     W = ch.NZ * ch.NXH * ch.NY
     I = ch.NZ2 * ch.NXH * ch.NY
+    nsys = ch.NXH * (ch.NZ // 2 + 1)
+    npencil = ch.NZ * ch.NXH
+    stages = [k for k in owt.KERNELS if k != "factor"]
 
     def timed(steps, warm):
         if warm:
@@ -100,20 +100,24 @@ def main():
 
     print("[bench] mixed batched", file=sys.stderr, flush=True)
     t_mixed = timed(K, Wm)
-    kern = {}
-    for name in owt.KERNELS:
-        kern[name] = ch.time_kernel(name, 5)
+    kern = {name: ch.time_kernel(name, 5) for name in stages}
+    t_factor_once = ch.time_kernel("factor", 1)
     print("[bench] fp64 batched", file=sys.stderr, flush=True)
     ch.set_precision(owt.FP64)
     t_fp64 = timed(K, 1)
-    kern64 = {name: ch.time_kernel(name, 3) for name in owt.KERNELS}
+    kern64 = {name: ch.time_kernel(name, 3) for name in stages}
     print("[bench] fp64 plain", file=sys.stderr, flush=True)
     ch.set_method(owt.PLAIN)
     t_plain = timed(max(1, args.plain_steps), 0)
+    t_plain_factor = ch.time_kernel("factor", 1)
     ch.close()
 
+    sb = stage_bytes(W, I, 8, N, nsys)
+    sb64 = stage_bytes(W, I, 16, N, nsys)
+    step_bytes = sum(sb.values())
+    step_bytes64 = sum(sb64.values())
+    plain_bytes = sum(stage_bytes(W, I, 16, N, npencil).values()) + 2 * npencil * N * N * 8
     ns = t_mixed * 1e6 / G
-    sb = stage_bytes(W, I, 8)
     dom = max(kern, key=kern.get)
     achieved = sb[dom] / (kern[dom] * 1e-3) / 1e9
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -134,12 +138,16 @@ def main():
         "data": "synthetic (seeded divergence-free perturbation on the laminar profile)",
         "config": {"workload": f"{args.config}: {ch.NX}x{ch.NY}x{ch.NZ} channel, Re_tau 550 box 2pi x 2 x pi",
                    "method": "batched (factor once, graph-captured step)", "precision": "mixed"},
-        "hbm_gbps_step": round(step_bytes_per_point(1) * G / (t_mixed * 1e-3) / 1e9, 1),
+        "hbm_gbps_step": round(step_bytes / (t_mixed * 1e-3) / 1e9, 1),
+        "step_bytes": step_bytes,
         "fp64_batched_ns": round(t_fp64 * 1e6 / G, 5),
-        "fp64_batched_hbm_gbps_step": round(step_bytes_per_point(0) * G / (t_fp64 * 1e-3) / 1e9, 1),
+        "fp64_batched_hbm_gbps_step": round(step_bytes64 / (t_fp64 * 1e-3) / 1e9, 1),
         "fp64_plain_ns": round(t_plain * 1e6 / G, 5),
+        "fp64_plain_hbm_gbps_step": round(plain_bytes / (t_plain * 1e-3) / 1e9, 1),
         "speedup_mixed_batched_vs_plain_fp64": round(t_plain / t_mixed, 2),
         "speedup_mixed_vs_fp64_batched": round(t_fp64 / t_mixed, 3),
+        "factor_once_ms": round(t_factor_once, 3),
+        "plain_factor_ms_per_step": round(t_plain_factor, 3),
         "kernels_ms_mixed": {k: round(v, 4) for k, v in kern.items()},
         "kernels_ms_fp64": {k: round(v, 4) for k, v in kern64.items()},
         "roofline": roof,

@@ -113,12 +113,21 @@ int owt_lamb(owt_handle *h, const double *six, double *three);
 /* host-side operators, column-major N x N: 0..3 DY1..DY4 */
 int owt_get_matrix(owt_handle *h, int which, double *out);
 
+/* LU factor of the implicit operator op (0 Av, 1 Ag; src/solver.cpp:577-603)
+ * of the +-kz system sys = mz*NXH + i (mz in [0, NZH]) as the batched method
+ * stores it: lu N x N column-major in dgetrf's layout (unit-lower L and U,
+ * rows in pivot order) and perm[k] = original row of pivot k. No reference
+ * counterpart (the reference factors inside time_Integration); it exposes the
+ * factors for validation. */
+int owt_get_factor(owt_handle *h, int sys, int op, double *lu, int *perm);
+
 /* timing: enqueue nsteps steps on the handle's stream between two HIP events
  * and return the elapsed milliseconds (inputs stay resident on the device) */
 int owt_time_steps(owt_handle *h, int nsteps, double *ms);
 /* timing of one stage kernel launched reps times on the handle's stream:
- * 0 y_rhs, 1 implicit solves, 2 horizontal, 3 z inverse, 4 x phase,
- * 5 z forward, 6 nonlinear. Returns the mean milliseconds per launch. */
+ * 0 right-hand sides + implicit solves, 1 horizontal, 2 mean flow, 3 z inverse,
+ * 4 x phase, 5 z forward, 6 nonlinear, 7 LU factorization (the plain method
+ * refactors every step). Returns the mean milliseconds per launch. */
 int owt_time_kernel(owt_handle *h, int kernel, int reps, double *ms);
 
 #ifdef __cplusplus

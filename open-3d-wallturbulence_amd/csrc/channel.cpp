@@ -91,7 +91,6 @@ void Channel::alloc() {
     const size_t es = P_.precision == 0 ? sizeof(double2) : sizeof(float2);
     c.hist = (double2 *)dmalloc(12 * c.W * sizeof(double2));
     c.cur = (double2 *)dmalloc(4 * c.W * sizeof(double2));
-    c.src = (double2 *)dmalloc(2 * c.W * sizeof(double2));
     // transform buffers sized for FP64 so the precision can be switched in place
     c.lmb = dmalloc(3 * c.W * sizeof(double2));
     c.zbuf = dmalloc(6 * c.I * sizeof(double2));
@@ -105,7 +104,6 @@ void Channel::alloc() {
     c.kz = (const double *)dmalloc(c.NZ * sizeof(double));
     c.means = (double2 *)dmalloc(12 * N * sizeof(double2));
     c.fric = (double *)dmalloc(8 * sizeof(double));
-    c.mrhs = (double2 *)dmalloc(2 * N * sizeof(double2));
     c.msol = (double2 *)dmalloc(2 * N * sizeof(double2));
     c.ctrl = (int *)dmalloc(16 * sizeof(int));
     c.diag_cap = 1 << 16;
@@ -116,8 +114,6 @@ void Channel::alloc() {
     c.twzf = (const float2 *)dmalloc(c.NZ2 * sizeof(float2));
     c.px = make_plan(c.NX2);
     c.pz = make_plan(c.NZ2);
-    c.LUv = c.LUg = nullptr;
-    c.permv = c.permg = nullptr;
     OWT_HIP(hipMemsetAsync(c.hist, 0, 12 * c.W * sizeof(double2), s_));
     OWT_HIP(hipMemsetAsync(c.cur, 0, 4 * c.W * sizeof(double2), s_));
     OWT_HIP(hipMemsetAsync(c.lmb, 0, 3 * c.W * sizeof(double2), s_));
@@ -151,20 +147,36 @@ void Channel::upload_operators() {
     OWT_HIP(hipStreamSynchronize(s_));
 }
 
-// The operators Av(k^2), Ag(k^2) do not change in time: factor each distinct
-// k^2 once (the reference refactors every pencil every step).
+// The operators Av(k^2), Ag(k^2) do not change in time: factor each +-kz
+// system once (the reference refactors every pencil every step).
 void Channel::factor_operators() {
     if (factored_) return;
     const size_t N = c_.NY, ns = c_.nsys;
-    c_.LUv = (double *)dmalloc(ns * N * N * sizeof(double));
-    c_.LUg = (double *)dmalloc(ns * N * N * sizeof(double));
-    c_.permv = (int *)dmalloc(ns * N * sizeof(int));
-    c_.permg = (int *)dmalloc(ns * N * sizeof(int));
-    launch_lu_factor_store(c_, s_);
+    if (!Fb_) {
+        Fb_ = (double *)dmalloc(2 * ns * N * N * sizeof(double));
+        Pb_ = (int *)dmalloc(2 * ns * N * sizeof(int));
+    }
+    launch_factor(c_, Fb_, Pb_, c_.nsys, s_);
     OWT_HIP(hipGetLastError());
     OWT_HIP(hipStreamSynchronize(s_));
     check_errors();
     factored_ = true;
+}
+
+void Channel::get_factor(int sys, int op, double *lu, int *perm) {
+    if (sys < 0 || sys >= c_.nsys || op < 0 || op > 1) throw Error(2, "factor index out of range");
+    factor_operators();
+    const size_t N = c_.NY, f = (size_t)sys * 2 + op;
+    OWT_HIP(hipMemcpy(lu, Fb_ + f * N * N, N * N * sizeof(double), hipMemcpyDeviceToHost));
+    OWT_HIP(hipMemcpy(perm, Pb_ + f * N, N * sizeof(int), hipMemcpyDeviceToHost));
+}
+
+// the plain method's per-pencil factor set (refilled every step)
+void Channel::ensure_plain_factors() {
+    if (Fp_) return;
+    const size_t N = c_.NY, np = c_.npencil;
+    Fp_ = (double *)dmalloc(2 * np * N * N * sizeof(double));
+    Pp_ = (int *)dmalloc(2 * np * N * sizeof(int));
 }
 
 void Channel::check_errors() {
@@ -178,10 +190,12 @@ void Channel::check_errors() {
 }
 
 void Channel::drop_graph() {
-    if (gexec_) (void)hipGraphExecDestroy(gexec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
-    gexec_ = nullptr;
-    graph_ = nullptr;
+    for (int r = 0; r < 3; ++r) {
+        if (gexec_[r]) (void)hipGraphExecDestroy(gexec_[r]);
+        if (graph_[r]) (void)hipGraphDestroy(graph_[r]);
+        gexec_[r] = nullptr;
+        graph_[r] = nullptr;
+    }
 }
 
 void Channel::set_method(int method) {
@@ -193,6 +207,7 @@ void Channel::set_method(int method) {
 void Channel::set_precision(int precision) {
     if (precision != 0 && precision != 1) throw Error(2, "precision must be 0 (fp64) or 1 (mixed)");
     if (precision == P_.precision) return;
+    sync();
     // the Lamb vector is kept in the transform precision: convert it in place
     std::vector<double> l(3 * c_.W * 2);
     if (P_.precision == 0) {
@@ -411,6 +426,7 @@ void Channel::set_velocity(const double *u, const double *v, const double *w) {
     OWT_HIP(hipMemcpyAsync(c_.cur + W, w, W * sizeof(double2), hipMemcpyHostToDevice, s_));
     OWT_HIP(hipMemcpyAsync(c_.hist + (size_t)(H_V * 3) * W, v, W * sizeof(double2), hipMemcpyHostToDevice, s_));
     host_steps_ = 0;
+    ring_ = 0;
     init_post();
 }
 
@@ -423,7 +439,8 @@ void Channel::init_post() {
     launch_z_inverse(c_, six, 6, c_.zbuf, s_);
     launch_x_lamb(c_, s_);
     launch_z_forward(c_, s_);
-    launch_nonlinear(c_, s_);
+    // Hv, Hg of the initial field land in slot 1 (the new slot of ring phase 0)
+    launch_nonlinear(c_, step_ptrs(0), s_);
     launch_init_history(c_, s_);
     launch_init_mean(c_, s_);
     OWT_HIP(hipGetLastError());
@@ -433,49 +450,94 @@ void Channel::init_post() {
 // ---------------------------------------------------------------------------
 // time step
 // ---------------------------------------------------------------------------
-void Channel::enqueue_step_body() {
-    const bool plain = P_.method == 0;
-    launch_y_rhs(c_, s_);
-    if (plain) launch_lu_plain(c_, s_);
-    else launch_lu_apply(c_, 0, s_);
-    launch_horizontal(c_, s_);
-    launch_mean_rhs(c_, s_);
-    launch_mean_solve(c_, plain, s_);
-    launch_mean_post(c_, s_);
-    // the six transformed fields; v and omega_y are read from the history slot
-    // the solves just wrote, found on the device from the ring counter
-    launch_z_inverse(c_, nullptr, -6, c_.zbuf, s_);
-    launch_x_lamb(c_, s_);
-    launch_z_forward(c_, s_);
-    launch_nonlinear(c_, s_);
-    launch_advance(c_, s_);
+StepPtrs Channel::step_ptrs(int ring) const {
+    StepPtrs sp{};
+    const size_t W = c_.W;
+    auto slot = [&](int f, int sl) { return c_.hist + (size_t)(f * 3 + sl) * W; };
+    for (int l = 0; l < 3; ++l) {
+        const int sl = (ring - l + 3) % 3;
+        sp.s[l] = sl;
+        sp.v[l] = slot(H_V, sl);
+        sp.g[l] = slot(H_G, sl);
+        sp.hv[l] = slot(H_HV, sl);
+        sp.hg[l] = slot(H_HG, sl);
+    }
+    sp.sn = (ring + 1) % 3;
+    sp.vn = slot(H_V, sp.sn);
+    sp.gn = slot(H_G, sp.sn);
+    sp.hvn = slot(H_HV, sp.sn);
+    sp.hgn = slot(H_HG, sp.sn);
+    return sp;
 }
 
-void Channel::build_graph() {
-    drop_graph();
+// time_Integration (src/solver.cpp:541-627) for the ring phase `ring`
+void Channel::enqueue_step_body(int ring) {
+    const bool plain = P_.method == 0;
+    const StepPtrs sp = step_ptrs(ring);
+    const double *F = plain ? Fp_ : Fb_;
+    const int *P = plain ? Pp_ : Pb_;
+    if (plain) launch_factor(c_, Fp_, Pp_, c_.npencil, s_);
+    launch_solve(c_, sp, F, P, plain, s_);
+    launch_horizontal(c_, sp, s_);
+    launch_mean(c_, sp, F, P, s_);
+    // the six transformed fields: u, v, w, omega_x, omega_y, omega_z of the new level
+    const size_t W = c_.W;
+    const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
+    launch_z_inverse(c_, six, 6, c_.zbuf, s_);
+    launch_x_lamb(c_, s_);
+    launch_z_forward(c_, s_);
+    launch_nonlinear(c_, sp, s_);
+}
+
+void Channel::build_graph(int ring) {
     OWT_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
     try {
-        enqueue_step_body();
+        enqueue_step_body(ring);
     } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(s_, &g);
         if (g) (void)hipGraphDestroy(g);
         throw;
     }
-    OWT_HIP(hipStreamEndCapture(s_, &graph_));
-    OWT_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+    OWT_HIP(hipStreamEndCapture(s_, &graph_[ring]));
+    OWT_HIP(hipGraphInstantiate(&gexec_[ring], graph_[ring], nullptr, nullptr, 0));
 }
 
 void Channel::enqueue_step() {
     if (P_.method == 1) {
         factor_operators();
-        if (!gexec_) build_graph();
-        OWT_HIP(hipGraphLaunch(gexec_, s_));
+        if (!gexec_[ring_]) build_graph(ring_);
+        OWT_HIP(hipGraphLaunch(gexec_[ring_], s_));
     } else {
-        enqueue_step_body();
+        ensure_plain_factors();
+        enqueue_step_body(ring_);
         OWT_HIP(hipGetLastError());
     }
+    ring_ = (ring_ + 1) % 3;
     ++host_steps_;
+}
+
+void Channel::enqueue_kernel(int id) {
+    const bool plain = P_.method == 0;
+    if (plain) ensure_plain_factors();
+    else factor_operators();
+    const StepPtrs sp = step_ptrs(ring_);
+    const double *F = plain ? Fp_ : Fb_;
+    const int *P = plain ? Pp_ : Pb_;
+    const size_t W = c_.W;
+    const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
+    switch (id) {
+        case 0: launch_solve(c_, sp, F, P, plain, s_); break;
+        case 1: launch_horizontal(c_, sp, s_); break;
+        case 2: launch_mean(c_, sp, F, P, s_); break;
+        case 3: launch_z_inverse(c_, six, 6, c_.zbuf, s_); break;
+        case 4: launch_x_lamb(c_, s_); break;
+        case 5: launch_z_forward(c_, s_); break;
+        case 6: launch_nonlinear(c_, sp, s_); break;
+        case 7: launch_factor(c_, plain ? Fp_ : Fb_, plain ? Pp_ : Pb_, plain ? c_.npencil : c_.nsys, s_); break;
+        default: throw Error(2, "kernel id must be 0..7");
+    }
+    OWT_HIP(hipGetLastError());
 }
 
 void Channel::sync() { OWT_HIP(hipStreamSynchronize(s_)); }
@@ -496,6 +558,7 @@ void Channel::step(int n) {
 }
 
 long Channel::steps_done() {
+    sync();
     int ctrl[2];
     OWT_HIP(hipMemcpy(ctrl, c_.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
     return ctrl[C_STEP];
@@ -507,9 +570,7 @@ long Channel::steps_done() {
 void Channel::get_field(int f, double *out) {
     if (f < 0 || f >= F_COUNT) throw Error(2, "unknown field id " + std::to_string(f));
     sync();
-    int ctrl[2];
-    OWT_HIP(hipMemcpy(ctrl, c_.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
-    const int r = ctrl[C_RING];
+    const int r = ring_;
     const size_t W = c_.W;
     const double2 *src = nullptr;
     switch (f) {
@@ -665,7 +726,10 @@ void Channel::checkpoint(const std::string &path) {
     dump(c_.lmb, 3 * W * (c_.precision == 0 ? sizeof(double2) : sizeof(float2)));
     dump(c_.means, 12 * N * sizeof(double2));
     dump(c_.fric, 8 * sizeof(double));
-    dump(c_.ctrl, 16 * sizeof(int));
+    int ctrl[16];
+    OWT_HIP(hipMemcpy(ctrl, c_.ctrl, sizeof(ctrl), hipMemcpyDeviceToHost));
+    ctrl[C_RING] = ring_;
+    f.write((const char *)ctrl, sizeof(ctrl));
     dump(c_.diag, (size_t)c_.diag_cap * 3 * sizeof(double));
     if (!f) throw Error(5, "write failed: " + path);
 }
@@ -693,7 +757,12 @@ void Channel::restore(const std::string &path) {
     load(c_.lmb, 3 * W * (c_.precision == 0 ? sizeof(double2) : sizeof(float2)));
     load(c_.means, 12 * N * sizeof(double2));
     load(c_.fric, 8 * sizeof(double));
-    load(c_.ctrl, 16 * sizeof(int));
+    int ctrl[16];
+    f.read((char *)ctrl, sizeof(ctrl));
+    if (!f || ctrl[C_RING] < 0 || ctrl[C_RING] > 2) throw Error(5, "corrupt checkpoint " + path);
+    ring_ = ctrl[C_RING];
+    ctrl[C_RING] = 0;
+    OWT_HIP(hipMemcpy(c_.ctrl, ctrl, sizeof(ctrl), hipMemcpyHostToDevice));
     load(c_.diag, (size_t)c_.diag_cap * 3 * sizeof(double));
 }
 

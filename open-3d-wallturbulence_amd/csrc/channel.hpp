@@ -47,6 +47,12 @@ class Channel {
 
     // device-side timing helpers (bench): run one step on the stream without syncing
     void enqueue_step();
+    // one launch of a single stage of the step at the current ring phase:
+    // 0 solve, 1 horizontal, 2 mean, 3 z inverse, 4 x phase, 5 z forward,
+    // 6 nonlinear, 7 factorization
+    void enqueue_kernel(int id);
+    // stored factor of one +-kz system (batched factor set)
+    void get_factor(int sys, int op, double *lu, int *perm);
     void sync();
 
   private:
@@ -54,9 +60,11 @@ class Channel {
     void free_all();
     void upload_operators();
     void factor_operators();
+    void ensure_plain_factors();
     void init_post();
-    void enqueue_step_body();
-    void build_graph();
+    StepPtrs step_ptrs(int ring) const;
+    void enqueue_step_body(int ring);
+    void build_graph(int ring);
     void drop_graph();
     void check_errors();
     void *dmalloc(size_t bytes);
@@ -69,9 +77,15 @@ class Channel {
     bool verbose_ = false;
     hipStream_t s_ = nullptr;
     std::vector<void *> allocs_;
+    // factor sets (lu_kernels.hip): batched, one per +-kz system, computed once;
+    // plain, one per pencil, recomputed every step
+    double *Fb_ = nullptr, *Fp_ = nullptr;
+    int *Pb_ = nullptr, *Pp_ = nullptr;
     bool factored_ = false;
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t gexec_ = nullptr;
+    // history ring phase: slot of the current level; one captured graph per phase
+    int ring_ = 0;
+    hipGraph_t graph_[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t gexec_[3] = {nullptr, nullptr, nullptr};
     long host_steps_ = 0;
 };
 

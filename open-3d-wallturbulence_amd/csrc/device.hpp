@@ -24,7 +24,7 @@ enum HistField { H_V = 0, H_G = 1, H_HV = 2, H_HG = 3 };
 // mean-flow history indices inside Chan::means ([quantity][slot][NY])
 enum MeanField { M_LMBX = 0, M_LMBZ = 1, M_VELX = 2, M_VELZ = 3 };
 // control words inside Chan::ctrl
-enum Ctrl { C_RING = 0, C_STEP = 1, C_ERR = 2 };
+enum Ctrl { C_RING = 0, C_STEP = 1, C_ERR = 2 };  // C_RING is written only into checkpoints
 // error bits
 enum ErrBits { E_SINGULAR = 1 };
 
@@ -39,23 +39,27 @@ struct Chan {
     double2 *hist;  // 12 W: v, omega_y, Hv, Hg at 3 levels
     double2 *cur;   // 4 W: u, w, omega_x, omega_z
     void *lmb;      // 3 W of double2 or float2
-    double2 *src;   // 2 W: right-hand sides of the v and omega_y systems
     void *zbuf;     // 6 I of double2 or float2 (z-inverse output)
     void *xbuf;     // 3 I of double2 or float2 (x-phase output)
     const double *DY1, *DY2, *DY4;  // N x N column-major
     const double *kx, *kz;
     double2 *means;  // 4 x 3 x NY
     double *fric;    // fx[3], fz[3]
-    double2 *mrhs;   // 2 x NY mean-flow right-hand sides
     double2 *msol;   // 2 x NY mean-flow solutions
     int *ctrl;
     double *diag;    // diag_cap x 3: fx_fric, bottom, upper per friction evaluation
     int diag_cap;
-    double *LUv, *LUg;   // nsys x N x N factors (rows in pivot order), column-major
-    int *permv, *permg;  // nsys x N pivot rows
     const double2 *twx, *twz;  // e^{-2 pi i m / L}, fp64
     const float2 *twxf, *twzf; // same, fp32
     FFTPlan px, pz;
+};
+
+// History slots of one step: level 0 is the current field, levels 1 and 2 the
+// two before it; the new level overwrites level 2 (slot sn = s[2]).
+struct StepPtrs {
+    const double2 *v[3], *g[3], *hv[3], *hg[3];
+    double2 *vn, *gn, *hvn, *hgn;
+    int s[3], sn;
 };
 
 // ---- FFT phase launches (fft_kernels.hip) ----
@@ -74,17 +78,15 @@ void launch_z_forward_to(const Chan &c, const void *in, int nf, double2 *out, hi
 // max over the physical grid of u^2+v^2+w^2 (three consecutive fields), stored as bits
 void launch_peak3(const double *phys, size_t n, unsigned long long *out, hipStream_t s);
 
-// ---- wall-normal phase launches (y_kernels.hip) ----
-void launch_y_rhs(const Chan &c, hipStream_t s);
-void launch_lu_plain(const Chan &c, hipStream_t s);           // per-step factor + solve (reference method)
-void launch_lu_factor_store(const Chan &c, hipStream_t s);    // factor once per distinct k^2
-void launch_lu_apply(const Chan &c, int grid, hipStream_t s); // persistent solve with stored factors
-void launch_horizontal(const Chan &c, hipStream_t s);
-void launch_mean_rhs(const Chan &c, hipStream_t s);
-void launch_mean_solve(const Chan &c, bool plain, hipStream_t s);
-void launch_mean_post(const Chan &c, hipStream_t s);
-void launch_nonlinear(const Chan &c, hipStream_t s);
-void launch_advance(const Chan &c, hipStream_t s);
+// ---- wall-normal phase launches (lu_kernels.hip, y_kernels.hip) ----
+// LU factors of Av and Ag for nfac factor indices (see lu_kernels.hip)
+void launch_factor(const Chan &c, double *F, int *P, int nfac, hipStream_t s);
+// right-hand sides + solves; plain: one job per pencil, else one per +-kz system
+void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, bool plain, hipStream_t s);
+void launch_horizontal(const Chan &c, const StepPtrs &sp, hipStream_t s);
+// mean flow, friction force and the k = 0 pencil (factor 0 holds Ag(0))
+void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *P, hipStream_t s);
+void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s);
 // initialisation helpers (src/solver.cpp:379-425)
 void launch_vor_from_vel(const Chan &c, hipStream_t s);
 void launch_init_history(const Chan &c, hipStream_t s);

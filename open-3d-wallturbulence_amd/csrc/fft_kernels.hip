@@ -145,12 +145,9 @@ __device__ __forceinline__ const float2 *tw_of<float2>(const Chan &c, bool xdir)
 // ---------------------------------------------------------------------------
 // z inverse (src/FFT.cpp:68-95): per (kx index i, y tile, field)
 // ---------------------------------------------------------------------------
-// Input fields of the z inverse. With ring set, fields 1 and 4 are v and
-// omega_y in the history slot written by this step's solves, located from the
-// device-side ring counter so that one captured graph serves every step.
+// Input fields of the z inverse
 struct PtrTable {
     const double2 *p[6];
-    int ring;
 };
 
 template <typename T, int TJ, int TL, int LMAX>
@@ -167,10 +164,6 @@ __global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typen
     const C *twg = tw_of<C>(c, false);
     for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = twg[n];
     const double2 *src = in.p[f];
-    if (in.ring && (f == 1 || f == 4)) {
-        const int ns = (c.ctrl[C_RING] + 1) % 3;
-        src = c.hist + (size_t)((f == 1 ? H_V : H_G) * 3 + ns) * c.W;
-    }
     for (int n = tl; n < L; n += TL) {
         // slot n <- mode n (n < NZH); slots NZH..NZ are padding and the dropped z-Nyquist
         const int mode = n < NZH ? n : (n <= NZ ? -1 : n - NZH);
@@ -445,17 +438,7 @@ size_t smem_bytes() {
 
 void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out, hipStream_t s) {
     PtrTable t{};
-    if (!in) {
-        // step mode: u, v, w, omega_x, omega_y, omega_z of the current step
-        nf = 6;
-        t.ring = 1;
-        t.p[0] = c.cur;
-        t.p[2] = c.cur + c.W;
-        t.p[3] = c.cur + 2 * c.W;
-        t.p[5] = c.cur + 3 * c.W;
-    } else {
-        for (int f = 0; f < nf; ++f) t.p[f] = in[f];
-    }
+    for (int f = 0; f < nf; ++f) t.p[f] = in[f];
     OWT_LEN_DISPATCH(c.NZ2, {
         dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, nf);
         if (c.precision == 0)

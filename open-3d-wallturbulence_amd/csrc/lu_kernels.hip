@@ -1,0 +1,340 @@
+// Implicit wall-normal solves of the v and omega_y equations
+// (reference src/solver.cpp:568-603).
+//
+//   Av(k^2) = -nu DY4 + (gamma0/dt + 2 nu k^2) DY2 - (nu k^4 + gamma0 k^2/dt) I
+//             rows 0, N-1: v = 0;  rows 1, N-2: DY1 v = 0 (rows 0 and N-1 of DY1)
+//   Ag(k^2) = -nu DY2 + (nu k^2 + gamma0/dt) I,  rows 0, N-1: omega_y = 0
+//
+// k_factor  LU with partial pivoting of one operator per block, LAPACK's pivot
+//           rule (first largest |a| in the interchanged row order, reciprocal
+//           scaling when |pivot| >= DBL_MIN; src/linalg.hpp:600-650 calls dgetrf).
+//           Factors are stored with rows in pivot order, as dgetrf leaves them.
+// k_solve   right-hand sides from the three history levels, then forward and
+//           back substitution with the stored factors (dgetrs).
+//
+// A factor set holds, for factor index f and operator op (0 = Av, 1 = Ag),
+// N*N doubles at F + (2f + op) N^2 (column-major) and N pivot rows at
+// P + (2f + op) N. The method decides what f is:
+//   plain   f = pencil k*NXH + i: every pencil factored every step, as the
+//           reference does;
+//   batched f = system mz*NXH + i, mz in [0, NZH]: the operators depend on k^2
+//           only, so +kz and -kz share one factor, computed once.
+// In both cases f / NXH and f % NXH index kz and kx, and factor 0 is the
+// k = 0 pencil whose Ag is the mean-flow operator.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "device.hpp"
+#include "y_common.hpp"
+
+namespace owt {
+
+// ---------------------------------------------------------------------------
+// Factorization. TC waves per block; lane l holds rows l + 64a (a < RPT) and
+// wave w holds columns w + TC*b (b < CPT), so every wave holds whole columns
+// and the pivot search of a column is a wave reduction. Rows are never moved:
+// the pivot of step k is recorded and its row retires; the pivot step of a row
+// is its row index in the stored (interchanged) factor.
+// ---------------------------------------------------------------------------
+template <int RPT, int CPT, int TC>
+__global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P) {
+    constexpr int NR = 64 * RPT;
+    __shared__ double lbuf[2][NR];
+    __shared__ int pos[NR], rowAt[NR];
+    __shared__ int pivrow[2];
+    __shared__ int sing;
+
+    const int N = c.NY;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int f = blockIdx.x, op = blockIdx.y;
+    const double k2 = k2_of(c, f / c.NXH, f % c.NXH);
+    const double nu = c.nu, dt = c.dt;
+    const double k4 = mul(k2, k2);
+    const double s1 = add(__ddiv_rn(G0, dt), mul(mul(2.0, nu), k2));
+    const double s2 = add(mul(nu, k4), __ddiv_rn(mul(G0, k2), dt));
+    const double sg = add(mul(nu, k2), __ddiv_rn(G0, dt));
+
+    // assemble (src/solver.cpp:577-591, 601-603)
+    double M[RPT][CPT];
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) {
+        const int r = lane + 64 * a;
+#pragma unroll
+        for (int b = 0; b < CPT; ++b) {
+            const int col = w + TC * b;
+            double v = 0.0;
+            if (r < N && col < N) {
+                const size_t e = (size_t)col * N + r;
+                if (r == 0 || r == N - 1) {
+                    v = (r == col) ? 1.0 : 0.0;
+                } else if (op == 0) {
+                    if (r == 1) v = c.DY1[(size_t)col * N];
+                    else if (r == N - 2) v = c.DY1[(size_t)col * N + N - 1];
+                    else {
+                        v = add(mul(-nu, c.DY4[e]), mul(s1, c.DY2[e]));
+                        if (r == col) v = sub(v, s2);
+                    }
+                } else {
+                    v = mul(-nu, c.DY2[e]);
+                    if (r == col) v = add(v, sg);
+                }
+            }
+            M[a][b] = v;
+        }
+    }
+    for (int r = threadIdx.x; r < NR; r += 64 * TC) {
+        pos[r] = r;
+        rowAt[r] = r;
+    }
+    if (threadIdx.x == 0) sing = 0;
+    unsigned active = 0;
+#pragma unroll
+    for (int a = 0; a < RPT; ++a)
+        if (lane + 64 * a < N) active |= 1u << a;
+    double *Fo = F + ((size_t)f * 2 + op) * N * N;
+    int *Po = P + ((size_t)f * 2 + op) * N;
+    __syncthreads();
+
+#pragma unroll
+    for (int bk = 0; bk < CPT; ++bk) {
+        for (int wk = 0; wk < TC; ++wk) {
+            const int k = bk * TC + wk;
+            if (k >= N) break;
+            const int sel = k & 1;
+            if (w == wk) {
+                // pivot: largest |a| among active rows, first in the interchanged order
+                double best = -1.0;
+                int bpos = INT_MAX, brow = -1;
+#pragma unroll
+                for (int a = 0; a < RPT; ++a) {
+                    if ((active >> a) & 1u) {
+                        const int r = lane + 64 * a;
+                        const double m = fabs(M[a][bk]);
+                        const int ps = pos[r];
+                        if (m > best || (m == best && ps < bpos)) {
+                            best = m;
+                            bpos = ps;
+                            brow = r;
+                        }
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ob = __shfl_xor(best, off);
+                    const int opos = __shfl_xor(bpos, off), orow = __shfl_xor(brow, off);
+                    if (ob > best || (ob == best && opos < bpos)) {
+                        best = ob;
+                        bpos = opos;
+                        brow = orow;
+                    }
+                }
+                int p = __builtin_amdgcn_readfirstlane(brow);
+                // a column without a finite candidate: take the row at position k
+                // (always active) so that every index stays valid; flagged below
+                if (p < 0) p = __builtin_amdgcn_readfirstlane(rowAt[k]);
+                const int pl = p & 63, pa = p >> 6;
+                double pv = M[0][bk];
+#pragma unroll
+                for (int a = 1; a < RPT; ++a)
+                    if (a == pa) pv = M[a][bk];
+                pv = readlane_d(pv, pl);
+                if (lane == 0) {
+                    const int q = pos[p], rk = rowAt[k];
+                    rowAt[q] = rk;
+                    pos[rk] = q;
+                    rowAt[k] = p;
+                    pos[p] = k;
+                    pivrow[sel] = p;
+                    Po[k] = p;
+                    if (!(pv != 0.0) || brow < 0) sing = 1;
+                }
+                if (lane == pl) active &= ~(1u << pa);
+                // multipliers (dgetf2: scale by the reciprocal unless |pivot| < sfmin)
+                const bool recip = fabs(pv) >= DBL_MIN;
+                const double rinv = 1.0 / pv;
+#pragma unroll
+                for (int a = 0; a < RPT; ++a) {
+                    if ((active >> a) & 1u) {
+                        const double l = recip ? M[a][bk] * rinv : M[a][bk] / pv;
+                        M[a][bk] = l;
+                        lbuf[sel][lane + 64 * a] = l;
+                    }
+                }
+            }
+            __syncthreads();
+            const int p = __builtin_amdgcn_readfirstlane(pivrow[sel]);
+            const int pl = p & 63, pa = p >> 6;
+            if (lane == pl) active &= ~(1u << pa);
+            double l[RPT];
+#pragma unroll
+            for (int a = 0; a < RPT; ++a) l[a] = ((active >> a) & 1u) ? lbuf[sel][lane + 64 * a] : 0.0;
+            // rank-1 update of the trailing columns (dger)
+#pragma unroll
+            for (int b = bk; b < CPT; ++b) {
+                const int col = w + TC * b;
+                if (b == bk && w <= wk) continue;
+                if (col >= N) continue;
+                double u = M[0][b];
+#pragma unroll
+                for (int a = 1; a < RPT; ++a)
+                    if (a == pa) u = M[a][b];
+                u = readlane_d(u, pl);
+#pragma unroll
+                for (int a = 0; a < RPT; ++a)
+                    if ((active >> a) & 1u) M[a][b] = fma(-l[a], u, M[a][b]);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && sing) atomicOr(c.ctrl + C_ERR, E_SINGULAR);
+    // the final position of a row in the interchanged order is its pivot step
+#pragma unroll
+    for (int a = 0; a < RPT; ++a) {
+        const int r = lane + 64 * a;
+        if (r >= N) continue;
+        int rk = pos[r];
+        if ((unsigned)rk >= (unsigned)N) rk = 0;
+#pragma unroll
+        for (int b = 0; b < CPT; ++b) {
+            const int col = w + TC * b;
+            if (col < N) Fo[(size_t)col * N + rk] = M[a][b];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Right-hand sides (src/solver.cpp:568-574, 593-599) and dgetrs.
+// Four waves per block: wave w takes job 2*block + w/2 and operator w%2.
+// A batched job carries the +kz and -kz pencils (four real right-hand sides),
+// a plain job one pencil.
+// ---------------------------------------------------------------------------
+template <int NQ>
+__global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P, int plain) {
+    __shared__ double sh[4][4][64 * NQ];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int job = blockIdx.x * 2 + (w >> 1), op = w & 1;
+    const int N = c.NY;
+    const int njobs = plain ? c.npencil : c.nsys;
+    const bool live = job < njobs;
+    const int kz = live ? job / c.NXH : 0, ix = live ? job % c.NXH : 0;
+    const int p0 = job;
+    const int p1 = (live && !plain && kz > 0 && kz < c.NZH) ? (c.NZ - kz) * c.NXH + ix : -1;
+    double(*S)[64 * NQ] = sh[w];
+
+    const double2 *X0 = op == 0 ? sp.v[0] : sp.g[0], *X1 = op == 0 ? sp.v[1] : sp.g[1], *X2 = op == 0 ? sp.v[2] : sp.g[2];
+    const double2 *H0 = op == 0 ? sp.hv[0] : sp.hg[0], *H1 = op == 0 ? sp.hv[1] : sp.hg[1], *H2 = op == 0 ? sp.hv[2] : sp.hg[2];
+    double2 *XN = op == 0 ? sp.vn : sp.gn;
+    const double dt = c.dt;
+    const double k2 = k2_of(c, kz, ix);
+    const double k2dt = __ddiv_rn(k2, dt);
+
+    // 1. combine the history levels
+    double t[4][NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int row = lane + 64 * q;
+#pragma unroll
+        for (int pp = 0; pp < 2; ++pp) {
+            double2 a = c2(0, 0), h = c2(0, 0);
+            const int pn = pp == 0 ? p0 : p1;
+            if (live && row < N && pn >= 0) {
+                const size_t o = (size_t)pn * N + row;
+                a = caddc(caddc(cscale(X0[o], AL0), cscale(X1[o], AL1)), cscale(X2[o], AL2));
+                h = caddc(caddc(cscale(H0[o], BE0), cscale(H1[o], BE1)), cscale(H2[o], BE2));
+            }
+            double2 s;
+            if (op == 0) {
+                s = csubc(h, cscale(a, k2dt));
+                S[2 * pp][row] = a.x;
+                S[2 * pp + 1][row] = a.y;
+            } else {
+                s = caddc(h, cdiv(a, dt));
+            }
+            t[2 * pp][q] = s.x;
+            t[2 * pp + 1][q] = s.y;
+        }
+    }
+    __syncthreads();
+    // 2. v: + DY2 (sum alpha v) / dt, the mat-vec in the reference's column order
+    if (op == 0 && live) {
+        double acc[4][NQ];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc[r][q] = 0.0;
+        for (int j = 0; j < N; ++j) {
+            const double a0 = S[0][j], a1 = S[1][j], a2 = S[2][j], a3 = S[3][j];
+            const double *col = c.DY2 + (size_t)j * N;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int row = lane + 64 * q;
+                if (row < N) {
+                    const double d = col[row];
+                    acc[0][q] = fma(d, a0, acc[0][q]);
+                    acc[1][q] = fma(d, a1, acc[1][q]);
+                    acc[2][q] = fma(d, a2, acc[2][q]);
+                    acc[3][q] = fma(d, a3, acc[3][q]);
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) t[r][q] = add(t[r][q], __ddiv_rn(acc[r][q], dt));
+    }
+    // boundary rows
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int row = lane + 64 * q;
+        const bool bc = op == 0 ? (row <= 1 || row >= N - 2) : (row == 0 || row == N - 1);
+        if (bc || row >= N)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) t[r][q] = 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) S[r][lane + 64 * q] = t[r][q];
+    __syncthreads();
+    if (!live) return;
+    // 3. rows in pivot order, forward and back substitution
+    const double *Fo = F + ((size_t)job * 2 + op) * N * N;
+    const int *Po = P + ((size_t)job * 2 + op) * N;
+    lu_gather<NQ>(S, Po, N, lane, t);
+    lu_substitute<NQ>(Fo, N, lane, t);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int row = lane + 64 * q;
+        if (row < N) {
+            XN[(size_t)p0 * N + row] = c2(t[0][q], t[1][q]);
+            if (p1 >= 0) XN[(size_t)p1 * N + row] = c2(t[2][q], t[3][q]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+#define OWT_FACTOR_DISPATCH(N, BODY)                                                    \
+    do {                                                                                \
+        const int n_ = (N);                                                             \
+        if (n_ <= 64) { constexpr int RPT = 1, CPT = 16, TC = 4; BODY; }                \
+        else if (n_ <= 128) { constexpr int RPT = 2, CPT = 16, TC = 8; BODY; }          \
+        else if (n_ <= 192) { constexpr int RPT = 3, CPT = 16, TC = 12; BODY; }         \
+        else if (n_ <= 204) { constexpr int RPT = 4, CPT = 17, TC = 12; BODY; }         \
+        else { constexpr int RPT = 4, CPT = 16, TC = 16; BODY; }                        \
+    } while (0)
+
+void launch_factor(const Chan &c, double *F, int *P, int nfac, hipStream_t s) {
+    OWT_FACTOR_DISPATCH(c.NY, hipLaunchKernelGGL((k_factor<RPT, CPT, TC>), dim3(nfac, 2), dim3(64 * TC), 0, s, c, F, P));
+}
+
+void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, bool plain, hipStream_t s) {
+    const int njobs = plain ? c.npencil : c.nsys;
+    const int blocks = (njobs + 1) / 2;
+    OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_solve<NQ>, dim3(blocks), dim3(256), 0, s, c, sp, F, P, plain ? 1 : 0));
+}
+
+}  // namespace owt

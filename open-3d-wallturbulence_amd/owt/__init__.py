@@ -24,7 +24,8 @@ FIELDS = {"u": 0, "v": 1, "w": 2, "omx": 3, "omy": 4, "omz": 5,
           "lmbx": 6, "lmby": 7, "lmbz": 8, "Hv": 9, "Hg": 10}
 PLAIN, BATCHED = 0, 1
 FP64, MIXED = 0, 1
-KERNELS = {"y_rhs": 0, "solve": 1, "horizontal": 2, "z_inverse": 3, "x_lamb": 4, "z_forward": 5, "nonlinear": 6}
+KERNELS = {"solve": 0, "horizontal": 1, "mean": 2, "z_inverse": 3, "x_lamb": 4, "z_forward": 5, "nonlinear": 6,
+           "factor": 7}
 
 # every symbol include/owt.h declares
 EXPORTS = [
@@ -32,7 +33,7 @@ EXPORTS = [
     "owt_init_fields", "owt_set_velocity", "owt_step", "owt_wall_shear", "owt_diag",
     "owt_steps_done", "owt_get_field", "owt_set_method", "owt_set_precision",
     "owt_checkpoint", "owt_restore", "owt_f2p", "owt_p2f", "owt_lamb", "owt_get_matrix",
-    "owt_time_steps", "owt_time_kernel",
+    "owt_time_steps", "owt_time_kernel", "owt_get_factor",
 ]
 
 
@@ -91,6 +92,7 @@ def lib():
         L.owt_get_matrix.argtypes = [P, I, P]
         L.owt_time_steps.argtypes = [P, I, P]
         L.owt_time_kernel.argtypes = [P, I, I, P]
+        L.owt_get_factor.argtypes = [P, I, I, P, P]
         _lib = L
     return _lib
 
@@ -220,6 +222,14 @@ class Channel:
         out = np.empty((self.NY, self.NY), dtype=np.float64, order="F")
         self._check(lib().owt_get_matrix(self._h, idx, _p(out)))
         return out
+
+    def factor(self, sys, op):
+        """LU factor (dgetrf layout) and pivot rows of operator op (0 Av, 1 Ag)
+        of the +-kz system sys = mz*NXH + i."""
+        lu = np.empty((self.NY, self.NY), dtype=np.float64, order="F")
+        perm = np.empty(self.NY, dtype=np.int32)
+        self._check(lib().owt_get_factor(self._h, int(sys), int(op), _p(lu), _p(perm)))
+        return lu, perm
 
     def time_steps(self, n):
         ms = ctypes.c_double()
