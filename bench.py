@@ -63,6 +63,18 @@ def cpu_baseline(threads, budget_s=20.0):
                       f"{steps} steps, {el:.1f} s, {n} threads"}
 
 
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A device-to-device copy of nbytes, reps times: the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"mode": "copy", "bytes_per_copy": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,12 +83,18 @@ def main():
     ap.add_argument("--config", default="re550_384")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mode", choices=("full", "profile", "copy"), default="full",
+                    help="full: the benchmark line; profile: only the mixed batched step, for "
+                         "rocprofv3; copy: a plain device copy of known size, to scale counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               f"(multi-GPU sharding is a design in DESIGN.md, not implemented)", file=sys.stderr)
         sys.exit(2)
 
+    if args.mode == "copy":
+        copy_calibration()
+        return
     import owt
     cfg = dict(owt.CONFIGS[args.config])
     amp = 0.05 * 1.5 if args.config == "re550_384" else 0.15
@@ -100,6 +118,11 @@ def main():
 
     print("[bench] mixed batched", file=sys.stderr, flush=True)
     t_mixed = timed(K, Wm)
+    if args.mode == "profile":
+        ch.close()
+        print(json.dumps({"mode": "profile", "config": args.config, "steps": K, "warmup": Wm,
+                          "ms_per_step": round(t_mixed, 4)}), flush=True)
+        return
     kern = {name: ch.time_kernel(name, 5) for name in stages}
     t_factor_once = ch.time_kernel("factor", 1)
     print("[bench] fp64 batched", file=sys.stderr, flush=True)

@@ -209,8 +209,9 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P) {
 // A batched job carries the +kz and -kz pencils (four real right-hand sides),
 // a plain job one pencil.
 // ---------------------------------------------------------------------------
-template <int NQ>
-__global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P, int plain) {
+template <int NQ, bool PLAIN>
+__global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P) {
+    constexpr bool plain = PLAIN;
     __shared__ double sh[4][4][64 * NQ];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int job = blockIdx.x * 2 + (w >> 1), op = w & 1;
@@ -334,7 +335,10 @@ void launch_factor(const Chan &c, double *F, int *P, int nfac, hipStream_t s) {
 void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, bool plain, hipStream_t s) {
     const int njobs = plain ? c.npencil : c.nsys;
     const int blocks = (njobs + 1) / 2;
-    OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_solve<NQ>, dim3(blocks), dim3(256), 0, s, c, sp, F, P, plain ? 1 : 0));
+    if (plain)
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, true>), dim3(blocks), dim3(256), 0, s, c, sp, F, P));
+    else
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false>), dim3(blocks), dim3(256), 0, s, c, sp, F, P));
 }
 
 }  // namespace owt

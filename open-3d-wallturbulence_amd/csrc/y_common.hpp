@@ -58,60 +58,94 @@ __device__ __forceinline__ void lu_gather(const double (*S)[64 * NQ], const int 
 
 // Forward substitution with the unit lower factor, then back substitution with
 // the upper factor (dtrsm as called by dgetrs, src/linalg.hpp:660-680). The
-// factor is N x N column-major; column k is read coalesced, the pivot value of
-// each step is broadcast from its lane.
-template <int NQ>
+// factor is N x N column-major. Columns are loaded CK at a time (coalesced,
+// all CK loads in flight before the first is used), and the value of each step
+// is broadcast from the lane that holds it.
+template <int NQ, int CK = 8>
 __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int N, int lane, double (&t)[4][NQ]) {
-    for (int k = 0; k < N; ++k) {
-        const int qk = k >> 6, lk = k & 63;
-        double yk[4];
+    for (int k0 = 0; k0 < N; k0 += CK) {
+        double cb[CK][NQ];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            double v = t[r][0];
+        for (int c = 0; c < CK; ++c) {
+            const int k = k0 + c;
+            const double *col = Fo + (size_t)k * N;
 #pragma unroll
-            for (int q = 1; q < NQ; ++q)
-                if (q == qk) v = t[r][q];
-            yk[r] = readlane_d(v, lk);
+            for (int q = 0; q < NQ; ++q) {
+                const int row = lane + 64 * q;
+                cb[c][q] = (k < N && row > k && row < N) ? col[row] : 0.0;
+            }
         }
-        const double *col = Fo + (size_t)k * N;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int row = lane + 64 * q;
-            if (row > k && row < N) {
-                const double m = col[row];
+        for (int c = 0; c < CK; ++c) {
+            const int k = k0 + c;
+            if (k >= N) break;
+            const int qk = k >> 6, lk = k & 63;
+            double yk[4];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) t[r][q] = fma(-m, yk[r], t[r][q]);
+            for (int r = 0; r < 4; ++r) {
+                double v = t[r][0];
+#pragma unroll
+                for (int q = 1; q < NQ; ++q)
+                    if (q == qk) v = t[r][q];
+                yk[r] = readlane_d(v, lk);
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int row = lane + 64 * q;
+                if (row > k && row < N) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) t[r][q] = fma(-cb[c][q], yk[r], t[r][q]);
+                }
             }
         }
     }
-    for (int k = N - 1; k >= 0; --k) {
-        const int qk = k >> 6, lk = k & 63;
-        const double *col = Fo + (size_t)k * N;
-        const double d = col[k];
-        double xk[4];
+    for (int k0 = N - 1; k0 >= 0; k0 -= CK) {
+        double cb[CK][NQ];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            double v = t[r][0];
+        for (int c = 0; c < CK; ++c) {
+            const int k = k0 - c;
+            const double *col = Fo + (size_t)k * N;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int row = lane + 64 * q;
+                cb[c][q] = (k >= 0 && row <= k) ? col[row] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < CK; ++c) {
+            const int k = k0 - c;
+            if (k < 0) break;
+            const int qk = k >> 6, lk = k & 63;
+            // the diagonal U(k,k) sits in lane lk, register qk
+            double dv = cb[c][0];
 #pragma unroll
             for (int q = 1; q < NQ; ++q)
-                if (q == qk) v = t[r][q];
-            xk[r] = __ddiv_rn(readlane_d(v, lk), d);
-        }
-        if (lane == lk) {
+                if (q == qk) dv = cb[c][q];
+            const double d = readlane_d(dv, lk);
+            double xk[4];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q)
-                if (q == qk) {
+            for (int r = 0; r < 4; ++r) {
+                double v = t[r][0];
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) t[r][q] = xk[r];
+                for (int q = 1; q < NQ; ++q)
+                    if (q == qk) v = t[r][q];
+                xk[r] = __ddiv_rn(readlane_d(v, lk), d);
+            }
+            if (lane == lk) {
+#pragma unroll
+                for (int q = 0; q < NQ; ++q)
+                    if (q == qk) {
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) t[r][q] = xk[r];
+                    }
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int row = lane + 64 * q;
+                if (row < k) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) t[r][q] = fma(-cb[c][q], xk[r], t[r][q]);
                 }
-        }
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int row = lane + 64 * q;
-            if (row < k) {
-                const double m = col[row];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) t[r][q] = fma(-m, xk[r], t[r][q]);
             }
         }
     }
