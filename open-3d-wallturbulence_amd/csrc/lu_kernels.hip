@@ -230,19 +230,22 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
     const double k2 = k2_of(c, kz, ix);
     const double k2dt = __ddiv_rn(k2, dt);
 
-    // 1. combine the history levels
+    // 1. combine the history levels (loads from clamped, valid addresses; masked after)
     double t[4][NQ];
+    const int p0c = live ? p0 : 0;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int row = lane + 64 * q;
+        const int rowc = row < N ? row : N - 1;
 #pragma unroll
         for (int pp = 0; pp < 2; ++pp) {
-            double2 a = c2(0, 0), h = c2(0, 0);
             const int pn = pp == 0 ? p0 : p1;
-            if (live && row < N && pn >= 0) {
-                const size_t o = (size_t)pn * N + row;
-                a = caddc(caddc(cscale(X0[o], AL0), cscale(X1[o], AL1)), cscale(X2[o], AL2));
-                h = caddc(caddc(cscale(H0[o], BE0), cscale(H1[o], BE1)), cscale(H2[o], BE2));
+            const size_t o = (size_t)(pp == 0 || p1 < 0 ? p0c : p1) * N + rowc;
+            double2 a = caddc(caddc(cscale(X0[o], AL0), cscale(X1[o], AL1)), cscale(X2[o], AL2));
+            double2 h = caddc(caddc(cscale(H0[o], BE0), cscale(H1[o], BE1)), cscale(H2[o], BE2));
+            if (!(live && row < N && pn >= 0)) {
+                a = c2(0, 0);
+                h = c2(0, 0);
             }
             double2 s;
             if (op == 0) {
@@ -264,18 +267,30 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
         for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int q = 0; q < NQ; ++q) acc[r][q] = 0.0;
-        for (int j = 0; j < N; ++j) {
-            const double a0 = S[0][j], a1 = S[1][j], a2 = S[2][j], a3 = S[3][j];
-            const double *col = c.DY2 + (size_t)j * N;
+        // DY2 columns CK at a time from clamped addresses (rows >= N are discarded below)
+        constexpr int CK = 8;
+        for (int j0 = 0; j0 < N; j0 += CK) {
+            double dc[CK][NQ];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const int row = lane + 64 * q;
-                if (row < N) {
-                    const double d = col[row];
-                    acc[0][q] = fma(d, a0, acc[0][q]);
-                    acc[1][q] = fma(d, a1, acc[1][q]);
-                    acc[2][q] = fma(d, a2, acc[2][q]);
-                    acc[3][q] = fma(d, a3, acc[3][q]);
+            for (int cc = 0; cc < CK; ++cc) {
+                const int j = j0 + cc < N ? j0 + cc : N - 1;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    const int row = lane + 64 * q;
+                    dc[cc][q] = c.DY2[(size_t)j * N + (row < N ? row : N - 1)];
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < CK; ++cc) {
+                const int j = j0 + cc;
+                if (j >= N) break;
+                const double a0 = S[0][j], a1 = S[1][j], a2 = S[2][j], a3 = S[3][j];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    acc[0][q] = fma(dc[cc][q], a0, acc[0][q]);
+                    acc[1][q] = fma(dc[cc][q], a1, acc[1][q]);
+                    acc[2][q] = fma(dc[cc][q], a2, acc[2][q]);
+                    acc[3][q] = fma(dc[cc][q], a3, acc[3][q]);
                 }
             }
         }

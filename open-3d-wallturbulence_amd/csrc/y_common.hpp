@@ -59,8 +59,11 @@ __device__ __forceinline__ void lu_gather(const double (*S)[64 * NQ], const int 
 // Forward substitution with the unit lower factor, then back substitution with
 // the upper factor (dtrsm as called by dgetrs, src/linalg.hpp:660-680). The
 // factor is N x N column-major. Columns are loaded CK at a time (coalesced,
-// all CK loads in flight before the first is used), and the value of each step
-// is broadcast from the lane that holds it.
+// all CK loads in flight before the first is used); every load reads a valid
+// address (row and column indices clamped into the matrix) and the values
+// outside the triangle are masked afterwards, so that no load sits behind a
+// per-lane branch. The value of each step is broadcast from the lane that
+// holds it.
 template <int NQ, int CK = 8>
 __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int N, int lane, double (&t)[4][NQ]) {
     for (int k0 = 0; k0 < N; k0 += CK) {
@@ -68,13 +71,20 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
 #pragma unroll
         for (int c = 0; c < CK; ++c) {
             const int k = k0 + c;
-            const double *col = Fo + (size_t)k * N;
+            const double *col = Fo + (size_t)(k < N ? k : N - 1) * N;
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 const int row = lane + 64 * q;
-                cb[c][q] = (k < N && row > k && row < N) ? col[row] : 0.0;
+                cb[c][q] = col[row < N ? row : N - 1];
             }
         }
+#pragma unroll
+        for (int c = 0; c < CK; ++c)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int row = lane + 64 * q, k = k0 + c;
+                if (!(k < N && row > k && row < N)) cb[c][q] = 0.0;
+            }
 #pragma unroll
         for (int c = 0; c < CK; ++c) {
             const int k = k0 + c;
@@ -104,13 +114,20 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
 #pragma unroll
         for (int c = 0; c < CK; ++c) {
             const int k = k0 - c;
-            const double *col = Fo + (size_t)k * N;
+            const double *col = Fo + (size_t)(k >= 0 ? k : 0) * N;
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 const int row = lane + 64 * q;
-                cb[c][q] = (k >= 0 && row <= k) ? col[row] : 0.0;
+                cb[c][q] = col[row < N ? row : N - 1];
             }
         }
+#pragma unroll
+        for (int c = 0; c < CK; ++c)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int row = lane + 64 * q, k = k0 - c;
+                if (!(k >= 0 && row <= k)) cb[c][q] = 0.0;
+            }
 #pragma unroll
         for (int c = 0; c < CK; ++c) {
             const int k = k0 - c;

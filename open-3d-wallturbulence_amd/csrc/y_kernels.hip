@@ -20,6 +20,9 @@
 
 namespace owt {
 
+// operator columns fetched per batch in the row-per-thread mat-vecs
+constexpr int JC = 8;
+
 
 // ---------------------------------------------------------------------------
 // horizontal velocity and vorticity (src/solver.cpp:516-539)
@@ -47,17 +50,30 @@ __global__ __launch_bounds__(256) void k_horizontal(Chan c, StepPtrs sp) {
     double2 d1v[TP], d2v[TP], d1g[TP];
 #pragma unroll
     for (int pp = 0; pp < TP; ++pp) d1v[pp] = d2v[pp] = d1g[pp] = c2(0, 0);
-    for (int j = 0; j < N; ++j) {
-        const double a = c.DY1[(size_t)j * N + row], b = c.DY2[(size_t)j * N + row];
+    // operator columns JC at a time, all loads in flight before use
+    for (int j0 = 0; j0 < N; j0 += JC) {
+        double da[JC], db[JC];
 #pragma unroll
-        for (int pp = 0; pp < TP; ++pp) {
-            const double2 x = sv[pp][j], y = sg[pp][j];
-            d1v[pp].x = fma(a, x.x, d1v[pp].x);
-            d1v[pp].y = fma(a, x.y, d1v[pp].y);
-            d2v[pp].x = fma(b, x.x, d2v[pp].x);
-            d2v[pp].y = fma(b, x.y, d2v[pp].y);
-            d1g[pp].x = fma(a, y.x, d1g[pp].x);
-            d1g[pp].y = fma(a, y.y, d1g[pp].y);
+        for (int cc = 0; cc < JC; ++cc) {
+            const size_t e = (size_t)(j0 + cc < N ? j0 + cc : N - 1) * N + row;
+            da[cc] = c.DY1[e];
+            db[cc] = c.DY2[e];
+        }
+#pragma unroll
+        for (int cc = 0; cc < JC; ++cc) {
+            const int j = j0 + cc;
+            if (j >= N) break;
+            const double a = da[cc], b = db[cc];
+#pragma unroll
+            for (int pp = 0; pp < TP; ++pp) {
+                const double2 x = sv[pp][j], y = sg[pp][j];
+                d1v[pp].x = fma(a, x.x, d1v[pp].x);
+                d1v[pp].y = fma(a, x.y, d1v[pp].y);
+                d2v[pp].x = fma(b, x.x, d2v[pp].x);
+                d2v[pp].y = fma(b, x.y, d2v[pp].y);
+                d1g[pp].x = fma(a, y.x, d1g[pp].x);
+                d1g[pp].y = fma(a, y.y, d1g[pp].y);
+            }
         }
     }
     double2 *u = c.cur, *w = c.cur + c.W, *ox = c.cur + 2 * c.W, *oz = c.cur + 3 * c.W;
@@ -234,12 +250,19 @@ __global__ __launch_bounds__(256) void k_nonlinear(Chan c, StepPtrs sp) {
     double2 acc[TP];
 #pragma unroll
     for (int pp = 0; pp < TP; ++pp) acc[pp] = c2(0, 0);
-    for (int j = 0; j < N; ++j) {
-        const double d = c.DY1[(size_t)j * N + row];
+    for (int j0 = 0; j0 < N; j0 += JC) {
+        double dj[JC];
 #pragma unroll
-        for (int pp = 0; pp < TP; ++pp) {
-            acc[pp].x = fma(d, st[pp][j].x, acc[pp].x);
-            acc[pp].y = fma(d, st[pp][j].y, acc[pp].y);
+        for (int cc = 0; cc < JC; ++cc) dj[cc] = c.DY1[(size_t)(j0 + cc < N ? j0 + cc : N - 1) * N + row];
+#pragma unroll
+        for (int cc = 0; cc < JC; ++cc) {
+            const int j = j0 + cc;
+            if (j >= N) break;
+#pragma unroll
+            for (int pp = 0; pp < TP; ++pp) {
+                acc[pp].x = fma(dj[cc], st[pp][j].x, acc[pp].x);
+                acc[pp].y = fma(dj[cc], st[pp][j].y, acc[pp].y);
+            }
         }
     }
 #pragma unroll
