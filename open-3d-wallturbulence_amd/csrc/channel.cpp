@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
 
 namespace owt {
 
@@ -147,16 +148,45 @@ void Channel::upload_operators() {
     OWT_HIP(hipStreamSynchronize(s_));
 }
 
-// The operators Av(k^2), Ag(k^2) do not change in time: factor each +-kz
-// system once (the reference refactors every pencil every step).
+// The operators Av(k^2), Ag(k^2) do not change in time and depend on k^2
+// only: factor each distinct value once (the reference refactors every pencil
+// every step). Systems are grouped by the bit pattern of k^2 as the device
+// computes it, kx*kx + kz*kz with each product rounded.
 void Channel::factor_operators() {
     if (factored_) return;
-    const size_t N = c_.NY, ns = c_.nsys;
+    const size_t N = c_.NY;
+    const int ns = c_.nsys, NXH = c_.NXH;
     if (!Fb_) {
-        Fb_ = (double *)dmalloc(2 * ns * N * N * sizeof(double));
-        Pb_ = (int *)dmalloc(2 * ns * N * sizeof(int));
+        std::map<double, std::vector<int>> groups;
+        for (int sys = 0; sys < ns; ++sys) {
+            const int mz = sys / NXH, i = sys % NXH;
+            double kzv = (double)mz;
+            if (kzv >= c_.NZH) kzv -= c_.NZ;
+            volatile double a = ((double)i * P_.dkx) * ((double)i * P_.dkx);
+            volatile double b = (kzv * P_.dkz) * (kzv * P_.dkz);
+            groups[a + b].push_back(sys);
+        }
+        std::vector<int> jobs, frep;
+        fmap_host_.assign(ns, 0);
+        for (auto &g : groups) {
+            const int f = (int)frep.size();
+            frep.push_back(g.second.front());
+            for (int sys : g.second) {
+                fmap_host_[sys] = f;
+                jobs.push_back(sys);
+            }
+        }
+        nfac_ = (int)frep.size();
+        Fb_ = (double *)dmalloc(2 * (size_t)nfac_ * N * N * sizeof(double));
+        Pb_ = (int *)dmalloc(2 * (size_t)nfac_ * N * sizeof(int));
+        jobs_ = (int *)dmalloc(ns * sizeof(int));
+        fmap_ = (int *)dmalloc(ns * sizeof(int));
+        frep_ = (int *)dmalloc(nfac_ * sizeof(int));
+        OWT_HIP(hipMemcpy(jobs_, jobs.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+        OWT_HIP(hipMemcpy(fmap_, fmap_host_.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+        OWT_HIP(hipMemcpy(frep_, frep.data(), nfac_ * sizeof(int), hipMemcpyHostToDevice));
     }
-    launch_factor(c_, Fb_, Pb_, c_.nsys, s_);
+    launch_factor(c_, Fb_, Pb_, nfac_, frep_, s_);
     OWT_HIP(hipGetLastError());
     OWT_HIP(hipStreamSynchronize(s_));
     check_errors();
@@ -166,7 +196,7 @@ void Channel::factor_operators() {
 void Channel::get_factor(int sys, int op, double *lu, int *perm) {
     if (sys < 0 || sys >= c_.nsys || op < 0 || op > 1) throw Error(2, "factor index out of range");
     factor_operators();
-    const size_t N = c_.NY, f = (size_t)sys * 2 + op;
+    const size_t N = c_.NY, f = (size_t)fmap_host_[sys] * 2 + op;
     OWT_HIP(hipMemcpy(lu, Fb_ + f * N * N, N * N * sizeof(double), hipMemcpyDeviceToHost));
     OWT_HIP(hipMemcpy(perm, Pb_ + f * N, N * sizeof(int), hipMemcpyDeviceToHost));
 }
@@ -476,10 +506,10 @@ void Channel::enqueue_step_body(int ring) {
     const StepPtrs sp = step_ptrs(ring);
     const double *F = plain ? Fp_ : Fb_;
     const int *P = plain ? Pp_ : Pb_;
-    if (plain) launch_factor(c_, Fp_, Pp_, c_.npencil, s_);
-    launch_solve(c_, sp, F, P, plain, s_);
+    if (plain) launch_factor(c_, Fp_, Pp_, c_.npencil, nullptr, s_);
+    launch_solve(c_, sp, F, P, jobs_, fmap_, plain, s_);
     launch_horizontal(c_, sp, s_);
-    launch_mean(c_, sp, F, P, s_);
+    launch_mean(c_, sp, F, P, plain ? 0 : fmap_host_[0], s_);
     // the six transformed fields: u, v, w, omega_x, omega_y, omega_z of the new level
     const size_t W = c_.W;
     const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
@@ -527,14 +557,17 @@ void Channel::enqueue_kernel(int id) {
     const size_t W = c_.W;
     const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
     switch (id) {
-        case 0: launch_solve(c_, sp, F, P, plain, s_); break;
+        case 0: launch_solve(c_, sp, F, P, jobs_, fmap_, plain, s_); break;
         case 1: launch_horizontal(c_, sp, s_); break;
-        case 2: launch_mean(c_, sp, F, P, s_); break;
+        case 2: launch_mean(c_, sp, F, P, plain ? 0 : fmap_host_[0], s_); break;
         case 3: launch_z_inverse(c_, six, 6, c_.zbuf, s_); break;
         case 4: launch_x_lamb(c_, s_); break;
         case 5: launch_z_forward(c_, s_); break;
         case 6: launch_nonlinear(c_, sp, s_); break;
-        case 7: launch_factor(c_, plain ? Fp_ : Fb_, plain ? Pp_ : Pb_, plain ? c_.npencil : c_.nsys, s_); break;
+        case 7:
+            if (plain) launch_factor(c_, Fp_, Pp_, c_.npencil, nullptr, s_);
+            else launch_factor(c_, Fb_, Pb_, nfac_, frep_, s_);
+            break;
         default: throw Error(2, "kernel id must be 0..7");
     }
     OWT_HIP(hipGetLastError());

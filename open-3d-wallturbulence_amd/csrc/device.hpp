@@ -79,13 +79,16 @@ void launch_z_forward_to(const Chan &c, const void *in, int nf, double2 *out, hi
 void launch_peak3(const double *phys, size_t n, unsigned long long *out, hipStream_t s);
 
 // ---- wall-normal phase launches (lu_kernels.hip, y_kernels.hip) ----
-// LU factors of Av and Ag for nfac factor indices (see lu_kernels.hip)
-void launch_factor(const Chan &c, double *F, int *P, int nfac, hipStream_t s);
-// right-hand sides + solves; plain: one job per pencil, else one per +-kz system
-void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, bool plain, hipStream_t s);
+// LU factors of Av and Ag for nfac factor indices (see lu_kernels.hip);
+// frep[f]: representative system or pencil of factor f (null: f itself)
+void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, hipStream_t s);
+// right-hand sides + solves; plain: one job per pencil with its own factor;
+// batched: job j solves system jobs[j] with factor fmap[jobs[j]]
+void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int *jobs, const int *fmap, bool plain,
+                  hipStream_t s);
 void launch_horizontal(const Chan &c, const StepPtrs &sp, hipStream_t s);
-// mean flow, friction force and the k = 0 pencil (factor 0 holds Ag(0))
-void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *P, hipStream_t s);
+// mean flow, friction force and the k = 0 pencil (factor mf holds Ag(0))
+void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *P, int mf, hipStream_t s);
 void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s);
 // initialisation helpers (src/solver.cpp:379-425)
 void launch_vor_from_vel(const Chan &c, hipStream_t s);

@@ -14,13 +14,15 @@
 //
 // A factor set holds, for factor index f and operator op (0 = Av, 1 = Ag),
 // N*N doubles at F + (2f + op) N^2 (column-major) and N pivot rows at
-// P + (2f + op) N. The method decides what f is:
-//   plain   f = pencil k*NXH + i: every pencil factored every step, as the
-//           reference does;
-//   batched f = system mz*NXH + i, mz in [0, NZH]: the operators depend on k^2
-//           only, so +kz and -kz share one factor, computed once.
-// In both cases f / NXH and f % NXH index kz and kx, and factor 0 is the
-// k = 0 pencil whose Ag is the mean-flow operator.
+// P + (2f + op) N. Factor f is the operator of a representative index
+// s = frep[f] (or s = f when frep is null), whose s / NXH and s % NXH index kz
+// and kx. The method decides what the factors are:
+//   plain   one per pencil k*NXH + i (frep null): every pencil factored every
+//           step, as the reference does;
+//   batched one per distinct value of k^2: the operators depend on k^2 only,
+//           so +kz and -kz, and every other pair (kx, kz) with the same k^2
+//           bit for bit, share one factor, computed once. fmap[s] maps a
+//           system s = mz*NXH + i (mz in [0, NZH]) to its factor.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -39,7 +41,7 @@ namespace owt {
 // is its row index in the stored (interchanged) factor.
 // ---------------------------------------------------------------------------
 template <int RPT, int CPT, int TC>
-__global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P) {
+__global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, const int *frep) {
     constexpr int NR = 64 * RPT;
     __shared__ double lbuf[2][NR];
     __shared__ int pos[NR], rowAt[NR];
@@ -49,7 +51,8 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P) {
     const int N = c.NY;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int f = blockIdx.x, op = blockIdx.y;
-    const double k2 = k2_of(c, f / c.NXH, f % c.NXH);
+    const int rep = frep ? frep[f] : f;
+    const double k2 = k2_of(c, rep / c.NXH, rep % c.NXH);
     const double nu = c.nu, dt = c.dt;
     const double k4 = mul(k2, k2);
     const double s1 = add(__ddiv_rn(G0, dt), mul(mul(2.0, nu), k2));
@@ -206,11 +209,14 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P) {
 // ---------------------------------------------------------------------------
 // Right-hand sides (src/solver.cpp:568-574, 593-599) and dgetrs.
 // Four waves per block: wave w takes job 2*block + w/2 and operator w%2.
-// A batched job carries the +kz and -kz pencils (four real right-hand sides),
-// a plain job one pencil.
+// A batched job is a system jobs[job] carrying the +kz and -kz pencils (four
+// real right-hand sides); jobs are ordered by factor so that systems sharing a
+// factor run side by side and its second reading comes from cache. A plain
+// job is one pencil with its own factor.
 // ---------------------------------------------------------------------------
 template <int NQ, bool PLAIN>
-__global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P) {
+__global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P, const int *jobs,
+                                               const int *fmap) {
     constexpr bool plain = PLAIN;
     __shared__ double sh[4][4][64 * NQ];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -218,8 +224,10 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
     const int N = c.NY;
     const int njobs = plain ? c.npencil : c.nsys;
     const bool live = job < njobs;
-    const int kz = live ? job / c.NXH : 0, ix = live ? job % c.NXH : 0;
-    const int p0 = job;
+    const int sys = live ? (plain ? job : jobs[job]) : 0;
+    const int fac = plain ? sys : fmap[sys];
+    const int kz = sys / c.NXH, ix = sys % c.NXH;
+    const int p0 = sys;
     const int p1 = (live && !plain && kz > 0 && kz < c.NZH) ? (c.NZ - kz) * c.NXH + ix : -1;
     double(*S)[64 * NQ] = sh[w];
 
@@ -316,8 +324,8 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
     __syncthreads();
     if (!live) return;
     // 3. rows in pivot order, forward and back substitution
-    const double *Fo = F + ((size_t)job * 2 + op) * N * N;
-    const int *Po = P + ((size_t)job * 2 + op) * N;
+    const double *Fo = F + ((size_t)fac * 2 + op) * N * N;
+    const int *Po = P + ((size_t)fac * 2 + op) * N;
     lu_gather<NQ>(S, Po, N, lane, t);
     lu_substitute<NQ>(Fo, N, lane, t);
 #pragma unroll
@@ -343,17 +351,18 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
         else { constexpr int RPT = 4, CPT = 16, TC = 16; BODY; }                        \
     } while (0)
 
-void launch_factor(const Chan &c, double *F, int *P, int nfac, hipStream_t s) {
-    OWT_FACTOR_DISPATCH(c.NY, hipLaunchKernelGGL((k_factor<RPT, CPT, TC>), dim3(nfac, 2), dim3(64 * TC), 0, s, c, F, P));
+void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, hipStream_t s) {
+    OWT_FACTOR_DISPATCH(c.NY, hipLaunchKernelGGL((k_factor<RPT, CPT, TC>), dim3(nfac, 2), dim3(64 * TC), 0, s, c, F, P, frep));
 }
 
-void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, bool plain, hipStream_t s) {
+void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int *jobs, const int *fmap, bool plain,
+                  hipStream_t s) {
     const int njobs = plain ? c.npencil : c.nsys;
     const int blocks = (njobs + 1) / 2;
     if (plain)
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, true>), dim3(blocks), dim3(256), 0, s, c, sp, F, P));
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, true>), dim3(blocks), dim3(256), 0, s, c, sp, F, P, jobs, fmap));
     else
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false>), dim3(blocks), dim3(256), 0, s, c, sp, F, P));
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false>), dim3(blocks), dim3(256), 0, s, c, sp, F, P, jobs, fmap));
 }
 
 }  // namespace owt

@@ -161,9 +161,10 @@ __device__ void mean_friction(const Chan &c, const double2 *U, const double2 *Wm
 
 // One block: right-hand sides of the mean streamwise and spanwise momentum
 // equations with the friction forcing (src/solver.cpp:452-470), the solve with
-// Ag(0) (factor 0) in wave 0, then the friction and mean vorticity.
+// Ag(0) (factor mf, the k = 0 pencil's) in wave 0, then the friction and mean
+// vorticity.
 template <int NQ>
-__global__ __launch_bounds__(256) void k_mean(Chan c, StepPtrs sp, const double *F, const int *P) {
+__global__ __launch_bounds__(256) void k_mean(Chan c, StepPtrs sp, const double *F, const int *P, int mf) {
     __shared__ double S[4][64 * NQ];
     const int N = c.NY, lane = threadIdx.x & 63;
     const double dt = c.dt;
@@ -198,8 +199,8 @@ __global__ __launch_bounds__(256) void k_mean(Chan c, StepPtrs sp, const double 
     __syncthreads();
     if (threadIdx.x < 64) {
         double t[4][NQ];
-        lu_gather<NQ>(S, P + N, N, lane, t);
-        lu_substitute<NQ>(F + (size_t)N * N, N, lane, t);
+        lu_gather<NQ>(S, P + ((size_t)mf * 2 + 1) * N, N, lane, t);
+        lu_substitute<NQ>(F + ((size_t)mf * 2 + 1) * N * N, N, lane, t);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int row = lane + 64 * q;
@@ -369,8 +370,8 @@ void launch_horizontal(const Chan &c, const StepPtrs &sp, hipStream_t s) {
     hipLaunchKernelGGL(k_horizontal<kTP>, dim3(blocks_tp(c)), dim3(256), 0, s, c, sp);
 }
 
-void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *P, hipStream_t s) {
-    OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_mean<NQ>, dim3(1), dim3(256), 0, s, c, sp, F, P));
+void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *P, int mf, hipStream_t s) {
+    OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_mean<NQ>, dim3(1), dim3(256), 0, s, c, sp, F, P, mf));
 }
 
 void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s) {
