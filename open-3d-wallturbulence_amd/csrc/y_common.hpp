@@ -59,12 +59,12 @@ __device__ __forceinline__ void lu_gather(const double (*S)[64 * NQ], const int 
 // Forward substitution with the unit lower factor, then back substitution with
 // the upper factor (dtrsm as called by dgetrs, src/linalg.hpp:660-680). The
 // factor is N x N column-major. Columns are loaded CK at a time (coalesced,
-// all CK loads in flight before the first is used). A 64-row block of a column
-// is loaded only if it meets the triangle being applied, a wave-uniform test,
-// and inside a loaded block every address is valid (indices clamped into the
-// matrix) with the values outside the triangle masked afterwards, so that no
-// load sits behind a per-lane branch. The value of each step is broadcast from
-// the lane that holds it.
+// all CK loads in flight before the first is used). Every lane loads: a lane
+// whose row lies outside the triangle being applied reads the column's
+// diagonal element instead (one cache line shared by all such lanes), and its
+// value is masked afterwards, so that no load sits behind a branch and only
+// the triangle is fetched. The value of each step is broadcast from the lane
+// that holds it.
 template <int NQ, int CK = 8>
 __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int N, int lane, double (&t)[4][NQ]) {
     for (int k0 = 0; k0 < N; k0 += CK) {
@@ -76,8 +76,8 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 const int row = lane + 64 * q;
-                // rows below the diagonal: the block is needed if its last row is below k
-                cb[c][q] = (64 * q + 63 > k) ? col[row < N ? row : N - 1] : 0.0;
+                const int kc = k < N ? k : N - 1;
+                cb[c][q] = col[(row > kc && row < N) ? row : kc];
             }
         }
 #pragma unroll
@@ -120,8 +120,8 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 const int row = lane + 64 * q;
-                // rows on and above the diagonal: the block is needed if its first row is not below k
-                cb[c][q] = (64 * q <= k) ? col[row < N ? row : N - 1] : 0.0;
+                const int kc = k >= 0 ? k : 0;
+                cb[c][q] = col[row <= kc ? row : kc];
             }
         }
 #pragma unroll
