@@ -58,113 +58,92 @@ __device__ __forceinline__ void lu_gather(const double (*S)[64 * NQ], const int 
 
 // Forward substitution with the unit lower factor, then back substitution with
 // the upper factor (dtrsm as called by dgetrs, src/linalg.hpp:660-680). The
-// factor is N x N column-major. Columns are loaded CK at a time (coalesced,
-// all CK loads in flight before the first is used). Every lane loads: a lane
-// whose row lies outside the triangle being applied reads the column's
-// diagonal element instead (one cache line shared by all such lanes), and its
-// value is masked afterwards, so that no load sits behind a branch and only
-// the triangle is fetched. The value of each step is broadcast from the lane
-// that holds it.
+// factor is N x N column-major.
+//  - Steps run block by block: step k belongs to row block qk = k / 64, a
+//    compile-time index, so the pivot value is read from a fixed register of
+//    lane k % 64 and only the row blocks that meet the triangle are touched.
+//  - Columns are loaded CK at a time, all loads in flight before the first is
+//    used. Every lane of a touched block loads: a lane whose row lies outside
+//    the triangle reads the column's diagonal element instead (one cache line
+//    for all such lanes) and its value is masked, so no load sits behind a
+//    branch and only the triangle is fetched.
+//  - The back substitution scales by the reciprocal of the diagonal, as
+//    optimized trsm kernels do (one division per column).
 template <int NQ, int CK = 8>
 __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int N, int lane, double (&t)[4][NQ]) {
-    for (int k0 = 0; k0 < N; k0 += CK) {
-        double cb[CK][NQ];
 #pragma unroll
-        for (int c = 0; c < CK; ++c) {
-            const int k = k0 + c;
-            const double *col = Fo + (size_t)(k < N ? k : N - 1) * N;
+    for (int qk = 0; qk < NQ; ++qk) {
+        for (int l0 = 0; l0 < 64; l0 += CK) {
+            const int kb = 64 * qk + l0;
+            if (kb >= N) break;
+            double cb[CK][NQ];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const int row = lane + 64 * q;
-                const int kc = k < N ? k : N - 1;
-                cb[c][q] = col[(row > kc && row < N) ? row : kc];
-            }
-        }
+            for (int c = 0; c < CK; ++c) {
+                const int kc = kb + c < N ? kb + c : N - 1;
+                const double *col = Fo + (size_t)kc * N;
 #pragma unroll
-        for (int c = 0; c < CK; ++c)
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const int row = lane + 64 * q, k = k0 + c;
-                if (!(k < N && row > k && row < N)) cb[c][q] = 0.0;
+                for (int q = qk; q < NQ; ++q) {
+                    const int row = lane + 64 * q;
+                    cb[c][q] = col[(row > kc && row < N) ? row : kc];
+                }
             }
 #pragma unroll
-        for (int c = 0; c < CK; ++c) {
-            const int k = k0 + c;
-            if (k >= N) break;
-            const int qk = k >> 6, lk = k & 63;
-            double yk[4];
+            for (int c = 0; c < CK; ++c) {
+                const int k = kb + c;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                double v = t[r][0];
-#pragma unroll
-                for (int q = 1; q < NQ; ++q)
-                    if (q == qk) v = t[r][q];
-                yk[r] = readlane_d(v, lk);
+                for (int q = qk; q < NQ; ++q) {
+                    const int row = lane + 64 * q;
+                    if (!(row > k && row < N)) cb[c][q] = 0.0;
+                }
             }
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const int row = lane + 64 * q;
-                if (row > k && row < N) {
+            for (int c = 0; c < CK; ++c) {
+                const int k = kb + c;
+                if (k >= N) break;
+                double yk[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) yk[r] = readlane_d(t[r][qk], l0 + c);
+#pragma unroll
+                for (int q = qk; q < NQ; ++q)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) t[r][q] = fma(-cb[c][q], yk[r], t[r][q]);
-                }
             }
         }
     }
-    for (int k0 = N - 1; k0 >= 0; k0 -= CK) {
-        double cb[CK][NQ];
 #pragma unroll
-        for (int c = 0; c < CK; ++c) {
-            const int k = k0 - c;
-            const double *col = Fo + (size_t)(k >= 0 ? k : 0) * N;
+    for (int qk = NQ - 1; qk >= 0; --qk) {
+        const int ktop = (64 * qk + 63 < N - 1 ? 64 * qk + 63 : N - 1);
+        for (int kb = ktop; kb >= 64 * qk; kb -= CK) {
+            double cb[CK][NQ];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const int row = lane + 64 * q;
-                const int kc = k >= 0 ? k : 0;
-                cb[c][q] = col[row <= kc ? row : kc];
-            }
-        }
+            for (int c = 0; c < CK; ++c) {
+                const int kc = kb - c >= 64 * qk ? kb - c : 64 * qk;
+                const double *col = Fo + (size_t)kc * N;
 #pragma unroll
-        for (int c = 0; c < CK; ++c)
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const int row = lane + 64 * q, k = k0 - c;
-                if (!(k >= 0 && row <= k)) cb[c][q] = 0.0;
+                for (int q = 0; q <= qk; ++q) {
+                    const int row = lane + 64 * q;
+                    cb[c][q] = col[row <= kc ? row : kc];
+                }
             }
 #pragma unroll
-        for (int c = 0; c < CK; ++c) {
-            const int k = k0 - c;
-            if (k < 0) break;
-            const int qk = k >> 6, lk = k & 63;
-            // the diagonal U(k,k) sits in lane lk, register qk
-            double dv = cb[c][0];
+            for (int c = 0; c < CK; ++c) {
+                const int k = kb - c;
+                if (k < 64 * qk) break;
+                const int lk = k & 63;
+                const double rinv = 1.0 / readlane_d(cb[c][qk], lk);
+                double xk[4];
 #pragma unroll
-            for (int q = 1; q < NQ; ++q)
-                if (q == qk) dv = cb[c][q];
-            const double d = readlane_d(dv, lk);
-            double xk[4];
+                for (int r = 0; r < 4; ++r) xk[r] = readlane_d(t[r][qk], lk) * rinv;
+                if (lane == lk) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                double v = t[r][0];
+                    for (int r = 0; r < 4; ++r) t[r][qk] = xk[r];
+                }
 #pragma unroll
-                for (int q = 1; q < NQ; ++q)
-                    if (q == qk) v = t[r][q];
-                xk[r] = __ddiv_rn(readlane_d(v, lk), d);
-            }
-            if (lane == lk) {
+                for (int q = 0; q <= qk; ++q) {
+                    const int row = lane + 64 * q;
+                    const double m = row < k ? cb[c][q] : 0.0;
 #pragma unroll
-                for (int q = 0; q < NQ; ++q)
-                    if (q == qk) {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) t[r][q] = xk[r];
-                    }
-            }
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const int row = lane + 64 * q;
-                if (row < k) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) t[r][q] = fma(-cb[c][q], xk[r], t[r][q]);
+                    for (int r = 0; r < 4; ++r) t[r][q] = fma(-m, xk[r], t[r][q]);
                 }
             }
         }
