@@ -217,6 +217,12 @@ def test_laminar_flow_stays_put_on_gpu(owt_mod, oracle_mod):
 
 
 def test_step_keeps_hermitian_plane_and_divergence(owt_mod):
+    """The kx = 0 plane stays Hermitian in kz and the discrete divergence
+    i kx u + DY1 v + i kz w vanishes. The transforms give the +kz and -kz bins
+    of a real sequence to rounding only, and the v operator amplifies that
+    rounding (near the walls, where v is zero in exact arithmetic, what remains
+    is elimination roundoff), so the Hermitian defect is bounded by 1e-10 of the
+    field's largest value; a sign or index error would be of order one."""
     p = params(32, 33, 32)
     ch = gpu(owt_mod, p)
     ch.init_fields(2, seed=9, amp=0.3)
@@ -228,7 +234,7 @@ def test_step_keeps_hermitian_plane_and_divergence(owt_mod):
     for name in ("u", "v", "w", "omy", "Hv", "Hg"):
         f = ch.field(name)
         for m in range(1, ch.NZH):
-            assert np.abs(f[m, 0] - np.conj(f[ch.NZ - m, 0])).max() < 1e-12 * np.abs(f).max(), name
+            assert np.abs(f[m, 0] - np.conj(f[ch.NZ - m, 0])).max() < 1e-10 * np.abs(f).max(), name
     for k in range(ch.NZ):
         for i in range(ch.NXH):
             if kx[i] ** 2 + kz[k] ** 2 < 1e-8:
