@@ -11,7 +11,11 @@ in HBM; steps are timed with HIP events on the solver's stream.
 Prints one JSON line.
 """
 import argparse
+import csv
 import json
+import shutil
+import subprocess
+import tempfile
 
 import numpy as np
 import os
@@ -78,6 +82,49 @@ def cpu_baseline(threads, budget_s=20.0):
                       f"{steps} steps, {el:.1f} s, {n} threads"}
 
 
+def pmc_traffic(config):
+    """HBM bytes per launch of every step kernel, from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs (one counter per run) of this script in profile mode, each
+    scaled by what the counter reads for a 1 GiB device copy. Runs before this
+    process touches the GPU. Returns ({kernel symbol: bytes}, {scale}) or None."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    me = os.path.abspath(__file__)
+    out = tempfile.mkdtemp(prefix="owt_pmc_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+
+    def run(counter, mode, tag):
+        d = os.path.join(out, tag)
+        cmd = [exe, "--pmc", counter, "--kernel-trace", "--output-format", "csv", "-d", d, "-o", "run", "--",
+               sys.executable, me, "--mode", mode, "--config", config, "--steps", "2", "--warmup", "0"]
+        print(f"[bench] rocprofv3 --pmc {counter} ({mode})", file=sys.stderr, flush=True)
+        subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                       timeout=240, check=True)
+        per = {}
+        for root, _, files in os.walk(d):
+            for f in files:
+                if f.endswith("counter_collection.csv"):
+                    for r in csv.DictReader(open(os.path.join(root, f))):
+                        per.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]) * 1024.0)
+        return {k: sum(v) / len(v) for k, v in per.items()}
+
+    try:
+        res, scale = {}, {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            copy = run(counter, "copy", "copy_" + counter)
+            ref = [v for k, v in copy.items() if "copyBuffer" in k]
+            scale[counter] = (1 << 30) / ref[0]
+            for k, v in run(counter, "profile", "step_" + counter).items():
+                res[k] = res.get(k, 0.0) + v * scale[counter]
+        return res, scale
+    except (subprocess.SubprocessError, OSError, IndexError, KeyError, ZeroDivisionError) as e:
+        print(f"[bench] counter runs failed ({e}); traffic left null", file=sys.stderr, flush=True)
+        return None
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A device-to-device copy of nbytes, reps times: the reference point that
     scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -98,6 +145,7 @@ def main():
     ap.add_argument("--config", default="re550_384")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs (roofline.traffic null)")
     ap.add_argument("--mode", choices=("full", "profile", "copy"), default="full",
                     help="full: the benchmark line; profile: only the mixed batched step, for "
                          "rocprofv3; copy: a plain device copy of known size, to scale counters")
@@ -110,6 +158,9 @@ def main():
     if args.mode == "copy":
         copy_calibration()
         return
+    traffic = None
+    if args.mode == "full" and not args.no_pmc:
+        traffic = pmc_traffic(args.config)
     import owt
     cfg = dict(owt.CONFIGS[args.config])
     amp = 0.05 * 1.5 if args.config == "re550_384" else 0.15
@@ -161,6 +212,16 @@ def main():
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "bytes_per_launch": sb[dom], "ms_per_launch": round(kern[dom], 4)}
+    if traffic:
+        per, scale = traffic
+        # the mixed-precision batched variant of the kernel (profile mode runs only that
+        # step; FP64 transform variants run once at initialisation)
+        hits = [v for k, v in per.items() if (f"owt::k_{dom}<" in k or f"owt::k_{dom}(" in k)
+                and "true>" not in k and "<double" not in k and "type<double" not in k]
+        if hits:
+            roof["traffic"] = round(max(hits))
+            roof["traffic_note"] = (f"FETCH_SIZE x {scale['FETCH_SIZE']:.4f} + WRITE_SIZE x {scale['WRITE_SIZE']:.4f} "
+                                    f"(scales from a 1 GiB device copy), mixed batched step")
     out = {
         "metric": "ns/gridpoint/step at Re_tau550 384x193x384, mixed vs FP64; achieved HBM GB/s",
         "value": round(ns, 5),
