@@ -417,21 +417,26 @@ size_t smem_bytes() {
     return sizeof(typename CT<T>::t) * (size_t)LMAX * (TJ + 1);
 }
 
-#define OWT_LEN_DISPATCH(L, BODY)                                                   \
-    do {                                                                            \
-        if ((L) <= 96) {                                                            \
-            constexpr int TJ = 16, TL = 16, LMAX = 96;                              \
-            BODY;                                                                   \
-        } else if ((L) <= 192) {                                                    \
-            constexpr int TJ = 8, TL = 32, LMAX = 192;                              \
-            BODY;                                                                   \
-        } else if ((L) <= 576) {                                                    \
-            constexpr int TJ = 4, TL = 64, LMAX = 576;                              \
-            BODY;                                                                   \
-        } else {                                                                    \
-            constexpr int TJ = 2, TL = 128, LMAX = 1152;                            \
-            BODY;                                                                   \
-        }                                                                           \
+// Tile shapes. TJ consecutive y-points per block make each global access a
+// run of TJ complex values: 16 in FP32 and 8 in FP64 give 128-byte runs. TL
+// threads share one line; the x phase keeps three transformed pairs per thread
+// (L / TL complex values each) in registers, so it uses more threads per line.
+#define OWT_Z_DISPATCH(L, T, BODY)                                                            \
+    do {                                                                                      \
+        constexpr bool f32_ = sizeof(T) == 4;                                                 \
+        if ((L) <= 96) { constexpr int TJ = 16, TL = 16, LMAX = 96; BODY; }                   \
+        else if ((L) <= 192) { constexpr int TJ = f32_ ? 16 : 8, TL = f32_ ? 16 : 32, LMAX = 192; BODY; } \
+        else if ((L) <= 576) { constexpr int TJ = f32_ ? 16 : 8, TL = f32_ ? 16 : 32, LMAX = 576; BODY; } \
+        else { constexpr int TJ = f32_ ? 8 : 4, TL = f32_ ? 32 : 64, LMAX = 1152; BODY; }    \
+    } while (0)
+
+#define OWT_X_DISPATCH(L, T, BODY)                                                            \
+    do {                                                                                      \
+        constexpr bool f32_ = sizeof(T) == 4;                                                 \
+        if ((L) <= 96) { constexpr int TJ = 16, TL = 16, LMAX = 96; BODY; }                   \
+        else if ((L) <= 192) { constexpr int TJ = f32_ ? 16 : 8, TL = 32, LMAX = 192; BODY; } \
+        else if ((L) <= 576) { constexpr int TJ = f32_ ? 16 : 8, TL = f32_ ? 32 : 64, LMAX = 576; BODY; } \
+        else { constexpr int TJ = f32_ ? 8 : 4, TL = f32_ ? 64 : 128, LMAX = 1152; BODY; }   \
     } while (0)
 
 }  // namespace
@@ -439,45 +444,54 @@ size_t smem_bytes() {
 void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out, hipStream_t s) {
     PtrTable t{};
     for (int f = 0; f < nf; ++f) t.p[f] = in[f];
-    OWT_LEN_DISPATCH(c.NZ2, {
-        dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, nf);
-        if (c.precision == 0)
+    if (c.precision == 0)
+        OWT_Z_DISPATCH(c.NZ2, double, {
+            dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, nf);
             hipLaunchKernelGGL((k_z_inverse<double, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, t,
                                (double2 *)out);
-        else
+        });
+    else
+        OWT_Z_DISPATCH(c.NZ2, float, {
+            dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, nf);
             hipLaunchKernelGGL((k_z_inverse<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c, t,
                                (float2 *)out);
-    });
+        });
 }
 
 void launch_x_lamb(const Chan &c, hipStream_t s) {
-    OWT_LEN_DISPATCH(c.NX2, {
-        dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ);
-        if (c.precision == 0)
+    if (c.precision == 0)
+        OWT_X_DISPATCH(c.NX2, double, {
+            dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ);
             hipLaunchKernelGGL((k_x_lamb<double, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c,
                                (const double2 *)c.zbuf, (double2 *)c.xbuf);
-        else
+        });
+    else
+        OWT_X_DISPATCH(c.NX2, float, {
+            dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ);
             hipLaunchKernelGGL((k_x_lamb<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
                                (const float2 *)c.zbuf, (float2 *)c.xbuf);
-    });
+        });
 }
 
 void launch_z_forward(const Chan &c, hipStream_t s) {
-    OWT_LEN_DISPATCH(c.NZ2, {
-        dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, 3);
-        if (c.precision == 0)
+    if (c.precision == 0)
+        OWT_Z_DISPATCH(c.NZ2, double, {
+            dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, 3);
             hipLaunchKernelGGL((k_z_forward<double, TJ, TL, LMAX, double2>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s,
                                c, (const double2 *)c.xbuf, (double2 *)c.lmb);
-        else
+        });
+    else
+        OWT_Z_DISPATCH(c.NZ2, float, {
+            dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, 3);
             hipLaunchKernelGGL((k_z_forward<float, TJ, TL, LMAX, float2>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
                                (const float2 *)c.xbuf, (float2 *)c.lmb);
-    });
+        });
 }
 
 void launch_z_forward_to(const Chan &c, const void *in, int nf, double2 *out, hipStream_t s) {
     Chan d = c;
     d.precision = 0;
-    OWT_LEN_DISPATCH(c.NZ2, {
+    OWT_Z_DISPATCH(c.NZ2, double, {
         dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, nf);
         hipLaunchKernelGGL((k_z_forward<double, TJ, TL, LMAX, double2>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, d,
                            (const double2 *)in, out);
@@ -485,7 +499,7 @@ void launch_z_forward_to(const Chan &c, const void *in, int nf, double2 *out, hi
 }
 
 void launch_x_c2r(const Chan &c, const void *in, int nf, double *phys, hipStream_t s) {
-    OWT_LEN_DISPATCH(c.NX2, {
+    OWT_X_DISPATCH(c.NX2, double, {
         dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ, (nf + 1) / 2);
         hipLaunchKernelGGL((k_x_c2r<TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c,
                            (const double2 *)in, nf, phys);
@@ -493,7 +507,7 @@ void launch_x_c2r(const Chan &c, const void *in, int nf, double *phys, hipStream
 }
 
 void launch_x_r2c(const Chan &c, const double *phys, int nf, void *out, hipStream_t s) {
-    OWT_LEN_DISPATCH(c.NX2, {
+    OWT_X_DISPATCH(c.NX2, double, {
         dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ, (nf + 1) / 2);
         hipLaunchKernelGGL((k_x_r2c<TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, phys, nf,
                            (double2 *)out);

@@ -50,7 +50,8 @@ def orc(oracle_mod, p):
     return oracle_mod.Oracle(p["nu"], p["dt"], p["dkx"], p["dkz"], p["alpha_mesh"], p["NX"], p["NY"], p["NZ"])
 
 
-@pytest.mark.parametrize("shape", [(16, 17, 16), (64, 65, 64), (12, 33, 24)])
+# padded lengths 24..1152 cover every transform tile shape (<= 96, 192, 576, 1152)
+@pytest.mark.parametrize("shape", [(16, 17, 16), (64, 65, 64), (12, 33, 24), (128, 9, 96), (384, 9, 384), (768, 7, 32)])
 def test_transforms_match_oracle(owt_mod, oracle_mod, shape):
     p = params(*shape)
     ch, o = gpu(owt_mod, p), orc(oracle_mod, p)
@@ -64,7 +65,7 @@ def test_transforms_match_oracle(owt_mod, oracle_mod, shape):
         assert rel(back[q], o.p2f(P[q])) < 1e-13
 
 
-@pytest.mark.parametrize("shape", [(16, 17, 16), (64, 65, 64)])
+@pytest.mark.parametrize("shape", [(16, 17, 16), (64, 65, 64), (128, 9, 128), (384, 9, 384), (768, 7, 32)])
 def test_lamb_matches_oracle(owt_mod, oracle_mod, shape):
     p = params(*shape)
     ch, o = gpu(owt_mod, p), orc(oracle_mod, p)
