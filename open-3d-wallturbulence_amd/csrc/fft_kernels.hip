@@ -417,27 +417,18 @@ size_t smem_bytes() {
     return sizeof(typename CT<T>::t) * (size_t)LMAX * (TJ + 1);
 }
 
-// Tile shapes. TJ consecutive y-points per block make each global access a
-// run of TJ complex values: 16 in FP32 and 8 in FP64 give 128-byte runs. TL
-// threads share one line; the x phase keeps three transformed pairs per thread
-// (L / TL complex values each) in registers, so it uses more threads per line.
-#define OWT_Z_DISPATCH(L, T, BODY)                                                            \
-    do {                                                                                      \
-        constexpr bool f32_ = sizeof(T) == 4;                                                 \
-        if ((L) <= 96) { constexpr int TJ = 16, TL = 16, LMAX = 96; BODY; }                   \
-        else if ((L) <= 192) { constexpr int TJ = f32_ ? 16 : 8, TL = f32_ ? 16 : 32, LMAX = 192; BODY; } \
-        else if ((L) <= 576) { constexpr int TJ = f32_ ? 16 : 8, TL = f32_ ? 16 : 32, LMAX = 576; BODY; } \
-        else { constexpr int TJ = f32_ ? 8 : 4, TL = f32_ ? 32 : 64, LMAX = 1152; BODY; }    \
+// Tile shapes: TJ consecutive y-points per block (each global access is a run
+// of TJ complex values), TL threads per line. Wider runs (8 or 16 points, for
+// 64- or 128-byte runs) cost LDS per block, and with it blocks per CU; at the
+// lengths of the Re_tau 550 grid they measured slower than these.
+#define OWT_Z_DISPATCH(L, T, BODY)                                                   \
+    do {                                                                             \
+        if ((L) <= 96) { constexpr int TJ = 16, TL = 16, LMAX = 96; BODY; }          \
+        else if ((L) <= 192) { constexpr int TJ = 8, TL = 32, LMAX = 192; BODY; }    \
+        else if ((L) <= 576) { constexpr int TJ = 4, TL = 64, LMAX = 576; BODY; }    \
+        else { constexpr int TJ = 2, TL = 128, LMAX = 1152; BODY; }                  \
     } while (0)
-
-#define OWT_X_DISPATCH(L, T, BODY)                                                            \
-    do {                                                                                      \
-        constexpr bool f32_ = sizeof(T) == 4;                                                 \
-        if ((L) <= 96) { constexpr int TJ = 16, TL = 16, LMAX = 96; BODY; }                   \
-        else if ((L) <= 192) { constexpr int TJ = f32_ ? 16 : 8, TL = 32, LMAX = 192; BODY; } \
-        else if ((L) <= 576) { constexpr int TJ = f32_ ? 16 : 8, TL = f32_ ? 32 : 64, LMAX = 576; BODY; } \
-        else { constexpr int TJ = f32_ ? 8 : 4, TL = f32_ ? 64 : 128, LMAX = 1152; BODY; }   \
-    } while (0)
+#define OWT_X_DISPATCH(L, T, BODY) OWT_Z_DISPATCH(L, T, BODY)
 
 }  // namespace
 
