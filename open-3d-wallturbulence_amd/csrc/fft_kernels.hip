@@ -428,7 +428,15 @@ size_t smem_bytes() {
         else if ((L) <= 576) { constexpr int TJ = 4, TL = 64, LMAX = 576; BODY; }    \
         else { constexpr int TJ = 2, TL = 128, LMAX = 1152; BODY; }                  \
     } while (0)
-#define OWT_X_DISPATCH(L, T, BODY) OWT_Z_DISPATCH(L, T, BODY)
+// the same shapes for the x phase (spelled out: a forwarding macro would expand
+// the launch in BODY before substitution)
+#define OWT_X_DISPATCH(L, T, BODY)                                                   \
+    do {                                                                             \
+        if ((L) <= 96) { constexpr int TJ = 16, TL = 16, LMAX = 96; BODY; }          \
+        else if ((L) <= 192) { constexpr int TJ = 8, TL = 32, LMAX = 192; BODY; }    \
+        else if ((L) <= 576) { constexpr int TJ = 4, TL = 64, LMAX = 576; BODY; }    \
+        else { constexpr int TJ = 2, TL = 128, LMAX = 1152; BODY; }                  \
+    } while (0)
 
 }  // namespace
 
