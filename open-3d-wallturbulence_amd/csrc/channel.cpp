@@ -106,6 +106,7 @@ void Channel::alloc() {
     c.means = (double2 *)dmalloc(12 * N * sizeof(double2));
     c.fric = (double *)dmalloc(8 * sizeof(double));
     c.msol = (double2 *)dmalloc(2 * N * sizeof(double2));
+    c.d2v = (double2 *)dmalloc(3 * c.W * sizeof(double2));
     c.ctrl = (int *)dmalloc(16 * sizeof(int));
     c.diag_cap = 1 << 16;
     c.diag = (double *)dmalloc((size_t)c.diag_cap * 3 * sizeof(double));
@@ -464,6 +465,7 @@ void Channel::set_velocity(const double *u, const double *v, const double *w) {
 void Channel::init_post() {
     const size_t W = c_.W;
     launch_vor_from_vel(c_, s_);
+    launch_init_d2v(c_, s_);
     const double2 *six[6] = {c_.cur, c_.hist + (size_t)(H_V * 3) * W, c_.cur + W, c_.cur + 2 * W, c_.hist + (size_t)(H_G * 3) * W,
                              c_.cur + 3 * W};
     launch_z_inverse(c_, six, 6, c_.zbuf, s_);
@@ -491,12 +493,14 @@ StepPtrs Channel::step_ptrs(int ring) const {
         sp.g[l] = slot(H_G, sl);
         sp.hv[l] = slot(H_HV, sl);
         sp.hg[l] = slot(H_HG, sl);
+        sp.dv[l] = c_.d2v + (size_t)sl * W;
     }
     sp.sn = (ring + 1) % 3;
     sp.vn = slot(H_V, sp.sn);
     sp.gn = slot(H_G, sp.sn);
     sp.hvn = slot(H_HV, sp.sn);
     sp.hgn = slot(H_HG, sp.sn);
+    sp.dvn = c_.d2v + (size_t)sp.sn * W;
     return sp;
 }
 
@@ -736,7 +740,7 @@ void Channel::checkpoint(const std::string &path) {
     std::ofstream f(path, std::ios::binary);
     if (!f) throw Error(5, "cannot open " + path + " for writing");
     CkptHeader h{};
-    std::memcpy(h.magic, "OWTCKPT1", 8);
+    std::memcpy(h.magic, "OWTCKPT2", 8);
     h.version = 1;
     h.NX = c_.NX;
     h.NY = c_.NY;
@@ -755,6 +759,7 @@ void Channel::checkpoint(const std::string &path) {
     };
     const size_t W = c_.W, N = c_.NY;
     dump(c_.hist, 12 * W * sizeof(double2));
+    dump(c_.d2v, 3 * W * sizeof(double2));
     dump(c_.cur, 4 * W * sizeof(double2));
     dump(c_.lmb, 3 * W * (c_.precision == 0 ? sizeof(double2) : sizeof(float2)));
     dump(c_.means, 12 * N * sizeof(double2));
@@ -773,7 +778,7 @@ void Channel::restore(const std::string &path) {
     if (!f) throw Error(5, "cannot open " + path);
     CkptHeader h{};
     f.read((char *)&h, sizeof(h));
-    if (!f || std::memcmp(h.magic, "OWTCKPT1", 8) != 0) throw Error(5, path + " is not a checkpoint");
+    if (!f || std::memcmp(h.magic, "OWTCKPT2", 8) != 0) throw Error(5, path + " is not a checkpoint");
     if (h.NX != c_.NX || h.NY != c_.NY || h.NZ != c_.NZ || h.nu != P_.nu || h.dt != P_.dt || h.dkx != P_.dkx || h.dkz != P_.dkz ||
         h.alpha != P_.alpha_mesh)
         throw Error(5, "checkpoint " + path + " was written for a different case");
@@ -786,6 +791,7 @@ void Channel::restore(const std::string &path) {
     };
     const size_t W = c_.W, N = c_.NY;
     load(c_.hist, 12 * W * sizeof(double2));
+    load(c_.d2v, 3 * W * sizeof(double2));
     load(c_.cur, 4 * W * sizeof(double2));
     load(c_.lmb, 3 * W * (c_.precision == 0 ? sizeof(double2) : sizeof(float2)));
     load(c_.means, 12 * N * sizeof(double2));

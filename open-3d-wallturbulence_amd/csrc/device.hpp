@@ -46,6 +46,7 @@ struct Chan {
     double2 *means;  // 4 x 3 x NY
     double *fric;    // fx[3], fz[3]
     double2 *msol;   // 2 x NY mean-flow solutions
+    double2 *d2v;    // 3 W: DY2 v at the 3 history levels (slots as in hist)
     int *ctrl;
     double *diag;    // diag_cap x 3: fx_fric, bottom, upper per friction evaluation
     int diag_cap;
@@ -57,8 +58,8 @@ struct Chan {
 // History slots of one step: level 0 is the current field, levels 1 and 2 the
 // two before it; the new level overwrites level 2 (slot sn = s[2]).
 struct StepPtrs {
-    const double2 *v[3], *g[3], *hv[3], *hg[3];
-    double2 *vn, *gn, *hvn, *hgn;
+    const double2 *v[3], *g[3], *hv[3], *hg[3], *dv[3];
+    double2 *vn, *gn, *hvn, *hgn, *dvn;
     int s[3], sn;
 };
 
@@ -94,5 +95,7 @@ void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s);
 void launch_vor_from_vel(const Chan &c, hipStream_t s);
 void launch_init_history(const Chan &c, hipStream_t s);
 void launch_init_mean(const Chan &c, hipStream_t s);
+// DY2 v of the initial field into d2v slot 0
+void launch_init_d2v(const Chan &c, hipStream_t s);
 
 }  // namespace owt

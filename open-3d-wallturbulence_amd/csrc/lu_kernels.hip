@@ -233,12 +233,16 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
 
     const double2 *X0 = op == 0 ? sp.v[0] : sp.g[0], *X1 = op == 0 ? sp.v[1] : sp.g[1], *X2 = op == 0 ? sp.v[2] : sp.g[2];
     const double2 *H0 = op == 0 ? sp.hv[0] : sp.hg[0], *H1 = op == 0 ? sp.hv[1] : sp.hg[1], *H2 = op == 0 ? sp.hv[2] : sp.hg[2];
+    const double2 *D0 = sp.dv[0], *D1 = sp.dv[1], *D2 = sp.dv[2];
     double2 *XN = op == 0 ? sp.vn : sp.gn;
     const double dt = c.dt;
     const double k2 = k2_of(c, kz, ix);
     const double k2dt = __ddiv_rn(k2, dt);
 
-    // 1. combine the history levels (loads from clamped, valid addresses; masked after)
+    // 1. combine the history levels (loads from clamped, valid addresses; masked after).
+    // The v equation's DY2 (sum alpha v) / dt is formed from the stored DY2 v of
+    // each level, sum alpha (DY2 v), which the horizontal kernel computed when
+    // each level was new.
     double t[4][NQ];
     const int p0c = live ? p0 : 0;
 #pragma unroll
@@ -251,61 +255,17 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
             const size_t o = (size_t)(pp == 0 || p1 < 0 ? p0c : p1) * N + rowc;
             double2 a = caddc(caddc(cscale(X0[o], AL0), cscale(X1[o], AL1)), cscale(X2[o], AL2));
             double2 h = caddc(caddc(cscale(H0[o], BE0), cscale(H1[o], BE1)), cscale(H2[o], BE2));
-            if (!(live && row < N && pn >= 0)) {
-                a = c2(0, 0);
-                h = c2(0, 0);
-            }
             double2 s;
             if (op == 0) {
-                s = csubc(h, cscale(a, k2dt));
-                S[2 * pp][row] = a.x;
-                S[2 * pp + 1][row] = a.y;
+                const double2 d = caddc(caddc(cscale(D0[o], AL0), cscale(D1[o], AL1)), cscale(D2[o], AL2));
+                s = caddc(csubc(h, cscale(a, k2dt)), cdiv(d, dt));
             } else {
                 s = caddc(h, cdiv(a, dt));
             }
+            if (!(live && row < N && pn >= 0)) s = c2(0, 0);
             t[2 * pp][q] = s.x;
             t[2 * pp + 1][q] = s.y;
         }
-    }
-    __syncthreads();
-    // 2. v: + DY2 (sum alpha v) / dt, the mat-vec in the reference's column order
-    if (op == 0 && live) {
-        double acc[4][NQ];
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) acc[r][q] = 0.0;
-        // DY2 columns CK at a time from clamped addresses (rows >= N are discarded below)
-        constexpr int CK = 8;
-        for (int j0 = 0; j0 < N; j0 += CK) {
-            double dc[CK][NQ];
-#pragma unroll
-            for (int cc = 0; cc < CK; ++cc) {
-                const int j = j0 + cc < N ? j0 + cc : N - 1;
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) {
-                    const int row = lane + 64 * q;
-                    dc[cc][q] = c.DY2[(size_t)j * N + (row < N ? row : N - 1)];
-                }
-            }
-#pragma unroll
-            for (int cc = 0; cc < CK; ++cc) {
-                const int j = j0 + cc;
-                if (j >= N) break;
-                const double a0 = S[0][j], a1 = S[1][j], a2 = S[2][j], a3 = S[3][j];
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) {
-                    acc[0][q] = fma(dc[cc][q], a0, acc[0][q]);
-                    acc[1][q] = fma(dc[cc][q], a1, acc[1][q]);
-                    acc[2][q] = fma(dc[cc][q], a2, acc[2][q]);
-                    acc[3][q] = fma(dc[cc][q], a3, acc[3][q]);
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < NQ; ++q)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) t[r][q] = add(t[r][q], __ddiv_rn(acc[r][q], dt));
     }
     // boundary rows
 #pragma unroll
@@ -316,7 +276,6 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
 #pragma unroll
             for (int r = 0; r < 4; ++r) t[r][q] = 0.0;
     }
-    __syncthreads();
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll

@@ -81,13 +81,15 @@ __global__ __launch_bounds__(256) void k_horizontal(Chan c, StepPtrs sp) {
     for (int pp = 0; pp < TP; ++pp) {
         const int p = p0 + pp;
         if (p >= c.npencil) break;
+        const size_t o = (size_t)p * N + row;
+        // DY2 v of the new level, kept for the next steps' right-hand sides
+        sp.dvn[o] = d2v[pp];
         const int k = p / c.NXH, i = p % c.NXH;
         const double kx = c.kx[i], kz = c.kz[k];
         const double k2 = k2_of(c, k, i);
         if (k2 < 1.e-8) continue;  // the mean pencil is set by the mean-flow kernels
         const double ix = __ddiv_rn(kx, k2), iz = __ddiv_rn(kz, k2), ir = __ddiv_rn(1.0, k2);
         const double2 vv = sv[pp][row], gg = sg[pp][row];
-        const size_t o = (size_t)p * N + row;
         u[o] = csubc(imul(ix, d1v[pp]), imul(iz, gg));
         w[o] = caddc(imul(ix, gg), imul(iz, d1v[pp]));
         const double2 X = csubc(caddc(cscale(d1g[pp], kx), cscale(d2v[pp], kz)), cscale(vv, mul(kz, k2)));
@@ -328,16 +330,48 @@ __global__ __launch_bounds__(256) void k_vor_from_vel(Chan c) {
     }
 }
 
-// v, omega_y sit in slot 0 and Hv, Hg in slot 1 (written with ring 0): copy each to all levels
+// DY2 v of the initial field (slot 0), with the mat-vec of the horizontal kernel
+template <int TP>
+__global__ __launch_bounds__(256) void k_init_d2v(Chan c) {
+    __shared__ double2 sv[TP][kMaxNY];
+    const int N = c.NY;
+    const double2 *v = hist(c, H_V, 0);
+    const int p0 = blockIdx.x * TP;
+    for (int e = threadIdx.x; e < TP * N; e += blockDim.x) {
+        const int pp = e / N, j = e % N, p = p0 + pp;
+        sv[pp][j] = p < c.npencil ? v[(size_t)p * N + j] : c2(0, 0);
+    }
+    __syncthreads();
+    const int row = threadIdx.x;
+    if (row >= N) return;
+    double2 acc[TP];
+#pragma unroll
+    for (int pp = 0; pp < TP; ++pp) acc[pp] = c2(0, 0);
+    for (int j = 0; j < N; ++j) {
+        const double b = c.DY2[(size_t)j * N + row];
+#pragma unroll
+        for (int pp = 0; pp < TP; ++pp) {
+            acc[pp].x = fma(b, sv[pp][j].x, acc[pp].x);
+            acc[pp].y = fma(b, sv[pp][j].y, acc[pp].y);
+        }
+    }
+#pragma unroll
+    for (int pp = 0; pp < TP; ++pp)
+        if (p0 + pp < c.npencil) c.d2v[(size_t)(p0 + pp) * N + row] = acc[pp];
+}
+
+// v, omega_y and DY2 v sit in slot 0 and Hv, Hg in slot 1 (written with ring 0):
+// copy each to all levels
 __global__ void k_init_history(Chan c) {
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < c.W; e += (size_t)gridDim.x * blockDim.x) {
-        const double2 v = hist(c, H_V, 0)[e], g = hist(c, H_G, 0)[e];
+        const double2 v = hist(c, H_V, 0)[e], g = hist(c, H_G, 0)[e], dv = c.d2v[e];
         const double2 hv = hist(c, H_HV, 1)[e], hg = hist(c, H_HG, 1)[e];
         for (int s = 0; s < 3; ++s) {
             hist(c, H_V, s)[e] = v;
             hist(c, H_G, s)[e] = g;
             hist(c, H_HV, s)[e] = hv;
             hist(c, H_HG, s)[e] = hg;
+            c.d2v[(size_t)s * c.W + e] = dv;
         }
     }
 }
@@ -386,6 +420,10 @@ void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s) {
 
 void launch_vor_from_vel(const Chan &c, hipStream_t s) {
     hipLaunchKernelGGL(k_vor_from_vel<kTP>, dim3(blocks_tp(c)), dim3(256), 0, s, c);
+}
+
+void launch_init_d2v(const Chan &c, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_d2v<kTP>, dim3(blocks_tp(c)), dim3(256), 0, s, c);
 }
 
 void launch_init_history(const Chan &c, hipStream_t s) { hipLaunchKernelGGL(k_init_history, dim3(1024), dim3(256), 0, s, c); }
