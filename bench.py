@@ -50,8 +50,10 @@ def stage_bytes(W, I, es, N, nfac):
     c = 16
     factors = 2 * nfac * N * N * 8
     return {
-        "solve": (12 * W + 2 * W) * c + factors,
-        "horizontal": (2 * W + 4 * W) * c,
+        # v: 3 levels of v, Hv and DY2 v; omega_y: 3 levels of omega_y and Hg
+        "solve": (15 * W + 2 * W) * c + factors,
+        # reads v, omega_y; writes u, w, omega_x, omega_z and DY2 v
+        "horizontal": (2 * W + 5 * W) * c,
         "mean": 0,
         "z_inverse": 6 * W * c + 6 * I * es,
         "x_lamb": 6 * I * es + 3 * I * es,
