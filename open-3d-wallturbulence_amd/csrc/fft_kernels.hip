@@ -142,6 +142,19 @@ __device__ __forceinline__ const double2 *tw_of<double2>(const Chan &c, bool xdi
 template <>
 __device__ __forceinline__ const float2 *tw_of<float2>(const Chan &c, bool xdir) { return xdir ? c.twxf : c.twzf; }
 
+// Tile index of a 1-D grid launched with 8 * ceil(ntiles / 8) blocks. The
+// hardware hands consecutive workgroups to the 8 XCDs in turn (block b runs on
+// XCD b % 8, each XCD with its own L2), so tiles are numbered to give every XCD
+// a contiguous range: the y-tiles of one line set, which share cache lines,
+// run side by side in one L2. Padding blocks get -1.
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+    const int b = blockIdx.x, per = (ntiles + 7) / 8;
+    const int t = (b % 8) * per + b / 8;
+    return t < ntiles ? t : -1;
+}
+
+static inline int xcd_grid(int ntiles) { return 8 * ((ntiles + 7) / 8); }
+
 // ---------------------------------------------------------------------------
 // z inverse (src/FFT.cpp:68-95): per (kx index i, y tile, field)
 // ---------------------------------------------------------------------------
@@ -151,15 +164,18 @@ struct PtrTable {
 };
 
 template <typename T, int TJ, int TL, int LMAX>
-__global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typename CT<T>::t *out) {
+__global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typename CT<T>::t *out, int nf) {
     using C = typename CT<T>::t;
     extern __shared__ unsigned char smem[];
     C *tw = reinterpret_cast<C *>(smem);
     C *buf = tw + LMAX;
     const int L = c.NZ2, NZ = c.NZ, NZH = c.NZH, NXH = c.NXH, NY = c.NY;
-    const int i = blockIdx.x, f = blockIdx.z;
+    const int nyt = (NY + TJ - 1) / TJ;
+    const int tile = xcd_tile(nyt * NXH * nf);
+    if (tile < 0) return;
+    const int i = (tile / nyt) % NXH, f = tile / (nyt * NXH);
     const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
-    const int j = blockIdx.y * TJ + jj;
+    const int j = (tile % nyt) * TJ + jj;
     const bool valid = j < NY;
     const C *twg = tw_of<C>(c, false);
     for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = twg[n];
@@ -182,15 +198,18 @@ __global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typen
 // z forward (src/FFT.cpp:171-190): per (i, y tile, field)
 // ---------------------------------------------------------------------------
 template <typename T, int TJ, int TL, int LMAX, typename OutC>
-__global__ __launch_bounds__(TJ *TL) void k_z_forward(Chan c, const typename CT<T>::t *in, OutC *out) {
+__global__ __launch_bounds__(TJ *TL) void k_z_forward(Chan c, const typename CT<T>::t *in, OutC *out, int nf) {
     using C = typename CT<T>::t;
     extern __shared__ unsigned char smem[];
     C *tw = reinterpret_cast<C *>(smem);
     C *buf = tw + LMAX;
     const int L = c.NZ2, NZ = c.NZ, NZH = c.NZH, NXH = c.NXH, NY = c.NY;
-    const int i = blockIdx.x, f = blockIdx.z;
+    const int nyt = (NY + TJ - 1) / TJ;
+    const int tile = xcd_tile(nyt * NXH * nf);
+    if (tile < 0) return;
+    const int i = (tile / nyt) % NXH, f = tile / (nyt * NXH);
     const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
-    const int j = blockIdx.y * TJ + jj;
+    const int j = (tile % nyt) * TJ + jj;
     const bool valid = j < NY;
     const C *twg = tw_of<C>(c, false);
     for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = twg[n];
@@ -245,9 +264,12 @@ __global__ __launch_bounds__(TJ *TL) void k_x_lamb(Chan c, const typename CT<T>:
     C *tw = reinterpret_cast<C *>(smem);
     C *buf = tw + LMAX;
     const int L = c.NX2, NXH = c.NXH, NY = c.NY;
-    const int k2 = blockIdx.x;
+    const int nyt = (NY + TJ - 1) / TJ;
+    const int tile = xcd_tile(nyt * c.NZ2);
+    if (tile < 0) return;
+    const int k2 = tile / nyt;
     const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
-    const int j = blockIdx.y * TJ + jj;
+    const int j = (tile % nyt) * TJ + jj;
     const bool valid = j < NY;
     const C *twg = tw_of<C>(c, true);
     for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = twg[n];
@@ -315,9 +337,12 @@ __global__ __launch_bounds__(TJ *TL) void k_x_c2r(Chan c, const double2 *in, int
     C *tw = reinterpret_cast<C *>(smem);
     C *buf = tw + LMAX;
     const int L = c.NX2, NXH = c.NXH, NY = c.NY;
-    const int k2 = blockIdx.x, pr = blockIdx.z;
+    const int nyt = (NY + TJ - 1) / TJ;
+    const int tile = xcd_tile(nyt * c.NZ2 * ((nf + 1) / 2));
+    if (tile < 0) return;
+    const int k2 = (tile / nyt) % c.NZ2, pr = tile / (nyt * c.NZ2);
     const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
-    const int j = blockIdx.y * TJ + jj;
+    const int j = (tile % nyt) * TJ + jj;
     const bool valid = j < NY;
     for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = c.twx[n];
     const size_t plane = (size_t)k2 * NXH * NY + j;
@@ -359,9 +384,12 @@ __global__ __launch_bounds__(TJ *TL) void k_x_r2c(Chan c, const double *phys, in
     C *tw = reinterpret_cast<C *>(smem);
     C *buf = tw + LMAX;
     const int L = c.NX2, NXH = c.NXH, NY = c.NY;
-    const int k2 = blockIdx.x, pr = blockIdx.z;
+    const int nyt = (NY + TJ - 1) / TJ;
+    const int tile = xcd_tile(nyt * c.NZ2 * ((nf + 1) / 2));
+    if (tile < 0) return;
+    const int k2 = (tile / nyt) % c.NZ2, pr = tile / (nyt * c.NZ2);
     const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
-    const int j = blockIdx.y * TJ + jj;
+    const int j = (tile % nyt) * TJ + jj;
     const bool valid = j < NY;
     for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = c.twx[n];
     const int f0 = 2 * pr, f1 = 2 * pr + 1;
@@ -445,28 +473,28 @@ void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out
     for (int f = 0; f < nf; ++f) t.p[f] = in[f];
     if (c.precision == 0)
         OWT_Z_DISPATCH(c.NZ2, double, {
-            dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, nf);
+            dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * nf));
             hipLaunchKernelGGL((k_z_inverse<double, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, t,
-                               (double2 *)out);
+                               (double2 *)out, nf);
         });
     else
         OWT_Z_DISPATCH(c.NZ2, float, {
-            dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, nf);
+            dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * nf));
             hipLaunchKernelGGL((k_z_inverse<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c, t,
-                               (float2 *)out);
+                               (float2 *)out, nf);
         });
 }
 
 void launch_x_lamb(const Chan &c, hipStream_t s) {
     if (c.precision == 0)
         OWT_X_DISPATCH(c.NX2, double, {
-            dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ);
+            dim3 g(xcd_grid(c.NZ2 * ((c.NY + TJ - 1) / TJ)));
             hipLaunchKernelGGL((k_x_lamb<double, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c,
                                (const double2 *)c.zbuf, (double2 *)c.xbuf);
         });
     else
         OWT_X_DISPATCH(c.NX2, float, {
-            dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ);
+            dim3 g(xcd_grid(c.NZ2 * ((c.NY + TJ - 1) / TJ)));
             hipLaunchKernelGGL((k_x_lamb<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
                                (const float2 *)c.zbuf, (float2 *)c.xbuf);
         });
@@ -475,15 +503,15 @@ void launch_x_lamb(const Chan &c, hipStream_t s) {
 void launch_z_forward(const Chan &c, hipStream_t s) {
     if (c.precision == 0)
         OWT_Z_DISPATCH(c.NZ2, double, {
-            dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, 3);
+            dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * 3));
             hipLaunchKernelGGL((k_z_forward<double, TJ, TL, LMAX, double2>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s,
-                               c, (const double2 *)c.xbuf, (double2 *)c.lmb);
+                               c, (const double2 *)c.xbuf, (double2 *)c.lmb, 3);
         });
     else
         OWT_Z_DISPATCH(c.NZ2, float, {
-            dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, 3);
+            dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * 3));
             hipLaunchKernelGGL((k_z_forward<float, TJ, TL, LMAX, float2>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
-                               (const float2 *)c.xbuf, (float2 *)c.lmb);
+                               (const float2 *)c.xbuf, (float2 *)c.lmb, 3);
         });
 }
 
@@ -491,15 +519,15 @@ void launch_z_forward_to(const Chan &c, const void *in, int nf, double2 *out, hi
     Chan d = c;
     d.precision = 0;
     OWT_Z_DISPATCH(c.NZ2, double, {
-        dim3 g(c.NXH, (c.NY + TJ - 1) / TJ, nf);
+        dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * nf));
         hipLaunchKernelGGL((k_z_forward<double, TJ, TL, LMAX, double2>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, d,
-                           (const double2 *)in, out);
+                           (const double2 *)in, out, nf);
     });
 }
 
 void launch_x_c2r(const Chan &c, const void *in, int nf, double *phys, hipStream_t s) {
     OWT_X_DISPATCH(c.NX2, double, {
-        dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ, (nf + 1) / 2);
+        dim3 g(xcd_grid(c.NZ2 * ((c.NY + TJ - 1) / TJ) * ((nf + 1) / 2)));
         hipLaunchKernelGGL((k_x_c2r<TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c,
                            (const double2 *)in, nf, phys);
     });
@@ -507,7 +535,7 @@ void launch_x_c2r(const Chan &c, const void *in, int nf, double *phys, hipStream
 
 void launch_x_r2c(const Chan &c, const double *phys, int nf, void *out, hipStream_t s) {
     OWT_X_DISPATCH(c.NX2, double, {
-        dim3 g(c.NZ2, (c.NY + TJ - 1) / TJ, (nf + 1) / 2);
+        dim3 g(xcd_grid(c.NZ2 * ((c.NY + TJ - 1) / TJ) * ((nf + 1) / 2)));
         hipLaunchKernelGGL((k_x_r2c<TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, phys, nf,
                            (double2 *)out);
     });

@@ -62,62 +62,82 @@ __device__ __forceinline__ void lu_gather(const double (*S)[64 * NQ], const int 
 //  - Steps run block by block: step k belongs to row block qk = k / 64, a
 //    compile-time index, so the pivot value is read from a fixed register of
 //    lane k % 64 and only the row blocks that meet the triangle are touched.
-//  - Columns are loaded CK at a time, all loads in flight before the first is
-//    used. Every lane of a touched block loads: a lane whose row lies outside
-//    the triangle reads the column's diagonal element instead (one cache line
-//    for all such lanes) and its value is masked, so no load sits behind a
-//    branch and only the triangle is fetched.
+//  - Columns come in chunks of CK, and the loads of the next chunk are issued
+//    before the current one is used (two buffers), so a wave always has factor
+//    loads in flight. Every lane of a touched block loads: a lane whose row lies
+//    outside the triangle reads the column's diagonal element instead (one
+//    cache line for all such lanes) and its value is masked, so no load sits
+//    behind a branch and only the triangle is fetched.
 //  - The back substitution scales by the reciprocal of the diagonal, as
 //    optimized trsm kernels do (one division per column).
-template <int NQ, int CK = 8>
-__device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int N, int lane, double (&t)[4][NQ]) {
+template <int NQ, int CK>
+__device__ __forceinline__ void lu_load_lower(const double *__restrict__ Fo, int N, int lane, int qk, int kb, double (&cb)[CK][NQ]) {
 #pragma unroll
-    for (int qk = 0; qk < NQ; ++qk) {
-        for (int l0 = 0; l0 < 64; l0 += CK) {
-            const int kb = 64 * qk + l0;
-            if (kb >= N) break;
-            double cb[CK][NQ];
+    for (int c = 0; c < CK; ++c) {
+        const int kc = kb + c < N ? kb + c : N - 1;
+        const double *col = Fo + (size_t)kc * N;
 #pragma unroll
-            for (int c = 0; c < CK; ++c) {
-                const int kc = kb + c < N ? kb + c : N - 1;
-                const double *col = Fo + (size_t)kc * N;
-#pragma unroll
-                for (int q = qk; q < NQ; ++q) {
-                    const int row = lane + 64 * q;
-                    cb[c][q] = col[(row > kc && row < N) ? row : kc];
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < CK; ++c) {
-                const int k = kb + c;
-#pragma unroll
-                for (int q = qk; q < NQ; ++q) {
-                    const int row = lane + 64 * q;
-                    if (!(row > k && row < N)) cb[c][q] = 0.0;
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < CK; ++c) {
-                const int k = kb + c;
-                if (k >= N) break;
-                double yk[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) yk[r] = readlane_d(t[r][qk], l0 + c);
-#pragma unroll
-                for (int q = qk; q < NQ; ++q)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) t[r][q] = fma(-cb[c][q], yk[r], t[r][q]);
-            }
+        for (int q = 0; q < NQ; ++q) {
+            if (q < qk) continue;
+            const int row = lane + 64 * q;
+            cb[c][q] = col[(row > kc && row < N) ? row : kc];
         }
     }
+}
+
+template <int NQ, int CK>
+__device__ __forceinline__ void lu_apply_lower(int N, int lane, int qk, int kb, double (&cb)[CK][NQ], double (&t)[4][NQ]) {
+#pragma unroll
+    for (int c = 0; c < CK; ++c) {
+        const int k = kb + c;
+        if (k >= N) break;
+        double yk[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) yk[r] = readlane_d(t[r][qk], k & 63);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            if (q < qk) continue;
+            const int row = lane + 64 * q;
+            const double m = (row > k && row < N) ? cb[c][q] : 0.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) t[r][q] = fma(-m, yk[r], t[r][q]);
+        }
+    }
+}
+
+template <int NQ, int CK = 4>
+__device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int N, int lane, double (&t)[4][NQ]) {
+    // forward: chunks of CK columns inside each row block, next chunk in flight
+#pragma unroll
+    for (int qk = 0; qk < NQ; ++qk) {
+        const int kend = 64 * qk + 64 < N ? 64 * qk + 64 : N;
+        int kb = 64 * qk;
+        if (kb >= kend) break;
+        double ca[CK][NQ], cc[CK][NQ];
+        lu_load_lower<NQ, CK>(Fo, N, lane, qk, kb, ca);
+        while (kb < kend) {
+            const bool more = kb + CK < kend;
+            if (more) lu_load_lower<NQ, CK>(Fo, N, lane, qk, kb + CK, cc);
+            lu_apply_lower<NQ, CK>(N, lane, qk, kb, ca, t);
+            kb += CK;
+            if (!more) break;
+            const bool more2 = kb + CK < kend;
+            if (more2) lu_load_lower<NQ, CK>(Fo, N, lane, qk, kb + CK, ca);
+            lu_apply_lower<NQ, CK>(N, lane, qk, kb, cc, t);
+            kb += CK;
+            if (!more2) break;
+        }
+    }
+    // backward: the same, columns from the top of each row block down
 #pragma unroll
     for (int qk = NQ - 1; qk >= 0; --qk) {
-        const int ktop = (64 * qk + 63 < N - 1 ? 64 * qk + 63 : N - 1);
-        for (int kb = ktop; kb >= 64 * qk; kb -= CK) {
-            double cb[CK][NQ];
+        const int lo = 64 * qk;
+        int kb = (lo + 63 < N - 1 ? lo + 63 : N - 1);
+        if (kb < lo) continue;
+        auto load = [&](int k0, double (&cb)[CK][NQ]) {
 #pragma unroll
             for (int c = 0; c < CK; ++c) {
-                const int kc = kb - c >= 64 * qk ? kb - c : 64 * qk;
+                const int kc = k0 - c >= lo ? k0 - c : lo;
                 const double *col = Fo + (size_t)kc * N;
 #pragma unroll
                 for (int q = 0; q <= qk; ++q) {
@@ -125,10 +145,12 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
                     cb[c][q] = col[row <= kc ? row : kc];
                 }
             }
+        };
+        auto apply = [&](int k0, double (&cb)[CK][NQ]) {
 #pragma unroll
             for (int c = 0; c < CK; ++c) {
-                const int k = kb - c;
-                if (k < 64 * qk) break;
+                const int k = k0 - c;
+                if (k < lo) break;
                 const int lk = k & 63;
                 const double rinv = 1.0 / readlane_d(cb[c][qk], lk);
                 double xk[4];
@@ -146,6 +168,20 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
                     for (int r = 0; r < 4; ++r) t[r][q] = fma(-m, xk[r], t[r][q]);
                 }
             }
+        };
+        double ca[CK][NQ], cc[CK][NQ];
+        load(kb, ca);
+        while (kb >= lo) {
+            const bool more = kb - CK >= lo;
+            if (more) load(kb - CK, cc);
+            apply(kb, ca);
+            kb -= CK;
+            if (!more) break;
+            const bool more2 = kb - CK >= lo;
+            if (more2) load(kb - CK, ca);
+            apply(kb, cc);
+            kb -= CK;
+            if (!more2) break;
         }
     }
 }
