@@ -18,6 +18,13 @@ FFTPlan make_plan(int L) {
     while (n % 3 == 0) p.radix[p.nst++] = 3, n /= 3;
     while (n % 2 == 0) p.radix[p.nst++] = 2, n /= 2;
     if (n != 1 || p.nst > kMaxStages) throw Error(2, "transform length " + std::to_string(L) + " is not 2^a 3^b");
+    int ns = 1;
+    for (int s = 0; s < p.nst; ++s) {
+        p.ns[s] = ns;
+        p.tstep[s] = L / (ns * p.radix[s]);
+        p.mns[s] = ns == 1 ? 0u : (unsigned)((((unsigned long long)1 << 32) + (unsigned)ns - 1) / (unsigned)ns);
+        ns *= p.radix[s];
+    }
     return p;
 }
 

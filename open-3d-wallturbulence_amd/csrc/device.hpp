@@ -13,10 +13,16 @@ constexpr int kMaxStages = 8;
 constexpr int kMaxNY = 256;
 
 // 1-D transform plan: length L = prod(radix[s]), radices in {2, 3, 4}.
+// Stage s has span ns[s] (product of the earlier radices), twiddle stride
+// tstep[s] = L / (ns[s] * radix[s]) and mns[s] = ceil(2^32 / ns[s]), with which
+// j / ns[s] = umulhi(j, mns[s]) exactly for j < 2^16 (mns = 0 marks ns = 1).
 struct FFTPlan {
     int L = 0;
     int nst = 0;
     int radix[kMaxStages] = {0};
+    int ns[kMaxStages] = {0};
+    int tstep[kMaxStages] = {0};
+    unsigned mns[kMaxStages] = {0};
 };
 
 // history field indices inside Chan::hist ([field][slot] of W elements each)

@@ -73,17 +73,17 @@ __device__ __forceinline__ void dft(C (&x)[R], int sign) {
 }
 
 // One Stockham stage of radix R over TJ lines of length L held in buf[n*TJ + jj].
-// Ns = product of the radices of the earlier stages. tw[m] = e^{-2 pi i m/L}.
+// Ns = product of the radices of the earlier stages, mNs its division magic,
+// tstep = L / (Ns R). tw[m] = e^{-2 pi i m/L}.
 template <typename C, int R, int TJ, int TL, int MAXB>
-__device__ __forceinline__ void stage(C *buf, const C *tw, int L, int Ns, int sign, int jj, int tl) {
+__device__ __forceinline__ void stage(C *buf, const C *tw, int L, int Ns, unsigned mNs, int tstep, int sign, int jj, int tl) {
     const int Lr = L / R;
-    const int tstep = L / (Ns * R);
     C out[MAXB][R];
 #pragma unroll
     for (int b = 0; b < MAXB; ++b) {
         const int j = tl + b * TL;
         if (j < Lr) {
-            const int k = j % Ns;
+            const int k = j - (int)(mNs ? __umulhi((unsigned)j, mNs) : (unsigned)j) * Ns;
             C x[R];
 #pragma unroll
             for (int q = 0; q < R; ++q) x[q] = buf[(j + q * Lr) * TJ + jj];
@@ -103,7 +103,7 @@ __device__ __forceinline__ void stage(C *buf, const C *tw, int L, int Ns, int si
     for (int b = 0; b < MAXB; ++b) {
         const int j = tl + b * TL;
         if (j < Lr) {
-            const int k = j % Ns;
+            const int k = j - (int)(mNs ? __umulhi((unsigned)j, mNs) : (unsigned)j) * Ns;
             const int base = (j - k) * R + k;
 #pragma unroll
             for (int q = 0; q < R; ++q) buf[(base + q * Ns) * TJ + jj] = out[b][q];
@@ -116,16 +116,14 @@ __device__ __forceinline__ void stage(C *buf, const C *tw, int L, int Ns, int si
 // Callers synchronise before the call; the call ends synchronised.
 template <typename C, int TJ, int TL, int LMAX>
 __device__ void fft_lines(C *buf, const C *tw, const FFTPlan &P, int sign, int jj, int tl) {
-    int Ns = 1;
     for (int s = 0; s < P.nst; ++s) {
         const int R = P.radix[s];
         if (R == 4)
-            stage<C, 4, TJ, TL, (LMAX / 4 + TL - 1) / TL>(buf, tw, P.L, Ns, sign, jj, tl);
+            stage<C, 4, TJ, TL, (LMAX / 4 + TL - 1) / TL>(buf, tw, P.L, P.ns[s], P.mns[s], P.tstep[s], sign, jj, tl);
         else if (R == 3)
-            stage<C, 3, TJ, TL, (LMAX / 3 + TL - 1) / TL>(buf, tw, P.L, Ns, sign, jj, tl);
+            stage<C, 3, TJ, TL, (LMAX / 3 + TL - 1) / TL>(buf, tw, P.L, P.ns[s], P.mns[s], P.tstep[s], sign, jj, tl);
         else
-            stage<C, 2, TJ, TL, (LMAX / 2 + TL - 1) / TL>(buf, tw, P.L, Ns, sign, jj, tl);
-        Ns *= R;
+            stage<C, 2, TJ, TL, (LMAX / 2 + TL - 1) / TL>(buf, tw, P.L, P.ns[s], P.mns[s], P.tstep[s], sign, jj, tl);
     }
 }
 
