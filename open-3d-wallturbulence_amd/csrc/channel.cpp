@@ -174,24 +174,24 @@ void Channel::factor_operators() {
             volatile double b = (kzv * P_.dkz) * (kzv * P_.dkz);
             groups[a + b].push_back(sys);
         }
-        std::vector<int> jobs, frep;
+        std::vector<int4> jobs;
+        std::vector<int> frep;
         fmap_host_.assign(ns, 0);
         for (auto &g : groups) {
             const int f = (int)frep.size();
-            frep.push_back(g.second.front());
-            for (int sys : g.second) {
-                fmap_host_[sys] = f;
-                jobs.push_back(sys);
-            }
+            const std::vector<int> &sys = g.second;
+            frep.push_back(sys.front());
+            for (int s : sys) fmap_host_[s] = f;
+            for (size_t a = 0; a < sys.size(); a += 2)
+                jobs.push_back(make_int4(f, sys[a], a + 1 < sys.size() ? sys[a + 1] : -1, 0));
         }
         nfac_ = (int)frep.size();
+        njobs_ = (int)jobs.size();
         Fb_ = (double *)dmalloc(2 * (size_t)nfac_ * N * N * sizeof(double));
         Pb_ = (int *)dmalloc(2 * (size_t)nfac_ * N * sizeof(int));
-        jobs_ = (int *)dmalloc(ns * sizeof(int));
-        fmap_ = (int *)dmalloc(ns * sizeof(int));
+        jobs_ = (int4 *)dmalloc(njobs_ * sizeof(int4));
         frep_ = (int *)dmalloc(nfac_ * sizeof(int));
-        OWT_HIP(hipMemcpy(jobs_, jobs.data(), ns * sizeof(int), hipMemcpyHostToDevice));
-        OWT_HIP(hipMemcpy(fmap_, fmap_host_.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+        OWT_HIP(hipMemcpy(jobs_, jobs.data(), njobs_ * sizeof(int4), hipMemcpyHostToDevice));
         OWT_HIP(hipMemcpy(frep_, frep.data(), nfac_ * sizeof(int), hipMemcpyHostToDevice));
     }
     launch_factor(c_, Fb_, Pb_, nfac_, frep_, s_);
@@ -518,7 +518,7 @@ void Channel::enqueue_step_body(int ring) {
     const double *F = plain ? Fp_ : Fb_;
     const int *P = plain ? Pp_ : Pb_;
     if (plain) launch_factor(c_, Fp_, Pp_, c_.npencil, nullptr, s_);
-    launch_solve(c_, sp, F, P, jobs_, fmap_, plain, s_);
+    launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, s_);
     launch_horizontal(c_, sp, s_);
     launch_mean(c_, sp, F, P, plain ? 0 : fmap_host_[0], s_);
     // the six transformed fields: u, v, w, omega_x, omega_y, omega_z of the new level
@@ -568,7 +568,7 @@ void Channel::enqueue_kernel(int id) {
     const size_t W = c_.W;
     const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
     switch (id) {
-        case 0: launch_solve(c_, sp, F, P, jobs_, fmap_, plain, s_); break;
+        case 0: launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, s_); break;
         case 1: launch_horizontal(c_, sp, s_); break;
         case 2: launch_mean(c_, sp, F, P, plain ? 0 : fmap_host_[0], s_); break;
         case 3: launch_z_inverse(c_, six, 6, c_.zbuf, s_); break;

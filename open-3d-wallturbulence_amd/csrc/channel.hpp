@@ -82,10 +82,12 @@ class Channel {
     double *Fb_ = nullptr, *Fp_ = nullptr;
     int *Pb_ = nullptr, *Pp_ = nullptr;
     bool factored_ = false;
-    // batched factor tables: one factor per distinct k^2; jobs (systems in
-    // factor order), fmap (system -> factor), frep (factor -> a system)
-    int nfac_ = 0;
-    int *jobs_ = nullptr, *fmap_ = nullptr, *frep_ = nullptr;
+    // batched factor tables: one factor per distinct k^2; jobs ({factor, system,
+    // system or -1}: systems sharing a factor, two at a time), frep (factor -> a
+    // system); fmap_host_ (system -> factor)
+    int nfac_ = 0, njobs_ = 0;
+    int4 *jobs_ = nullptr;
+    int *frep_ = nullptr;
     std::vector<int> fmap_host_;
     // history ring phase: slot of the current level; one captured graph per phase
     int ring_ = 0;

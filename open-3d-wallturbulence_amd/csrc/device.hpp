@@ -89,9 +89,9 @@ void launch_peak3(const double *phys, size_t n, unsigned long long *out, hipStre
 // LU factors of Av and Ag for nfac factor indices (see lu_kernels.hip);
 // frep[f]: representative system or pencil of factor f (null: f itself)
 void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, hipStream_t s);
-// right-hand sides + solves; plain: one job per pencil with its own factor;
-// batched: job j solves system jobs[j] with factor fmap[jobs[j]]
-void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int *jobs, const int *fmap, bool plain,
+// right-hand sides + solves; plain: njobs = pencils, each with its own factor;
+// batched: job j = {factor, system, system or -1}
+void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int4 *jobs, int njobs, bool plain,
                   hipStream_t s);
 void launch_horizontal(const Chan &c, const StepPtrs &sp, hipStream_t s);
 // mean flow, friction force and the k = 0 pencil (factor mf holds Ag(0))

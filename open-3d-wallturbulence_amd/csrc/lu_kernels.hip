@@ -21,8 +21,9 @@
 //           step, as the reference does;
 //   batched one per distinct value of k^2: the operators depend on k^2 only,
 //           so +kz and -kz, and every other pair (kx, kz) with the same k^2
-//           bit for bit, share one factor, computed once. fmap[s] maps a
-//           system s = mz*NXH + i (mz in [0, NZH]) to its factor.
+//           bit for bit, share one factor, computed once; the systems
+//           s = mz*NXH + i (mz in [0, NZH]) that share a factor are solved
+//           together.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -209,28 +210,40 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, c
 // ---------------------------------------------------------------------------
 // Right-hand sides (src/solver.cpp:568-574, 593-599) and dgetrs.
 // Four waves per block: wave w takes job 2*block + w/2 and operator w%2.
-// A batched job is a system jobs[job] carrying the +kz and -kz pencils (four
-// real right-hand sides); jobs are ordered by factor so that systems sharing a
-// factor run side by side and its second reading comes from cache. A plain
-// job is one pencil with its own factor.
+// A batched job applies one factor to up to two +-kz systems that share it
+// (jobs[j] = {factor, system, system or -1}): up to four pencils, eight real
+// right-hand sides, one pass over the factor. A plain job is one pencil with
+// its own factor. Each lane forms the right-hand side directly at the original
+// row that its position in the pivot order names (dlaswp folded into the
+// loads), so the solve needs no data exchange before the substitution.
 // ---------------------------------------------------------------------------
 template <int NQ, bool PLAIN>
-__global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P, const int *jobs,
-                                               const int *fmap) {
-    constexpr bool plain = PLAIN;
-    __shared__ double sh[4][4][64 * NQ];
+__global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P, const int4 *jobs, int njobs) {
+    constexpr int NP = PLAIN ? 1 : 4;  // pencils per job
+    constexpr int NR = 2 * NP;         // real right-hand sides
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int job = blockIdx.x * 2 + (w >> 1), op = w & 1;
     const int N = c.NY;
-    const int njobs = plain ? c.npencil : c.nsys;
-    const bool live = job < njobs;
-    const int sys = live ? (plain ? job : jobs[job]) : 0;
-    const int fac = plain ? sys : fmap[sys];
-    const int kz = sys / c.NXH, ix = sys % c.NXH;
-    const int p0 = sys;
-    const int p1 = (live && !plain && kz > 0 && kz < c.NZH) ? (c.NZ - kz) * c.NXH + ix : -1;
-    double(*S)[64 * NQ] = sh[w];
-
+    if (job >= njobs) return;
+    int fac, pc[NP], kz, ix;
+    if (PLAIN) {
+        fac = job;
+        pc[0] = job;
+        kz = job / c.NXH;
+        ix = job % c.NXH;
+    } else {
+        const int4 jb = jobs[job];
+        fac = jb.x;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int sys = h == 0 ? jb.y : jb.z;
+            const int mz = sys >= 0 ? sys / c.NXH : 0, i = sys >= 0 ? sys % c.NXH : 0;
+            pc[2 * h] = sys >= 0 ? sys : -1;
+            pc[2 * h + 1] = (sys >= 0 && mz > 0 && mz < c.NZH) ? (c.NZ - mz) * c.NXH + i : -1;
+        }
+        kz = jb.y / c.NXH;
+        ix = jb.y % c.NXH;
+    }
     const double2 *X0 = op == 0 ? sp.v[0] : sp.g[0], *X1 = op == 0 ? sp.v[1] : sp.g[1], *X2 = op == 0 ? sp.v[2] : sp.g[2];
     const double2 *H0 = op == 0 ? sp.hv[0] : sp.hg[0], *H1 = op == 0 ? sp.hv[1] : sp.hg[1], *H2 = op == 0 ? sp.hv[2] : sp.hg[2];
     const double2 *D0 = sp.dv[0], *D1 = sp.dv[1], *D2 = sp.dv[2];
@@ -238,62 +251,45 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
     const double dt = c.dt;
     const double k2 = k2_of(c, kz, ix);
     const double k2dt = __ddiv_rn(k2, dt);
-
-    // 1. combine the history levels (loads from clamped, valid addresses; masked after).
-    // The v equation's DY2 (sum alpha v) / dt is formed from the stored DY2 v of
-    // each level, sum alpha (DY2 v), which the horizontal kernel computed when
-    // each level was new.
-    double t[4][NQ];
-    const int p0c = live ? p0 : 0;
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int row = lane + 64 * q;
-        const int rowc = row < N ? row : N - 1;
-#pragma unroll
-        for (int pp = 0; pp < 2; ++pp) {
-            const int pn = pp == 0 ? p0 : p1;
-            const size_t o = (size_t)(pp == 0 || p1 < 0 ? p0c : p1) * N + rowc;
-            double2 a = caddc(caddc(cscale(X0[o], AL0), cscale(X1[o], AL1)), cscale(X2[o], AL2));
-            double2 h = caddc(caddc(cscale(H0[o], BE0), cscale(H1[o], BE1)), cscale(H2[o], BE2));
-            double2 s;
-            if (op == 0) {
-                const double2 d = caddc(caddc(cscale(D0[o], AL0), cscale(D1[o], AL1)), cscale(D2[o], AL2));
-                s = caddc(csubc(h, cscale(a, k2dt)), cdiv(d, dt));
-            } else {
-                s = caddc(h, cdiv(a, dt));
-            }
-            if (!(live && row < N && pn >= 0)) s = c2(0, 0);
-            t[2 * pp][q] = s.x;
-            t[2 * pp + 1][q] = s.y;
-        }
-    }
-    // boundary rows
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const int row = lane + 64 * q;
-        const bool bc = op == 0 ? (row <= 1 || row >= N - 2) : (row == 0 || row == N - 1);
-        if (bc || row >= N)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) t[r][q] = 0.0;
-    }
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) S[r][lane + 64 * q] = t[r][q];
-    __syncthreads();
-    if (!live) return;
-    // 3. rows in pivot order, forward and back substitution
     const double *Fo = F + ((size_t)fac * 2 + op) * N * N;
     const int *Po = P + ((size_t)fac * 2 + op) * N;
-    lu_gather<NQ>(S, Po, N, lane, t);
-    lu_substitute<NQ>(Fo, N, lane, t);
+
+    // right-hand sides at the pivot rows. The v equation's DY2 (sum alpha v) / dt
+    // is formed from the stored DY2 v of each level, which the horizontal kernel
+    // computed when that level was new. Loads read valid addresses (missing
+    // pencils read the first one) and are masked afterwards.
+    double t[NR][NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int pos = lane + 64 * q;
+        int r = pos < N ? Po[pos] : 0;
+        if ((unsigned)r >= (unsigned)N) r = 0;
+        const bool bc = op == 0 ? (r <= 1 || r >= N - 2) : (r == 0 || r == N - 1);
+#pragma unroll
+        for (int pp = 0; pp < NP; ++pp) {
+            const size_t o = (size_t)(pc[pp] >= 0 ? pc[pp] : pc[0]) * N + r;
+            const double2 a = caddc(caddc(cscale(X0[o], AL0), cscale(X1[o], AL1)), cscale(X2[o], AL2));
+            const double2 h = caddc(caddc(cscale(H0[o], BE0), cscale(H1[o], BE1)), cscale(H2[o], BE2));
+            double2 rhs;
+            if (op == 0) {
+                const double2 d = caddc(caddc(cscale(D0[o], AL0), cscale(D1[o], AL1)), cscale(D2[o], AL2));
+                rhs = caddc(csubc(h, cscale(a, k2dt)), cdiv(d, dt));
+            } else {
+                rhs = caddc(h, cdiv(a, dt));
+            }
+            if (bc || pos >= N || pc[pp] < 0) rhs = c2(0, 0);
+            t[2 * pp][q] = rhs.x;
+            t[2 * pp + 1][q] = rhs.y;
+        }
+    }
+    lu_substitute<NQ, NR>(Fo, N, lane, t);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int row = lane + 64 * q;
-        if (row < N) {
-            XN[(size_t)p0 * N + row] = c2(t[0][q], t[1][q]);
-            if (p1 >= 0) XN[(size_t)p1 * N + row] = c2(t[2][q], t[3][q]);
-        }
+        if (row >= N) continue;
+#pragma unroll
+        for (int pp = 0; pp < NP; ++pp)
+            if (pc[pp] >= 0) XN[(size_t)pc[pp] * N + row] = c2(t[2 * pp][q], t[2 * pp + 1][q]);
     }
 }
 
@@ -314,14 +310,13 @@ void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, 
     OWT_FACTOR_DISPATCH(c.NY, hipLaunchKernelGGL((k_factor<RPT, CPT, TC>), dim3(nfac, 2), dim3(64 * TC), 0, s, c, F, P, frep));
 }
 
-void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int *jobs, const int *fmap, bool plain,
+void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int4 *jobs, int njobs, bool plain,
                   hipStream_t s) {
-    const int njobs = plain ? c.npencil : c.nsys;
     const int blocks = (njobs + 1) / 2;
     if (plain)
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, true>), dim3(blocks), dim3(256), 0, s, c, sp, F, P, jobs, fmap));
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, true>), dim3(blocks), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
     else
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false>), dim3(blocks), dim3(256), 0, s, c, sp, F, P, jobs, fmap));
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false>), dim3(blocks), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
 }
 
 }  // namespace owt

@@ -85,28 +85,28 @@ __device__ __forceinline__ void lu_load_lower(const double *__restrict__ Fo, int
     }
 }
 
-template <int NQ, int CK>
-__device__ __forceinline__ void lu_apply_lower(int N, int lane, int qk, int kb, double (&cb)[CK][NQ], double (&t)[4][NQ]) {
+template <int NR, int NQ, int CK>
+__device__ __forceinline__ void lu_apply_lower(int N, int lane, int qk, int kb, double (&cb)[CK][NQ], double (&t)[NR][NQ]) {
 #pragma unroll
     for (int c = 0; c < CK; ++c) {
         const int k = kb + c;
         if (k >= N) break;
-        double yk[4];
+        double yk[NR];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) yk[r] = readlane_d(t[r][qk], k & 63);
+        for (int r = 0; r < NR; ++r) yk[r] = readlane_d(t[r][qk], k & 63);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             if (q < qk) continue;
             const int row = lane + 64 * q;
             const double m = (row > k && row < N) ? cb[c][q] : 0.0;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) t[r][q] = fma(-m, yk[r], t[r][q]);
+            for (int r = 0; r < NR; ++r) t[r][q] = fma(-m, yk[r], t[r][q]);
         }
     }
 }
 
-template <int NQ, int CK = 4>
-__device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int N, int lane, double (&t)[4][NQ]) {
+template <int NQ, int NR = 4, int CK = 4>
+__device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int N, int lane, double (&t)[NR][NQ]) {
     // forward: chunks of CK columns inside each row block, next chunk in flight
 #pragma unroll
     for (int qk = 0; qk < NQ; ++qk) {
@@ -118,12 +118,12 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
         while (kb < kend) {
             const bool more = kb + CK < kend;
             if (more) lu_load_lower<NQ, CK>(Fo, N, lane, qk, kb + CK, cc);
-            lu_apply_lower<NQ, CK>(N, lane, qk, kb, ca, t);
+            lu_apply_lower<NR, NQ, CK>(N, lane, qk, kb, ca, t);
             kb += CK;
             if (!more) break;
             const bool more2 = kb + CK < kend;
             if (more2) lu_load_lower<NQ, CK>(Fo, N, lane, qk, kb + CK, ca);
-            lu_apply_lower<NQ, CK>(N, lane, qk, kb, cc, t);
+            lu_apply_lower<NR, NQ, CK>(N, lane, qk, kb, cc, t);
             kb += CK;
             if (!more2) break;
         }
@@ -153,19 +153,19 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
                 if (k < lo) break;
                 const int lk = k & 63;
                 const double rinv = 1.0 / readlane_d(cb[c][qk], lk);
-                double xk[4];
+                double xk[NR];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) xk[r] = readlane_d(t[r][qk], lk) * rinv;
+                for (int r = 0; r < NR; ++r) xk[r] = readlane_d(t[r][qk], lk) * rinv;
                 if (lane == lk) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) t[r][qk] = xk[r];
+                    for (int r = 0; r < NR; ++r) t[r][qk] = xk[r];
                 }
 #pragma unroll
                 for (int q = 0; q <= qk; ++q) {
                     const int row = lane + 64 * q;
                     const double m = row < k ? cb[c][q] : 0.0;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) t[r][q] = fma(-m, xk[r], t[r][q]);
+                    for (int r = 0; r < NR; ++r) t[r][q] = fma(-m, xk[r], t[r][q]);
                 }
             }
         };
