@@ -40,7 +40,10 @@ enum {
     OWT_U = 0, OWT_V = 1, OWT_W = 2,           /* velx, vely, velz */
     OWT_OMX = 3, OWT_OMY = 4, OWT_OMZ = 5,     /* vorx, vory, vorz */
     OWT_LMBX = 6, OWT_LMBY = 7, OWT_LMBZ = 8,  /* Lamb vector */
-    OWT_HV = 9, OWT_HG = 10                     /* nonlinear sources of the v and omega_y equations */
+    OWT_HV = 9, OWT_HG = 10,                    /* nonlinear sources of the v and omega_y equations */
+    OWT_P = 11  /* dynamic pressure p + |u|^2/2, computed on request (the reference has no pressure,
+                   src/setup.hpp:58-79): (DY2 - k^2) P = i kx Lx + DY1 Ly + i kz Lz, wall rows
+                   DY1 P = Ly + nu DY2 v, k = 0 pencil gauged to zero mean over y */
 };
 
 /* methods and precisions */

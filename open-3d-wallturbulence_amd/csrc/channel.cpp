@@ -108,6 +108,7 @@ void Channel::alloc() {
     c.DY1 = (const double *)dmalloc(N * N * sizeof(double));
     c.DY2 = (const double *)dmalloc(N * N * sizeof(double));
     c.DY4 = (const double *)dmalloc(N * N * sizeof(double));
+    c.qw = (const double *)dmalloc(N * sizeof(double));
     c.kx = (const double *)dmalloc(c.NXH * sizeof(double));
     c.kz = (const double *)dmalloc(c.NZ * sizeof(double));
     c.means = (double2 *)dmalloc(12 * N * sizeof(double2));
@@ -135,6 +136,14 @@ void Channel::upload_operators() {
     OWT_HIP(hipMemcpyAsync((void *)c_.DY1, op_.DY1.data(), N * N * sizeof(double), hipMemcpyHostToDevice, s_));
     OWT_HIP(hipMemcpyAsync((void *)c_.DY2, op_.DY2.data(), N * N * sizeof(double), hipMemcpyHostToDevice, s_));
     OWT_HIP(hipMemcpyAsync((void *)c_.DY4, op_.DY4.data(), N * N * sizeof(double), hipMemcpyHostToDevice, s_));
+    // trapezoid weights of the y grid (pressure gauge): (y[j+1] - y[j-1]) / 2,
+    // one-sided at the walls
+    std::vector<double> qw(N);
+    for (size_t j = 0; j < N; ++j) {
+        const double lo = op_.y[j > 0 ? j - 1 : 0], hi = op_.y[j + 1 < N ? j + 1 : N - 1];
+        qw[j] = (hi - lo) / 2.0;
+    }
+    OWT_HIP(hipMemcpyAsync((void *)c_.qw, qw.data(), N * sizeof(double), hipMemcpyHostToDevice, s_));
     // wavenumbers (src/solver.cpp:22-37) with PX = PZ = 1
     std::vector<double> kx(c_.NXH), kz(c_.NZ);
     for (int i = 0; i < c_.NXH; ++i) kx[i] = (double)i * P_.dkx;
@@ -194,11 +203,35 @@ void Channel::factor_operators() {
         OWT_HIP(hipMemcpy(jobs_, jobs.data(), njobs_ * sizeof(int4), hipMemcpyHostToDevice));
         OWT_HIP(hipMemcpy(frep_, frep.data(), nfac_ * sizeof(int), hipMemcpyHostToDevice));
     }
-    launch_factor(c_, Fb_, Pb_, nfac_, frep_, s_);
+    launch_factor(c_, Fb_, Pb_, nfac_, frep_, 0, 2, s_);
     OWT_HIP(hipGetLastError());
     OWT_HIP(hipStreamSynchronize(s_));
     check_errors();
     factored_ = true;
+}
+
+// Dynamic pressure of the current level into out (W complex). The pressure
+// operator depends on DY1, DY2 and k^2 only: its factors (one per distinct k^2,
+// the batched tables' grouping) are computed on the first request and kept.
+// The step's transform buffer zbuf is free between steps and holds the
+// right-hand sides and the solution.
+void Channel::pressure(double *out) {
+    factor_operators();
+    const size_t N = c_.NY, W = c_.W;
+    if (!Fq_) {
+        Fq_ = (double *)dmalloc((size_t)nfac_ * N * N * sizeof(double));
+        Pq_ = (int *)dmalloc((size_t)nfac_ * N * sizeof(int));
+        launch_factor(c_, Fq_, Pq_, nfac_, frep_, 2, 1, s_);
+        OWT_HIP(hipGetLastError());
+        OWT_HIP(hipStreamSynchronize(s_));
+        check_errors();
+    }
+    double2 *R = (double2 *)c_.zbuf, *X = R + W;
+    launch_pressure_rhs(c_, c_.d2v + (size_t)ring_ * W, R, s_);
+    launch_pressure_solve(c_, Fq_, Pq_, jobs_, njobs_, R, X, s_);
+    OWT_HIP(hipGetLastError());
+    OWT_HIP(hipMemcpyAsync(out, X, W * sizeof(double2), hipMemcpyDeviceToHost, s_));
+    OWT_HIP(hipStreamSynchronize(s_));
 }
 
 void Channel::get_factor(int sys, int op, double *lu, int *perm) {
@@ -517,7 +550,7 @@ void Channel::enqueue_step_body(int ring) {
     const StepPtrs sp = step_ptrs(ring);
     const double *F = plain ? Fp_ : Fb_;
     const int *P = plain ? Pp_ : Pb_;
-    if (plain) launch_factor(c_, Fp_, Pp_, c_.npencil, nullptr, s_);
+    if (plain) launch_factor(c_, Fp_, Pp_, c_.npencil, nullptr, 0, 2, s_);
     launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, s_);
     launch_horizontal(c_, sp, s_);
     launch_mean(c_, sp, F, P, plain ? 0 : fmap_host_[0], s_);
@@ -576,8 +609,8 @@ void Channel::enqueue_kernel(int id) {
         case 5: launch_z_forward(c_, s_); break;
         case 6: launch_nonlinear(c_, sp, s_); break;
         case 7:
-            if (plain) launch_factor(c_, Fp_, Pp_, c_.npencil, nullptr, s_);
-            else launch_factor(c_, Fb_, Pb_, nfac_, frep_, s_);
+            if (plain) launch_factor(c_, Fp_, Pp_, c_.npencil, nullptr, 0, 2, s_);
+            else launch_factor(c_, Fb_, Pb_, nfac_, frep_, 0, 2, s_);
             break;
         default: throw Error(2, "kernel id must be 0..7");
     }
@@ -614,6 +647,10 @@ long Channel::steps_done() {
 void Channel::get_field(int f, double *out) {
     if (f < 0 || f >= F_COUNT) throw Error(2, "unknown field id " + std::to_string(f));
     sync();
+    if (f == F_P) {
+        pressure(out);
+        return;
+    }
     const int r = ring_;
     const size_t W = c_.W;
     const double2 *src = nullptr;

@@ -11,7 +11,7 @@
 namespace owt {
 
 // field ids of the C ABI (owt.h)
-enum FieldId { F_U = 0, F_V, F_W, F_OMX, F_OMY, F_OMZ, F_LMBX, F_LMBY, F_LMBZ, F_HV, F_HG, F_COUNT };
+enum FieldId { F_U = 0, F_V, F_W, F_OMX, F_OMY, F_OMZ, F_LMBX, F_LMBY, F_LMBZ, F_HV, F_HG, F_P, F_COUNT };
 
 class Channel {
   public:
@@ -61,6 +61,7 @@ class Channel {
     void upload_operators();
     void factor_operators();
     void ensure_plain_factors();
+    void pressure(double *out);
     void init_post();
     StepPtrs step_ptrs(int ring) const;
     void enqueue_step_body(int ring);
@@ -81,6 +82,9 @@ class Channel {
     // plain, one per pencil, recomputed every step
     double *Fb_ = nullptr, *Fp_ = nullptr;
     int *Pb_ = nullptr, *Pp_ = nullptr;
+    // pressure operator factors (batched grouping), computed on first request
+    double *Fq_ = nullptr;
+    int *Pq_ = nullptr;
     bool factored_ = false;
     // batched factor tables: one factor per distinct k^2; jobs ({factor, system,
     // system or -1}: systems sharing a factor, two at a time), frep (factor -> a

@@ -53,6 +53,7 @@ struct Chan {
     double *fric;    // fx[3], fz[3]
     double2 *msol;   // 2 x NY mean-flow solutions
     double2 *d2v;    // 3 W: DY2 v at the 3 history levels (slots as in hist)
+    const double *qw;  // NY trapezoid weights of y (pressure gauge)
     int *ctrl;
     double *diag;    // diag_cap x 3: fx_fric, bottom, upper per friction evaluation
     int diag_cap;
@@ -86,9 +87,16 @@ void launch_z_forward_to(const Chan &c, const void *in, int nf, double2 *out, hi
 void launch_peak3(const double *phys, size_t n, unsigned long long *out, hipStream_t s);
 
 // ---- wall-normal phase launches (lu_kernels.hip, y_kernels.hip) ----
-// LU factors of Av and Ag for nfac factor indices (see lu_kernels.hip);
+// LU factors for nfac factor indices (see lu_kernels.hip): operators op0 ..
+// op0 + nops - 1 (0 Av, 1 Ag, 2 pressure), stored nops per factor index;
 // frep[f]: representative system or pencil of factor f (null: f itself)
-void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, hipStream_t s);
+void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, int op0, int nops, hipStream_t s);
+// dynamic pressure of the current level: right-hand sides into R (W) from the
+// Lamb vector (in the transform precision) and dv = DY2 v of the level; then the
+// solves with the pressure factors (op 2 of the batched factor tables) into out
+void launch_pressure_rhs(const Chan &c, const double2 *dv, double2 *R, hipStream_t s);
+void launch_pressure_solve(const Chan &c, const double *F, const int *P, const int4 *jobs, int njobs, const double2 *R,
+                           double2 *out, hipStream_t s);
 // right-hand sides + solves; plain: njobs = pencils, each with its own factor;
 // batched: job j = {factor, system, system or -1}
 void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int4 *jobs, int njobs, bool plain,

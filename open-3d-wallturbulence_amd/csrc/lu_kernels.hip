@@ -42,7 +42,7 @@ namespace owt {
 // is its row index in the stored (interchanged) factor.
 // ---------------------------------------------------------------------------
 template <int RPT, int CPT, int TC>
-__global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, const int *frep) {
+__global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, const int *frep, int op0, int nops) {
     constexpr int NR = 64 * RPT;
     __shared__ double lbuf[2][NR];
     __shared__ int pos[NR], rowAt[NR];
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, c
 
     const int N = c.NY;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int f = blockIdx.x, op = blockIdx.y;
+    const int f = blockIdx.x, op = op0 + blockIdx.y;
     const int rep = frep ? frep[f] : f;
     const double k2 = k2_of(c, rep / c.NXH, rep % c.NXH);
     const double nu = c.nu, dt = c.dt;
@@ -71,7 +71,15 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, c
             double v = 0.0;
             if (r < N && col < N) {
                 const size_t e = (size_t)col * N + r;
-                if (r == 0 || r == N - 1) {
+                if (op == 2) {
+                    // pressure: Neumann wall rows; the k = 0 pencil's top row is the gauge
+                    if (r == 0) v = c.DY1[(size_t)col * N];
+                    else if (r == N - 1) v = k2 < 1.e-8 ? c.qw[col] : c.DY1[(size_t)col * N + N - 1];
+                    else {
+                        v = c.DY2[e];
+                        if (r == col) v = sub(v, k2);
+                    }
+                } else if (r == 0 || r == N - 1) {
                     v = (r == col) ? 1.0 : 0.0;
                 } else if (op == 0) {
                     if (r == 1) v = c.DY1[(size_t)col * N];
@@ -97,8 +105,8 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, c
 #pragma unroll
     for (int a = 0; a < RPT; ++a)
         if (lane + 64 * a < N) active |= 1u << a;
-    double *Fo = F + ((size_t)f * 2 + op) * N * N;
-    int *Po = P + ((size_t)f * 2 + op) * N;
+    double *Fo = F + ((size_t)f * nops + blockIdx.y) * N * N;
+    int *Po = P + ((size_t)f * nops + blockIdx.y) * N;
     __syncthreads();
 
 #pragma unroll
@@ -207,6 +215,19 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, c
     }
 }
 
+// The pencils of batched job jb = {factor, system, system or -1}: each system
+// s = mz*NXH + i is the +kz pencil s and, for 0 < mz < NZH, the -kz pencil
+// (NZ - mz)*NXH + i; missing ones are -1.
+__device__ __forceinline__ void job_pencils(const Chan &c, int4 jb, int (&pc)[4]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int sys = h == 0 ? jb.y : jb.z;
+        const int mz = sys >= 0 ? sys / c.NXH : 0, i = sys >= 0 ? sys % c.NXH : 0;
+        pc[2 * h] = sys >= 0 ? sys : -1;
+        pc[2 * h + 1] = (sys >= 0 && mz > 0 && mz < c.NZH) ? (c.NZ - mz) * c.NXH + i : -1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Right-hand sides (src/solver.cpp:568-574, 593-599) and dgetrs.
 // Four waves per block: wave w takes job 2*block + w/2 and operator w%2.
@@ -234,13 +255,10 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
     } else {
         const int4 jb = jobs[job];
         fac = jb.x;
+        int pj[4];
+        job_pencils(c, jb, pj);
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int sys = h == 0 ? jb.y : jb.z;
-            const int mz = sys >= 0 ? sys / c.NXH : 0, i = sys >= 0 ? sys % c.NXH : 0;
-            pc[2 * h] = sys >= 0 ? sys : -1;
-            pc[2 * h + 1] = (sys >= 0 && mz > 0 && mz < c.NZH) ? (c.NZ - mz) * c.NXH + i : -1;
-        }
+        for (int pp = 0; pp < NP; ++pp) pc[pp] = pj[pp];
         kz = jb.y / c.NXH;
         ix = jb.y % c.NXH;
     }
@@ -294,6 +312,50 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
 }
 
 // ---------------------------------------------------------------------------
+// Dynamic pressure (no reference counterpart): the right-hand sides R of the
+// current level (y_kernels.hip) solved with the pressure operator
+//   (DY2 - k^2) P = R on rows 1..N-2,  DY1 P = R on rows 0 and N-1,
+// the k = 0 pencil's row N-1 being the zero-mean gauge sum_j q_j P_j = 0.
+// Factors as the batched set (one per distinct k^2, op 2, one operator per
+// factor index); one wave per job, up to four pencils per pass over a factor.
+// ---------------------------------------------------------------------------
+template <int NQ>
+__global__ __launch_bounds__(256) void k_psolve(Chan c, const double *F, const int *P, const int4 *jobs, int njobs,
+                                                const double2 *R, double2 *out) {
+    const int lane = threadIdx.x & 63, job = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int N = c.NY;
+    if (job >= njobs) return;
+    const int4 jb = jobs[job];
+    int pc[4];
+    job_pencils(c, jb, pc);
+    const double *Fo = F + (size_t)jb.x * N * N;
+    const int *Po = P + (size_t)jb.x * N;
+    double t[8][NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int pos = lane + 64 * q;
+        int r = pos < N ? Po[pos] : 0;
+        if ((unsigned)r >= (unsigned)N) r = 0;
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) {
+            double2 v = R[(size_t)(pc[pp] >= 0 ? pc[pp] : pc[0]) * N + r];
+            if (pos >= N || pc[pp] < 0) v = c2(0, 0);
+            t[2 * pp][q] = v.x;
+            t[2 * pp + 1][q] = v.y;
+        }
+    }
+    lu_substitute<NQ, 8>(Fo, N, lane, t);
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int row = lane + 64 * q;
+        if (row >= N) continue;
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp)
+            if (pc[pp] >= 0) out[(size_t)pc[pp] * N + row] = c2(t[2 * pp][q], t[2 * pp + 1][q]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
 #define OWT_FACTOR_DISPATCH(N, BODY)                                                    \
@@ -306,8 +368,9 @@ __global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double
         else { constexpr int RPT = 4, CPT = 16, TC = 16; BODY; }                        \
     } while (0)
 
-void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, hipStream_t s) {
-    OWT_FACTOR_DISPATCH(c.NY, hipLaunchKernelGGL((k_factor<RPT, CPT, TC>), dim3(nfac, 2), dim3(64 * TC), 0, s, c, F, P, frep));
+void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, int op0, int nops, hipStream_t s) {
+    OWT_FACTOR_DISPATCH(c.NY, hipLaunchKernelGGL((k_factor<RPT, CPT, TC>), dim3(nfac, nops), dim3(64 * TC), 0, s, c, F, P, frep,
+                                                 op0, nops));
 }
 
 void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int4 *jobs, int njobs, bool plain,
@@ -317,6 +380,12 @@ void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int 
         OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, true>), dim3(blocks), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
     else
         OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false>), dim3(blocks), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
+}
+
+
+void launch_pressure_solve(const Chan &c, const double *F, const int *P, const int4 *jobs, int njobs, const double2 *R,
+                           double2 *out, hipStream_t s) {
+    OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_psolve<NQ>, dim3((njobs + 3) / 4), dim3(256), 0, s, c, F, P, jobs, njobs, R, out));
 }
 
 }  // namespace owt

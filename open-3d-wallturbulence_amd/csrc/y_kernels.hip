@@ -5,6 +5,7 @@
 //   mean        mean flow, friction force and mean vorticity on the single
 //               k = 0 pencil (src/solver.cpp:306-327, 428-514)
 //   nonlinear   Hv, Hg from the Lamb vector (src/solver.cpp:240-260)
+//   pressure    right-hand sides of the dynamic pressure (no reference counterpart)
 //   init_*      the post-processing of the initial field (src/solver.cpp:379-425)
 //
 // Pencil p = k*NXH + i holds NY contiguous complex values (y fastest). Mat-vecs
@@ -280,6 +281,65 @@ __global__ __launch_bounds__(256) void k_nonlinear(Chan c, StepPtrs sp) {
 }
 
 // ---------------------------------------------------------------------------
+// dynamic pressure P = p + |u|^2/2 of the current level (the reference has no
+// pressure). The divergence of the rotational momentum equation
+// du/dt = L - grad P + nu lap u (L the Lamb vector, u x omega) gives
+//   rows 1..N-2:  (DY2 - k^2) P = i kx Lx + DY1 Ly + i kz Lz
+// and its y component at the walls, where v = 0:
+//   rows 0, N-1:  DY1 P = Ly + nu DY2 v
+// The k = 0 pencil's row N-1 is its gauge row (right-hand side 0). This kernel
+// writes the right-hand sides; lu_kernels.hip solves.
+// ---------------------------------------------------------------------------
+template <int TP, typename LC>
+__global__ __launch_bounds__(256) void k_pressure_rhs(Chan c, const double2 *dv, double2 *R) {
+    __shared__ double2 st[TP][kMaxNY];
+    const int N = c.NY;
+    const LC *lx = reinterpret_cast<const LC *>(c.lmb);
+    const LC *ly = lx + c.W, *lz = lx + 2 * c.W;
+    const int p0 = blockIdx.x * TP;
+    for (int e = threadIdx.x; e < TP * N; e += blockDim.x) {
+        const int pp = e / N, j = e % N, p = p0 + pp;
+        st[pp][j] = p < c.npencil ? ld2(ly + (size_t)p * N + j) : c2(0, 0);
+    }
+    __syncthreads();
+    const int row = threadIdx.x;
+    if (row >= N) return;
+    double2 acc[TP];
+#pragma unroll
+    for (int pp = 0; pp < TP; ++pp) acc[pp] = c2(0, 0);
+    for (int j0 = 0; j0 < N; j0 += JC) {
+        double dj[JC];
+#pragma unroll
+        for (int cc = 0; cc < JC; ++cc) dj[cc] = c.DY1[(size_t)(j0 + cc < N ? j0 + cc : N - 1) * N + row];
+#pragma unroll
+        for (int cc = 0; cc < JC; ++cc) {
+            const int j = j0 + cc;
+            if (j >= N) break;
+#pragma unroll
+            for (int pp = 0; pp < TP; ++pp) {
+                acc[pp].x = fma(dj[cc], st[pp][j].x, acc[pp].x);
+                acc[pp].y = fma(dj[cc], st[pp][j].y, acc[pp].y);
+            }
+        }
+    }
+#pragma unroll
+    for (int pp = 0; pp < TP; ++pp) {
+        const int p = p0 + pp;
+        if (p >= c.npencil) break;
+        const int k = p / c.NXH, i = p % c.NXH;
+        const size_t o = (size_t)p * N + row;
+        double2 r;
+        if (row == 0 || row == N - 1) {
+            r = caddc(st[pp][row], cscale(dv[o], c.nu));
+            if (row == N - 1 && k2_of(c, k, i) < 1.e-8) r = c2(0, 0);
+        } else {
+            r = caddc(caddc(imul(c.kx[i], ld2(lx + o)), imul(c.kz[k], ld2(lz + o))), acc[pp]);
+        }
+        R[o] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // initialisation (src/solver.cpp:218-238, 379-425); ring = 0, current level in slot 0
 // ---------------------------------------------------------------------------
 template <int TP>
@@ -416,6 +476,13 @@ void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s) {
         hipLaunchKernelGGL((k_nonlinear<kTP, double2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, sp);
     else
         hipLaunchKernelGGL((k_nonlinear<kTP, float2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, sp);
+}
+
+void launch_pressure_rhs(const Chan &c, const double2 *dv, double2 *R, hipStream_t s) {
+    if (c.precision == 0)
+        hipLaunchKernelGGL((k_pressure_rhs<kTP, double2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, dv, R);
+    else
+        hipLaunchKernelGGL((k_pressure_rhs<kTP, float2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, dv, R);
 }
 
 void launch_vor_from_vel(const Chan &c, hipStream_t s) {

@@ -21,7 +21,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libowt.so")
 
 FIELDS = {"u": 0, "v": 1, "w": 2, "omx": 3, "omy": 4, "omz": 5,
-          "lmbx": 6, "lmby": 7, "lmbz": 8, "Hv": 9, "Hg": 10}
+          "lmbx": 6, "lmby": 7, "lmbz": 8, "Hv": 9, "Hg": 10, "p": 11}
 PLAIN, BATCHED = 0, 1
 FP64, MIXED = 0, 1
 KERNELS = {"solve": 0, "horizontal": 1, "mean": 2, "z_inverse": 3, "x_lamb": 4, "z_forward": 5, "nonlinear": 6,

@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
 
 FIELDS = {"u": 0, "v": 1, "w": 2, "omx": 3, "omy": 4, "omz": 5,
-          "lmbx": 6, "lmby": 7, "lmbz": 8, "Hv": 9, "Hg": 10}
+          "lmbx": 6, "lmby": 7, "lmbz": 8, "Hv": 9, "Hg": 10, "p": 11}
 MATRICES = {"DY1": 0, "DY2": 1, "DY3": 2, "DY4": 3, "A1": 4, "A2": 5, "A3": 6, "A4": 7,
             "B1": 8, "B2": 9, "B3": 10, "B4": 11, "D1": 12, "D2": 13, "D3": 14, "D4": 15}
 VECTORS = {"y": 0, "s": 1, "dyds1": 2, "dyds2": 3, "dyds3": 4, "dyds4": 5, "c11": 6, "c21": 7,
@@ -47,6 +47,7 @@ def lib():
         L.orc_f2p.argtypes = [P, P, P]
         L.orc_p2f.argtypes = [P, P, P]
         L.orc_lamb.argtypes = [P, P, P]
+        L.orc_pressure_of.argtypes = [P, P, P, P, P, P]
         L.orc_peak_velocity.argtypes = [P, P]
         L.orc_fft.argtypes = [I, I, P, P]
         L.orc_lu_solve.argtypes = [I, P, I, P]
@@ -111,6 +112,13 @@ class Oracle:
     def field(self, name):
         out = np.empty(self.wave_shape, dtype=np.complex128)
         self._check(lib().orc_get_field(self._h, FIELDS[name], _p(out)))
+        return out
+
+    def pressure_of(self, lx, ly, lz, v):
+        """Dynamic pressure from a given Lamb vector and v (field "p" uses the state's)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128).reshape(self.wave_shape) for a in (lx, ly, lz, v)]
+        out = np.empty(self.wave_shape, dtype=np.complex128)
+        self._check(lib().orc_pressure_of(self._h, *[_p(a) for a in arrs], _p(out)))
         return out
 
     def matrix(self, name):

@@ -14,6 +14,10 @@
 //   * mat-vec in column-axpy order                         src/linalg.hpp:434-453
 //   * 3/2-dealiased transforms                             src/FFT.cpp:49-194
 //   * time step, mean flow, friction, Lamb vector          src/solver.cpp:240-627
+//   * pressure: not in the reference (no p variable, src/setup.hpp:58-79); the
+//     dynamic pressure defined below is this build's, restated identically here
+//     and on the GPU. Parity unpinned beyond that identity and the laminar
+//     closed form.
 // FFTW is replaced by a plain recursive mixed-radix DFT (same sign and scaling
 // conventions: backward unnormalised e^{+i}, forward e^{-i}).
 //
@@ -640,6 +644,61 @@ struct Oracle {
                 if (std::pow(kx[i], 2) + std::pow(kz[k], 2) < 1.e-8) solve_mean();
     }
 
+    // Dynamic pressure Pi = p + |u|^2/2 of the current state (no reference
+    // counterpart). Divergence of the rotational momentum equation:
+    //   (DY2 - k^2) Pi = i kx Lx + DY1 Ly + i kz Lz        (rows 1..N-2)
+    // with y-momentum at the walls, where v = 0:
+    //   DY1 Pi = Ly + nu DY2 v                                (rows 0 and N-1)
+    // and, for the k = 0 pencil, row N-1 replaced by the gauge
+    //   sum_j q_j Pi_j = 0, q the trapezoid weights of y.
+    // The mean streamwise gradient is -fx_fric (not part of Pi).
+    std::vector<cd> pressure() { return pressure_of(lmbx.data(), lmby.data(), lmbz.data(), vely.data()); }
+
+    // the same from given Lamb vector components and v ([NZ][NXH][NY] each)
+    std::vector<cd> pressure_of(const cd *Lx, const cd *Ly, const cd *Lz, const cd *V) {
+        std::vector<cd> out(W, cd(0.0, 0.0));
+        std::vector<double> q(NY, 0.0);
+        for (int j = 0; j < NY; ++j) {
+            const double lo = j > 0 ? y[j - 1] : y[0], hi = j < NY - 1 ? y[j + 1] : y[NY - 1];
+            q[j] = (hi - lo) / 2.0;
+        }
+        int errflag = 0;
+#pragma omp parallel
+        {
+            std::vector<cd> r(NY), d(NY), rhs(NY);
+            Mat Ap(NY);
+#pragma omp for collapse(2) schedule(dynamic, 4)
+            for (int k = 0; k < NZ; ++k)
+                for (int i = 0; i < NXH; ++i) {
+                    const double kxx = kx[i], kzz = kz[k];
+                    const double k2 = kxx * kxx + kzz * kzz;
+                    const bool mean = k2 < 1.e-8;
+                    const size_t o = at(k, i);
+                    const cd ikx = cd(0.0, kxx), ikz = cd(0.0, kzz);
+                    matvec(DY1, &Ly[o], r.data());
+                    matvec(DY2, &V[o], d.data());
+                    for (int j = 0; j < NY; ++j) rhs[j] = (ikx * Lx[o + j] + ikz * Lz[o + j]) + r[j];
+                    rhs[0] = Ly[o] + nu * d[0];
+                    rhs[NY - 1] = mean ? cd(0.0, 0.0) : Ly[o + NY - 1] + nu * d[NY - 1];
+                    for (size_t e = 0; e < Ap.a.size(); ++e) Ap.a[e] = DY2.a[e];
+                    for (int j = 0; j < NY; ++j) Ap(j, j) = Ap(j, j) - k2;
+                    for (int j = 0; j < NY; ++j) {
+                        Ap(0, j) = DY1(0, j);
+                        Ap(NY - 1, j) = mean ? q[j] : DY1(NY - 1, j);
+                    }
+                    try {
+                        dense_solve_complex(Ap, rhs.data());
+                    } catch (...) {
+#pragma omp atomic write
+                        errflag = 1;
+                    }
+                    for (int j = 0; j < NY; ++j) out[o + j] = rhs[j];
+                }
+        }
+        if (errflag) throw std::runtime_error("pressure operator singular");
+        return out;
+    }
+
     // src/solver.cpp:541-627
     void time_integration() {
         const double g0 = 11.0 / 6.0, a0 = 3.0, a1 = -1.5, a2 = 1.0 / 3.0, b0 = 3.0, b1 = -3.0, b2 = 1.0;
@@ -895,9 +954,33 @@ int orc_step(void *h, int n) {
     }
 }
 
-// field ids: 0 u, 1 v, 2 w, 3 omx, 4 omy, 5 omz, 6 lmbx, 7 lmby, 8 lmbz, 9 Hv, 10 Hg
+// dynamic pressure (field 11) from given Lamb vector components and v
+int orc_pressure_of(void *h, const double *lx, const double *ly, const double *lz, const double *v, double *out) {
+    Oracle *o = (Oracle *)h;
+    try {
+        std::vector<cd> p = o->pressure_of((const cd *)lx, (const cd *)ly, (const cd *)lz, (const cd *)v);
+        std::memcpy(out, p.data(), o->W * sizeof(cd));
+        return 0;
+    } catch (std::exception &e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+// field ids: 0 u, 1 v, 2 w, 3 omx, 4 omy, 5 omz, 6 lmbx, 7 lmby, 8 lmbz, 9 Hv, 10 Hg,
+// 11 p (the dynamic pressure of Oracle::pressure, computed on request)
 int orc_get_field(void *h, int f, double *out) {
     Oracle *o = (Oracle *)h;
+    if (f == 11) {
+        try {
+            std::vector<cd> p = o->pressure();
+            std::memcpy(out, p.data(), o->W * sizeof(cd));
+            return 0;
+        } catch (std::exception &e) {
+            g_err = e.what();
+            return -1;
+        }
+    }
     const std::vector<cd> *src[] = {&o->velx, &o->vely, &o->velz, &o->vorx, &o->vory, &o->vorz,
                                     &o->lmbx, &o->lmby, &o->lmbz, &o->Hv, &o->Hg};
     if (f < 0 || f > 10) return -1;

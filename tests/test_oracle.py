@@ -14,6 +14,9 @@ So each part of the oracle is checked here against an independent fact:
   metric, DY_n    derivatives of tanh(a s)/tanh(a) and of smooth functions
   step            laminar Poiseuille balance fx_fric = 3 nu, Hermitian symmetry
                   of the kx = 0 plane, discrete divergence, seeded amplitude
+  pressure        (no reference counterpart) laminar P = u^2/2 - mean; a gradient
+                  Lamb vector L = grad phi gives P = phi; wall data nu v'' alone
+                  gives A cosh(ky) + B sinh(ky); each converges with NY
 """
 from math import factorial
 
@@ -316,6 +319,77 @@ def test_seeded_init_properties(oracle_mod):
     for m in range(1, o.NZH):
         for f in (u, v, w):
             assert np.abs(f[m, 0] - np.conj(f[o.NZ - m, 0])).max() < 1e-15
+
+
+def trapezoid_weights(y):
+    q = np.empty_like(y)
+    q[1:-1] = (y[2:] - y[:-2]) / 2
+    q[0], q[-1] = (y[1] - y[0]) / 2, (y[-1] - y[-2]) / 2
+    return q
+
+
+def test_pressure_laminar_closed_form(oracle_mod):
+    """Poiseuille flow: L = u x omega = (0, -u omega_z, 0) with omega_z = -u', so
+    dP/dy = Ly gives P = u^2/2 up to the gauge constant (zero trapezoid mean);
+    every other pencil is zero. The discrete DY1 and DY2 are exact to their
+    truncation error only on the stretched grid, so the error falls with NY."""
+    errs = []
+    for NY in (33, 65):
+        o = make(oracle_mod, 16, NY, 16)
+        u, v, w = laminar_fields(o)
+        o.set_velocity(u, v, w)
+        P = o.field("p")
+        y = o.vector("y")
+        ex = 0.5 * u[0, 0].real ** 2
+        q = trapezoid_weights(y)
+        ex -= (q * ex).sum() / q.sum()
+        errs.append(np.abs(P[0, 0] - ex).max() / np.abs(ex).max())
+        assert np.abs(P[1:]).max() < 1e-16 and np.abs(P[0, 1:]).max() < 1e-16
+    assert errs[0] < 5e-4 and errs[1] < 5e-5 and errs[1] < errs[0] / 6, errs
+
+
+def test_pressure_closed_forms(oracle_mod):
+    """Two manufactured cases of Oracle.pressure_of, on every pencil:
+      L = (i kx phi, phi', i kz phi), v = 0   ->  P = phi (k = 0: phi minus its mean)
+      L = 0, v given                          ->  (D^2 - k^2) P = 0 with P' = nu v''
+                                                  at each wall: A cosh ky + B sinh ky
+                                                  (k = 0: slope nu v''(-1), gauge mean 0)
+    phi differs from pencil to pencil (a swapped kx/kz fails), and v'' differs at
+    the two walls (swapped or dropped wall rows fail)."""
+    e1s, e2s = [], []
+    for NY in (33, 65, 129):
+        o = make(oracle_mod, 8, NY, 8)
+        y, kx, kz = o.vector("y"), o.vector("kx"), o.vector("kz")
+        K = np.sqrt(kz[:, None] ** 2 + kx[None, :] ** 2)
+        c = (1 + 0.5j) * (1 + np.arange(o.NZ))[:, None] * (1 + 0.3 * np.arange(o.NXH))[None, :]
+        f = np.exp(0.7 * y) * np.cos(1.9 * y + 0.4)
+        fp = 0.7 * f - 1.9 * np.exp(0.7 * y) * np.sin(1.9 * y + 0.4)
+        phi = c[..., None] * f
+        z = np.zeros_like(phi)
+        P = o.pressure_of(1j * kx[None, :, None] * phi, c[..., None] * fp, 1j * kz[:, None, None] * phi, z)
+        q = trapezoid_weights(y)
+        ex = phi.copy()
+        ex[0, 0] -= (q * phi[0, 0]).sum() / q.sum()
+        e1s.append(np.abs(P - ex).max() / np.abs(ex).max())
+
+        g, gp, gpp = (1 - y ** 2) ** 2, -4 * y * (1 - y ** 2), -4 + 12 * y ** 2
+        s, cs = np.sin(y + 0.3), np.cos(y + 0.3)
+        vpp = gpp * s + 2 * gp * cs - g * s
+        P2 = o.pressure_of(z, z, z, c[..., None] * (g * s))
+        g0, g1 = o.nu * vpp[0], o.nu * vpp[-1]
+        ex2 = np.empty_like(P2)
+        for k in range(o.NZ):
+            for i in range(o.NXH):
+                kk = K[k, i]
+                if kk == 0:
+                    ex2[k, i] = c[k, i] * g0 * y
+                else:
+                    A, B = (g1 - g0) / (2 * kk * np.sinh(kk)), (g1 + g0) / (2 * kk * np.cosh(kk))
+                    ex2[k, i] = c[k, i] * (A * np.cosh(kk * y) + B * np.sinh(kk * y))
+        e2s.append(np.abs(P2 - ex2).max() / np.abs(ex2).max())
+    # fourth order or better in the wall spacing
+    assert e1s[0] < 1e-3 and e1s[2] < 3e-6 and e1s[1] < e1s[0] / 10 and e1s[2] < e1s[1] / 10, e1s
+    assert e2s[0] < 2e-2 and e2s[2] < 6e-5 and e2s[1] < e2s[0] / 10 and e2s[2] < e2s[1] / 10, e2s
 
 
 def test_step_preserves_hermitian_plane_and_divergence(oracle_mod):
