@@ -7,7 +7,7 @@
  *   load_params + allocate_var + init_mat + init_fft   -> owt_create
  *   init_fields                                         -> owt_init_fields / owt_set_velocity
  *   time_Integration (x NT+1) and its stdout diagnostics -> owt_step, owt_wall_shear, owt_diag
- *   save_instant_field                                  -> owt_get_field (+ owt_checkpoint)
+ *   save_instant_field                                  -> owt_save (+ owt_checkpoint for restart)
  *   cleanup_fft + process exit                          -> owt_destroy
  *
  * Fields are exchanged in the reference's wave-space layout [NZ][NXH][NY]
@@ -100,6 +100,14 @@ int owt_get_field(owt_handle *h, int field, double *out);
 
 int owt_set_method(owt_handle *h, int method);
 int owt_set_precision(owt_handle *h, int precision);
+
+/* save_instant_field (src/IO.cpp:198-283): writes <dir>/F-<id_step, 8 digits>.H5
+ * with the datasets velx_real, velx_imag, vely_real, vely_imag, velz_real,
+ * velz_imag, each [NZ][NXH][NY] double (WRITE_3D, src/IO.cpp:118-157); the
+ * reference writes to ./DATA. The serial HDF5 C library (1.10 or later) is
+ * loaded at run time from $OWT_HDF5_LIB or libhdf5.so on the loader path;
+ * OWT_E_IO when it cannot be loaded or the file cannot be written. */
+int owt_save(owt_handle *h, const char *dir);
 
 /* full state: all history levels, mean-flow and friction history, step counter */
 int owt_checkpoint(owt_handle *h, const char *path);

@@ -162,6 +162,11 @@ int owt_set_method(owt_handle *h, int method) { return guard(h, [&] { h->ch->set
 
 int owt_set_precision(owt_handle *h, int precision) { return guard(h, [&] { h->ch->set_precision(precision); }); }
 
+int owt_save(owt_handle *h, const char *dir) {
+    if (!dir) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->save(dir); });
+}
+
 int owt_checkpoint(owt_handle *h, const char *path) {
     if (!path) return OWT_E_ARG;
     return guard(h, [&] { h->ch->checkpoint(path); });

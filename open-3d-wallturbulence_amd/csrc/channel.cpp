@@ -8,6 +8,8 @@
 #include <fstream>
 #include <map>
 
+#include "snapshot.hpp"
+
 namespace owt {
 
 FFTPlan make_plan(int L) {
@@ -778,6 +780,31 @@ struct CkptHeader {
     double nu, dt, dkx, dkz, alpha;
 };
 }  // namespace
+
+// save_instant_field (src/IO.cpp:198-283): <dir>/F-<id_step, 8 digits>.H5 with
+// the real and imaginary parts of u, v, w as [NZ][NXH][NY] doubles
+void Channel::save(const std::string &dir) {
+    const long id = steps_done();
+    char name[40];
+    std::snprintf(name, sizeof name, "F-%08ld.H5", id);
+    const std::string path = dir.empty() ? std::string(name) : dir + "/" + name;
+    const size_t W = c_.W;
+    const long dims[3] = {c_.NZ, c_.NXH, c_.NY};
+    std::vector<double> z(2 * W), re(W), im(W);
+    H5Writer f(path);
+    const struct { int id; const char *re, *im; } sets[] = {
+        {F_U, "velx_real", "velx_imag"}, {F_V, "vely_real", "vely_imag"}, {F_W, "velz_real", "velz_imag"}};
+    for (const auto &s : sets) {
+        get_field(s.id, z.data());
+        for (size_t e = 0; e < W; ++e) {
+            re[e] = z[2 * e];
+            im[e] = z[2 * e + 1];
+        }
+        f.write(s.re, dims, re.data());
+        f.write(s.im, dims, im.data());
+    }
+    f.close();
+}
 
 void Channel::checkpoint(const std::string &path) {
     sync();

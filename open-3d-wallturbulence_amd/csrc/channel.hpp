@@ -37,6 +37,8 @@ class Channel {
     long steps_done();
     void set_method(int method);
     void set_precision(int precision);
+    // save_instant_field: HDF5 snapshot <dir>/F-<id_step>.H5 of u, v, w
+    void save(const std::string &dir);
     void checkpoint(const std::string &path);
     void restore(const std::string &path);
 

@@ -8,6 +8,7 @@ The call surface mirrors the reference solver's free functions
     ch.step(n)                        # time_Integration x n
     ch.field("u")                     # the wave-space field, [NZ][NXH][NY] complex
     ch.wall_shear()                   # fx_fric, bottom, upper (src/solver.cpp:314)
+    ch.save("DATA")                   # save_instant_field (src/IO.cpp:198-283), HDF5
 
 Every call goes to the HIP implementation in lib/libowt.so; there is no CPU
 fallback, and a missing library raises.
@@ -33,7 +34,7 @@ EXPORTS = [
     "owt_init_fields", "owt_set_velocity", "owt_step", "owt_wall_shear", "owt_diag",
     "owt_steps_done", "owt_get_field", "owt_set_method", "owt_set_precision",
     "owt_checkpoint", "owt_restore", "owt_f2p", "owt_p2f", "owt_lamb", "owt_get_matrix",
-    "owt_time_steps", "owt_time_kernel", "owt_get_factor",
+    "owt_time_steps", "owt_time_kernel", "owt_get_factor", "owt_save",
 ]
 
 
@@ -84,6 +85,7 @@ def lib():
         L.owt_get_field.argtypes = [P, I, P]
         L.owt_set_method.argtypes = [P, I]
         L.owt_set_precision.argtypes = [P, I]
+        L.owt_save.argtypes = [P, ctypes.c_char_p]
         L.owt_checkpoint.argtypes = [P, ctypes.c_char_p]
         L.owt_restore.argtypes = [P, ctypes.c_char_p]
         L.owt_f2p.argtypes = [P, P, I, P]
@@ -190,6 +192,12 @@ class Channel:
 
     def set_precision(self, precision):
         self._check(lib().owt_set_precision(self._h, int(precision)))
+
+    def save(self, directory="DATA"):
+        """save_instant_field: HDF5 snapshot <directory>/F-<step, 8 digits>.H5 of u, v, w;
+        returns its path."""
+        self._check(lib().owt_save(self._h, os.fsencode(directory)))
+        return os.path.join(directory, "F-%08d.H5" % self.steps_done())
 
     def checkpoint(self, path):
         self._check(lib().owt_checkpoint(self._h, os.fsencode(path)))

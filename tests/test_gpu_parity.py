@@ -245,3 +245,24 @@ def test_step_keeps_hermitian_plane_and_divergence(owt_mod):
             d1v = DY1 @ v[k, i]
             div = 1j * kx[i] * u[k, i] + d1v + 1j * kz[k] * w[k, i]
             assert np.abs(div).max() <= 1e-12 * max(np.abs(d1v).max(), 1e-30)
+
+
+def test_save_snapshot_is_the_reference_format(owt_mod, tmp_path):
+    """save_instant_field (src/IO.cpp:198-283): F-<id_step, 8 digits>.H5 with the real
+    and imaginary parts of u, v, w as [NZ][NXH][NY] doubles, equal to the fields."""
+    import h5read
+
+    if h5read.lib() is None:
+        pytest.skip("HDF5 C library not available to read the snapshot back")
+    p = params(16, 17, 16)
+    ch = gpu(owt_mod, p)
+    ch.init_fields(2, seed=0, amp=0.15)
+    ch.step(3)
+    path = ch.save(str(tmp_path))
+    assert os.path.basename(path) == "F-00000003.H5" and os.path.exists(path)
+    for name, key in (("u", "velx"), ("v", "vely"), ("w", "velz")):
+        f = ch.field(name)
+        re, im = h5read.read(path, key + "_real"), h5read.read(path, key + "_imag")
+        assert re.shape == im.shape == ch.wave_shape
+        assert np.array_equal(re, f.real) and np.array_equal(im, f.imag), name
+    ch.close()
