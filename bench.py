@@ -127,6 +127,52 @@ def pmc_traffic(config):
         shutil.rmtree(out, ignore_errors=True)
 
 
+def accuracy_mixed_vs_fp64(owt, cfg, amp, steps):
+    """The mixed-precision step against the FP64 step (whose parity with the oracle
+    the GPU tests establish) from the same seeded field: relative L2 error per
+    field after `steps` steps, and the mean wall shear (fx_fric, the friction force
+    that holds the flow rate) over the run."""
+    names = ("u", "v", "w", "p")
+    res = {}
+    for prec in (owt.FP64, owt.MIXED):
+        with owt.Channel(params=cfg, precision=prec, method=owt.BATCHED) as ch:
+            ch.init_fields(2, seed=0, amp=amp)
+            ch.step(steps)
+            res[prec] = ({n: ch.field(n) for n in names}, ch.diag()[:, 0])
+    (f64, fx64), (fmx, fxmx) = res[owt.FP64], res[owt.MIXED]
+    rel = {n: float(np.linalg.norm(fmx[n] - f64[n]) / np.linalg.norm(f64[n])) for n in names}
+    return {"steps": steps, "reference": "FP64 batched step on the GPU, same seeded field",
+            "rel_l2": {n: float("%.3e" % v) for n, v in rel.items()},
+            "wall_shear_max_rel": float("%.3e" % (np.abs(fxmx - fx64).max() / np.abs(fx64).max())),
+            "wall_shear_mean_rel": float("%.3e" % (abs(fxmx.mean() - fx64.mean()) / abs(fx64.mean())))}
+
+
+def ensemble_throughput(owt, name, steps):
+    """An ensemble workload stepped together (one graph per step, members as
+    independent branches) against the same members stepped one after another
+    (each channel's own graph step, summed over the members)."""
+    ps = owt.ENSEMBLES[name]
+    with owt.Ensemble(ps, method=owt.BATCHED, precision=owt.FP64) as ens:
+        ens.init_fields(2, seeds=range(len(ps)), amp=0.15)
+        G = ens.gridpoints
+        t = ens.time_steps(steps) / steps
+    seq = 0.0
+    shapes = {}
+    for p in ps:
+        key = (p["NX"], p["NY"], p["NZ"])
+        shapes[key] = shapes.get(key, 0) + 1
+    for (NX, NY, NZ), count in shapes.items():
+        p = next(q for q in ps if (q["NX"], q["NY"], q["NZ"]) == (NX, NY, NZ))
+        with owt.Channel(params=p, method=owt.BATCHED, precision=owt.FP64) as ch:
+            ch.init_fields(2, seed=0, amp=0.15)
+            seq += count * ch.time_steps(steps) / steps
+    return {"workload": name, "members": len(ps),
+            "shapes": {f"{a}x{b}x{c}": n for (a, b, c), n in shapes.items()},
+            "gridpoints": G, "precision": "fp64", "steps": steps, "ms_per_step": round(t, 4),
+            "ns_per_gridpoint_step": round(t * 1e6 / G, 5),
+            "members_one_by_one_ms_per_step": round(seq, 4), "speedup_vs_one_by_one": round(seq / t, 2)}
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A device-to-device copy of nbytes, reps times: the reference point that
     scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -148,6 +194,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs (roofline.traffic null)")
+    ap.add_argument("--ensemble-steps", type=int, default=100,
+                    help="timed steps of the 32-member ensemble workload (0 skips it)")
+    ap.add_argument("--accuracy-steps", type=int, default=100,
+                    help="steps of the mixed-vs-FP64 accuracy comparison (0 skips it)")
     ap.add_argument("--mode", choices=("full", "profile", "copy"), default="full",
                     help="full: the benchmark line; profile: only the mixed batched step, for "
                          "rocprofv3; copy: a plain device copy of known size, to scale counters")
@@ -254,6 +304,12 @@ def main():
         "kernels_ms_fp64": {k: round(v, 4) for k, v in kern64.items()},
         "roofline": roof,
     }
+    if args.ensemble_steps > 0:
+        print("[bench] ensemble", file=sys.stderr, flush=True)
+        out["ensemble"] = ensemble_throughput(owt, "re180_32", args.ensemble_steps)
+    if args.accuracy_steps > 0:
+        print("[bench] accuracy mixed vs fp64", file=sys.stderr, flush=True)
+        out["accuracy_mixed_vs_fp64"] = accuracy_mixed_vs_fp64(owt, cfg, amp, args.accuracy_steps)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(min(16, os.cpu_count() or 1))

@@ -132,6 +132,27 @@ int owt_get_matrix(owt_handle *h, int which, double *out);
  * factors for validation. */
 int owt_get_factor(owt_handle *h, int sys, int op, double *lu, int *perm);
 
+/* Ensembles: n independent channels on one GPU (sizes and parameters may
+ * differ), stepped together. No reference counterpart (the reference runs one
+ * channel per job). With the batched method every member's step is captured
+ * into one graph per step, the members as independent branches. Member m is a
+ * borrowed owt_handle for every single-channel call (fields, diagnostics,
+ * checkpoint, save); owt_destroy on it does nothing, it lives until
+ * owt_ensemble_destroy. A member stepped on its own leaves the ensemble's step
+ * correct (members are then stepped one by one until their phases agree). */
+typedef struct owt_ensemble owt_ensemble;
+int owt_ensemble_create(int n, const owt_params *params, const owt_opts *opts, owt_ensemble **out);
+void owt_ensemble_destroy(owt_ensemble *e);
+/* message of the last failed call on e (or of the last failed create when e is NULL) */
+const char *owt_ensemble_last_error(const owt_ensemble *e);
+owt_handle *owt_ensemble_member(owt_ensemble *e, int m);
+/* init_fields of every member, member m with seeds[m] (seeds NULL: seed m) */
+int owt_ensemble_init_fields(owt_ensemble *e, int mode, const uint64_t *seeds, double amp);
+/* nsteps steps of every member */
+int owt_ensemble_step(owt_ensemble *e, int nsteps);
+/* nsteps ensemble steps between two HIP events; elapsed milliseconds */
+int owt_ensemble_time_steps(owt_ensemble *e, int nsteps, double *ms);
+
 /* timing: enqueue nsteps steps on the handle's stream between two HIP events
  * and return the elapsed milliseconds (inputs stay resident on the device) */
 int owt_time_steps(owt_handle *h, int nsteps, double *ms);

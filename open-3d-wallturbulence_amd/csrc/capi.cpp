@@ -8,9 +8,19 @@
 
 #include "../../include/owt.h"
 #include "channel.hpp"
+#include "ensemble.hpp"
 
+// A handle owns its channel, or borrows one member of an ensemble (own empty).
 struct owt_handle {
-    std::unique_ptr<owt::Channel> ch;
+    std::unique_ptr<owt::Channel> own;
+    owt::Channel *ch = nullptr;
+    std::string err;
+    bool member = false;
+};
+
+struct owt_ensemble {
+    std::unique_ptr<owt::Ensemble> e;
+    std::vector<std::unique_ptr<owt_handle>> members;
     std::string err;
 };
 
@@ -50,9 +60,43 @@ int create(const owt::Params &p, const owt_opts *opts, owt_handle **out) {
     if (q.precision < 0 || q.precision > 1 || q.method < 0 || q.method > 1)
         throw owt::Error(OWT_E_ARG, "precision and method must be 0 or 1");
     auto h = std::make_unique<owt_handle>();
-    h->ch = std::make_unique<owt::Channel>(q, device, verbose);
+    h->own = std::make_unique<owt::Channel>(q, device, verbose);
+    h->ch = h->own.get();
     *out = h.release();
     return OWT_OK;
+}
+
+owt::Params from_struct(const owt_params &s) {
+    owt::Params p;
+    p.nu = s.nu;
+    p.dt = s.dt;
+    p.dkx = s.dkx;
+    p.dkz = s.dkz;
+    p.alpha_mesh = s.alpha_mesh;
+    p.NX = s.NX;
+    p.NY = s.NY;
+    p.NZ = s.NZ;
+    p.precision = s.precision;
+    p.method = s.method;
+    return p;
+}
+
+template <typename F>
+int eguard(owt_ensemble *e, F &&f) {
+    if (!e) return OWT_E_ARG;
+    try {
+        f();
+        return OWT_OK;
+    } catch (const owt::Error &x) {
+        e->err = x.what();
+        return x.code;
+    } catch (const std::bad_alloc &) {
+        e->err = "host out of memory";
+        return OWT_E_DEVICE;
+    } catch (const std::exception &x) {
+        e->err = x.what();
+        return OWT_E_DEVICE;
+    }
 }
 
 int create_guarded(const owt::Params *p, const char *path, const owt_opts *opts, owt_handle **out) {
@@ -84,25 +128,83 @@ int owt_create(const char *toml_path, const owt_opts *opts, owt_handle **out) {
 
 int owt_create_params(const owt_params *params, const owt_opts *opts, owt_handle **out) {
     if (!params) return OWT_E_ARG;
-    owt::Params p;
-    p.nu = params->nu;
-    p.dt = params->dt;
-    p.dkx = params->dkx;
-    p.dkz = params->dkz;
-    p.alpha_mesh = params->alpha_mesh;
-    p.NX = params->NX;
-    p.NY = params->NY;
-    p.NZ = params->NZ;
-    p.precision = params->precision;
-    p.method = params->method;
+    const owt::Params p = from_struct(*params);
     return create_guarded(&p, nullptr, opts, out);
 }
 
 void owt_destroy(owt_handle *h) {
+    if (!h || h->member) return;  // ensemble members live as long as their ensemble
     try {
         delete h;
     } catch (...) {
     }
+}
+
+int owt_ensemble_create(int n, const owt_params *params, const owt_opts *opts, owt_ensemble **out) {
+    if (!out) return OWT_E_ARG;
+    *out = nullptr;
+    if (n < 1 || !params) return OWT_E_ARG;
+    try {
+        std::vector<owt::Params> ps;
+        for (int i = 0; i < n; ++i) {
+            owt::Params p = from_struct(params[i]);
+            owt::derive_sizes(p);
+            if (opts && opts->precision >= 0) p.precision = opts->precision;
+            if (opts && opts->method >= 0) p.method = opts->method;
+            if (p.precision < 0 || p.precision > 1 || p.method < 0 || p.method > 1)
+                throw owt::Error(OWT_E_ARG, "precision and method must be 0 or 1");
+            ps.push_back(p);
+        }
+        auto e = std::make_unique<owt_ensemble>();
+        e->e = std::make_unique<owt::Ensemble>(ps, opts ? opts->device : 0);
+        for (int i = 0; i < n; ++i) {
+            auto h = std::make_unique<owt_handle>();
+            h->ch = &e->e->member(i);
+            h->member = true;
+            e->members.push_back(std::move(h));
+        }
+        *out = e.release();
+        return OWT_OK;
+    } catch (const owt::Error &x) {
+        std::lock_guard<std::mutex> l(g_mu);
+        g_create_err = x.what();
+        return x.code;
+    } catch (const std::exception &x) {
+        std::lock_guard<std::mutex> l(g_mu);
+        g_create_err = x.what();
+        return OWT_E_DEVICE;
+    }
+}
+
+void owt_ensemble_destroy(owt_ensemble *e) {
+    try {
+        delete e;
+    } catch (...) {
+    }
+}
+
+const char *owt_ensemble_last_error(const owt_ensemble *e) {
+    if (e) return e->err.c_str();
+    std::lock_guard<std::mutex> l(g_mu);
+    return g_create_err.c_str();
+}
+
+owt_handle *owt_ensemble_member(owt_ensemble *e, int m) {
+    if (!e || m < 0 || m >= (int)e->members.size()) return nullptr;
+    return e->members[m].get();
+}
+
+int owt_ensemble_init_fields(owt_ensemble *e, int mode, const uint64_t *seeds, double amp) {
+    return eguard(e, [&] { e->e->init_fields(mode, seeds, amp); });
+}
+
+int owt_ensemble_step(owt_ensemble *e, int nsteps) {
+    return eguard(e, [&] { e->e->step(nsteps); });
+}
+
+int owt_ensemble_time_steps(owt_ensemble *e, int nsteps, double *ms) {
+    if (!ms) return OWT_E_ARG;
+    return eguard(e, [&] { *ms = e->e->time_steps(nsteps); });
 }
 
 const char *owt_last_error(const owt_handle *h) {

@@ -263,6 +263,7 @@ void Channel::check_errors() {
 }
 
 void Channel::drop_graph() {
+    ++generation_;
     for (int r = 0; r < 3; ++r) {
         if (gexec_[r]) (void)hipGraphExecDestroy(gexec_[r]);
         if (graph_[r]) (void)hipGraphDestroy(graph_[r]);
@@ -620,6 +621,16 @@ void Channel::enqueue_kernel(int id) {
 }
 
 void Channel::sync() { OWT_HIP(hipStreamSynchronize(s_)); }
+
+void Channel::prepare_step() {
+    if (P_.method == 1) factor_operators();
+    else ensure_plain_factors();
+}
+
+void Channel::finish() {
+    sync();
+    check_errors();
+}
 
 void Channel::step(int n) {
     if (n < 0) throw Error(2, "step count must be non-negative");

@@ -57,6 +57,22 @@ class Channel {
     void get_factor(int sys, int op, double *lu, int *perm);
     void sync();
 
+    // used by Ensemble, which captures the step bodies of several channels into
+    // one graph: the one-time preparation of a step (factors), the body of one
+    // step at the current ring phase on this channel's stream, the ring advance
+    // after such a body ran, and a counter that changes whenever this channel's
+    // captured kernels would change (precision, method)
+    void prepare_step();
+    int ring() const { return ring_; }
+    int method() const { return P_.method; }
+    void enqueue_body() { enqueue_step_body(ring_); }
+    void advance_ring() {
+        ring_ = (ring_ + 1) % 3;
+        ++host_steps_;
+    }
+    unsigned generation() const { return generation_; }
+    void finish();
+
   private:
     void alloc();
     void free_all();
@@ -100,6 +116,7 @@ class Channel {
     hipGraph_t graph_[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t gexec_[3] = {nullptr, nullptr, nullptr};
     long host_steps_ = 0;
+    unsigned generation_ = 0;
 };
 
 FFTPlan make_plan(int L);

@@ -35,6 +35,8 @@ EXPORTS = [
     "owt_steps_done", "owt_get_field", "owt_set_method", "owt_set_precision",
     "owt_checkpoint", "owt_restore", "owt_f2p", "owt_p2f", "owt_lamb", "owt_get_matrix",
     "owt_time_steps", "owt_time_kernel", "owt_get_factor", "owt_save",
+    "owt_ensemble_create", "owt_ensemble_destroy", "owt_ensemble_last_error", "owt_ensemble_member",
+    "owt_ensemble_init_fields", "owt_ensemble_step", "owt_ensemble_time_steps",
 ]
 
 
@@ -95,6 +97,16 @@ def lib():
         L.owt_time_steps.argtypes = [P, I, P]
         L.owt_time_kernel.argtypes = [P, I, I, P]
         L.owt_get_factor.argtypes = [P, I, I, P, P]
+        L.owt_ensemble_create.argtypes = [I, P, ctypes.POINTER(_Opts), PP]
+        L.owt_ensemble_destroy.argtypes = [P]
+        L.owt_ensemble_destroy.restype = None
+        L.owt_ensemble_last_error.argtypes = [P]
+        L.owt_ensemble_last_error.restype = ctypes.c_char_p
+        L.owt_ensemble_member.argtypes = [P, I]
+        L.owt_ensemble_member.restype = P
+        L.owt_ensemble_init_fields.argtypes = [P, I, P, D]
+        L.owt_ensemble_step.argtypes = [P, I]
+        L.owt_ensemble_time_steps.argtypes = [P, I, P]
         _lib = L
     return _lib
 
@@ -106,27 +118,31 @@ def _p(a):
 class Channel:
     """One channel on one GPU (a handle of the C ABI)."""
 
-    def __init__(self, case=None, *, params=None, device=0, precision=-1, method=-1, verbose=False):
+    def __init__(self, case=None, *, params=None, device=0, precision=-1, method=-1, verbose=False, _member=None):
         L = lib()
-        opts = _Opts(device, precision, method, 1 if verbose else 0)
-        h = ctypes.c_void_p()
-        if case is not None:
-            rc = L.owt_create(os.fsencode(case), ctypes.byref(opts), ctypes.byref(h))
+        self._owner = None
+        if _member is not None:
+            # a member of an Ensemble: borrowed handle, kept alive by the ensemble
+            self._h, self._owner = _member
         else:
-            p = params
-            prm = _Params(p["nu"], p["dt"], p["dkx"], p["dkz"], p["alpha_mesh"], p["NX"], p["NY"], p["NZ"],
-                          p.get("precision", 0), p.get("method", 1))
-            rc = L.owt_create_params(ctypes.byref(prm), ctypes.byref(opts), ctypes.byref(h))
-        if rc:
-            raise OwtError(rc, L.owt_last_error(None).decode())
-        self._h = h
+            opts = _Opts(device, precision, method, 1 if verbose else 0)
+            h = ctypes.c_void_p()
+            if case is not None:
+                rc = L.owt_create(os.fsencode(case), ctypes.byref(opts), ctypes.byref(h))
+            else:
+                prm = _params_struct(params)
+                rc = L.owt_create_params(ctypes.byref(prm), ctypes.byref(opts), ctypes.byref(h))
+            if rc:
+                raise OwtError(rc, L.owt_last_error(None).decode())
+            self._h = h
         sz = np.zeros(7, dtype=np.int32)
         L.owt_get_sizes(self._h, _p(sz))
         self.NX, self.NY, self.NZ, self.NXH, self.NZH, self.NX2, self.NZ2 = (int(v) for v in sz)
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().owt_destroy(self._h)
+            if self._owner is None:
+                lib().owt_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -250,6 +266,73 @@ class Channel:
         return ms.value
 
 
+def _params_struct(p):
+    return _Params(p["nu"], p["dt"], p["dkx"], p["dkz"], p["alpha_mesh"], p["NX"], p["NY"], p["NZ"],
+                   p.get("precision", 0), p.get("method", 1))
+
+
+class Ensemble:
+    """Independent channels on one GPU, stepped together (one captured graph per
+    step with the batched method). ens[m] is member m as a Channel."""
+
+    def __init__(self, params_list, *, device=0, precision=-1, method=-1):
+        L = lib()
+        arr = (_Params * len(params_list))(*[_params_struct(p) for p in params_list])
+        opts = _Opts(device, precision, method, 0)
+        e = ctypes.c_void_p()
+        rc = L.owt_ensemble_create(len(params_list), ctypes.cast(arr, ctypes.c_void_p), ctypes.byref(opts),
+                                   ctypes.byref(e))
+        if rc:
+            raise OwtError(rc, L.owt_ensemble_last_error(None).decode())
+        self._e = e
+        self.members = [Channel(_member=(ctypes.c_void_p(L.owt_ensemble_member(e, m)), self))
+                        for m in range(len(params_list))]
+
+    def __len__(self):
+        return len(self.members)
+
+    def __getitem__(self, m):
+        return self.members[m]
+
+    def _check(self, rc):
+        if rc:
+            raise OwtError(rc, lib().owt_ensemble_last_error(self._e).decode())
+
+    def init_fields(self, mode=2, seeds=None, amp=0.15):
+        if seeds is None:
+            seeds = range(len(self))
+        s = np.ascontiguousarray(seeds, dtype=np.uint64)
+        self._check(lib().owt_ensemble_init_fields(self._e, int(mode), _p(s), float(amp)))
+
+    def step(self, n=1):
+        self._check(lib().owt_ensemble_step(self._e, int(n)))
+
+    def time_steps(self, n):
+        ms = ctypes.c_double()
+        self._check(lib().owt_ensemble_time_steps(self._e, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    @property
+    def gridpoints(self):
+        return sum(m.gridpoints for m in self.members)
+
+    def close(self):
+        if getattr(self, "_e", None):
+            for m in self.members:
+                m._h = None
+            lib().owt_ensemble_destroy(self._e)
+            self._e = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 def case_params(nu, dt, dkx, dkz, alpha_mesh, NX, NY, NZ, precision=0, method=1):
     return dict(nu=nu, dt=dt, dkx=dkx, dkz=dkz, alpha_mesh=alpha_mesh, NX=NX, NY=NY, NZ=NZ,
                 precision=precision, method=method)
@@ -263,4 +346,12 @@ CONFIGS = {
     "re180_128": case_params(1 / 2800, 1e-3, 0.5, 1.5, 1.5, 128, 129, 128),
     # Re_tau 550, box 2pi x 2 x pi, 384x193x384
     "re550_384": case_params(1e-4, 1e-3, 1.0, 2.0, 2.0, 384, 193, 384),
+}
+
+# Ensemble workloads: lists of member parameters.
+ENSEMBLES = {
+    # 32 Re_tau 180 channels of uneven size in the 4pi x 2 x 2pi box:
+    # 16 at 64x65x64, 8 at 64x97x64, 8 at 96x49x96
+    "re180_32": [case_params(1 / 2800, 1e-3, 0.5, 1.0, 1.5, *shape)
+                 for shape in [(64, 65, 64)] * 16 + [(64, 97, 64)] * 8 + [(96, 49, 96)] * 8],
 }
