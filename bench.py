@@ -173,6 +173,51 @@ def ensemble_throughput(owt, name, steps):
             "members_one_by_one_ms_per_step": round(seq, 4), "speedup_vs_one_by_one": round(seq / t, 2)}
 
 
+def adjoint_check(owt, K=20, ckpt=5, first=11, last=20, seeds=range(1000, 1016), eps=1e-6):
+    """Discrete adjoint of K steps at Re_tau 180, 128x129x128, FP64: J = mean friction
+    force over steps first..last, gradient with respect to the initial velocity.
+    Checked on seeded random directions against central finite differences and
+    against the tangent (the adjoint identity <grad J, d> = dJ[d])."""
+    cfg = dict(owt.CONFIGS["re180_128"])
+    with owt.Channel(params=cfg, precision=owt.FP64, method=owt.BATCHED) as ch:
+        ch.init_fields(2, seed=0, amp=0.15)
+        x0 = [ch.field(n) for n in ("u", "v", "w")]
+        ch.set_velocity(*x0)
+        ch.adjoint_run(K, ckpt, first, last)  # warm: factors, graphs, checkpoint buffers
+        ch.set_velocity(*x0)
+        t0 = time.perf_counter()
+        J, g = ch.adjoint_run(K, ckpt, first, last)
+        t_adj = time.perf_counter() - t0
+        ch.set_velocity(*x0)
+        t0 = time.perf_counter()
+        ch.step(K)
+        t_fwd = time.perf_counter() - t0
+
+        def J_of(x):
+            ch.set_velocity(*x)
+            ch.step(K)
+            return ch.diag()[first:last + 1, 0].mean()
+
+        fd_err, id_err = [], []
+        for s in seeds:
+            rng = np.random.default_rng(s)
+            d = [0.1 * (rng.standard_normal(x0[0].shape) + 1j * rng.standard_normal(x0[0].shape)) for _ in range(3)]
+            dJ_a = sum(float(np.vdot(gg, dd).real) for gg, dd in zip(g, d))
+            dJ_fd = (J_of([a + eps * b for a, b in zip(x0, d)]) - J_of([a - eps * b for a, b in zip(x0, d)])) / (2 * eps)
+            ch.set_velocity(*x0)
+            t = ch.tangent_init(*d)
+            ch.step(K)
+            dJ_t = float(t.diag()[first:last + 1, 0].mean())
+            fd_err.append(abs(dJ_a - dJ_fd) / abs(dJ_fd))
+            id_err.append(abs(dJ_a - dJ_t) / abs(dJ_t))
+    return {"workload": "re180_128: 128x129x128, FP64", "K": K, "checkpoint_every": ckpt,
+            "objective": f"mean fx_fric over steps {first}..{last}", "J": J,
+            "adjoint_run_s": round(t_adj, 4), "forward_K_steps_s": round(t_fwd, 4),
+            "directions": len(fd_err), "fd_step": eps,
+            "max_rel_err_vs_central_fd": float("%.3e" % max(fd_err)),
+            "max_rel_err_adjoint_identity_vs_tangent": float("%.3e" % max(id_err))}
+
+
 def copy_calibration(nbytes=1 << 30, reps=5):
     """A device-to-device copy of nbytes, reps times: the reference point that
     scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -196,6 +241,7 @@ def main():
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs (roofline.traffic null)")
     ap.add_argument("--ensemble-steps", type=int, default=100,
                     help="timed steps of the 32-member ensemble workload (0 skips it)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient check")
     ap.add_argument("--accuracy-steps", type=int, default=100,
                     help="steps of the mixed-vs-FP64 accuracy comparison (0 skips it)")
     ap.add_argument("--mode", choices=("full", "profile", "copy"), default="full",
@@ -310,6 +356,9 @@ def main():
     if args.accuracy_steps > 0:
         print("[bench] accuracy mixed vs fp64", file=sys.stderr, flush=True)
         out["accuracy_mixed_vs_fp64"] = accuracy_mixed_vs_fp64(owt, cfg, amp, args.accuracy_steps)
+    if not args.no_adjoint:
+        print("[bench] adjoint", file=sys.stderr, flush=True)
+        out["adjoint"] = adjoint_check(owt)
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(min(16, os.cpu_count() or 1))

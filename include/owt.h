@@ -132,6 +132,32 @@ int owt_get_matrix(owt_handle *h, int which, double *out);
  * factors for validation. */
 int owt_get_factor(owt_handle *h, int sys, int op, double *lu, int *perm);
 
+/* Differentiation (FP64; no reference counterpart).
+ *
+ * Tangent: owt_tangent_init attaches to h a tangent state, the forward-mode
+ * linearization of the step about h's trajectory, started from the direction
+ * (du, dv, dw) of the initial velocity (wave layout) through the linearized
+ * initial post-processing. h must hold an initial field with no steps taken.
+ * owt_tangent_step(h, n) (or owt_step) then advances h and its tangent
+ * together. owt_tangent_handle(h) is the tangent state as a borrowed handle for
+ * owt_get_field, owt_diag (the tangent friction series) and the like; it is
+ * valid until h is re-initialised or destroyed. */
+int owt_tangent_init(owt_handle *h, const double *du, const double *dv, const double *dw);
+owt_handle *owt_tangent_handle(owt_handle *h);
+int owt_tangent_step(owt_handle *h, int nsteps);
+
+/* Adjoint: the discrete adjoint (transpose of the tangent) of K steps from the
+ * initial field h holds (no steps taken). objective OWT_OBJ_MEAN_FX: J = mean of
+ * the friction force fx_fric (the mean wall shear) over steps first..last
+ * (1 <= first <= last <= K). The forward sweep keeps a checkpoint of the state
+ * every ckpt_every steps in device memory; the reverse sweep recomputes each
+ * step's state from its checkpoint. Writes J and dJ/du, dJ/dv, dJ/dw of the
+ * initial field ([NZ][NXH][NY] complex: d/dRe + i d/dIm of each entry), and
+ * leaves h after step K. */
+enum { OWT_OBJ_MEAN_FX = 0 };
+int owt_adjoint_run(owt_handle *h, int K, int ckpt_every, int objective, int first, int last, double *J, double *grad_u,
+                    double *grad_v, double *grad_w);
+
 /* Ensembles: n independent channels on one GPU (sizes and parameters may
  * differ), stepped together. No reference counterpart (the reference runs one
  * channel per job). With the batched method every member's step is captured

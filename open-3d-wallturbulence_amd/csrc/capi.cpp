@@ -16,6 +16,7 @@ struct owt_handle {
     owt::Channel *ch = nullptr;
     std::string err;
     bool member = false;
+    std::unique_ptr<owt_handle> tangent;  // borrowed view of ch->tangent()
 };
 
 struct owt_ensemble {
@@ -138,6 +139,35 @@ void owt_destroy(owt_handle *h) {
         delete h;
     } catch (...) {
     }
+}
+
+int owt_tangent_init(owt_handle *h, const double *du, const double *dv, const double *dw) {
+    if (!du || !dv || !dw) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->tangent_init(du, dv, dw); });
+}
+
+owt_handle *owt_tangent_handle(owt_handle *h) {
+    if (!h || !h->ch || !h->ch->tangent()) return nullptr;
+    if (!h->tangent) {
+        h->tangent = std::make_unique<owt_handle>();
+        h->tangent->member = true;
+    }
+    h->tangent->ch = h->ch->tangent();
+    return h->tangent.get();
+}
+
+int owt_tangent_step(owt_handle *h, int nsteps) {
+    return guard(h, [&] {
+        if (!h->ch->tangent()) throw owt::Error(OWT_E_ARG, "no tangent attached (owt_tangent_init)");
+        h->ch->step(nsteps);
+    });
+}
+
+int owt_adjoint_run(owt_handle *h, int K, int ckpt_every, int objective, int first, int last, double *J, double *grad_u,
+                    double *grad_v, double *grad_w) {
+    if (!J || !grad_u || !grad_v || !grad_w) return OWT_E_ARG;
+    if (objective != OWT_OBJ_MEAN_FX) return OWT_E_ARG;
+    return guard(h, [&] { *J = h->ch->adjoint_run(K, ckpt_every, first, last, grad_u, grad_v, grad_w); });
 }
 
 int owt_ensemble_create(int n, const owt_params *params, const owt_opts *opts, owt_ensemble **out) {

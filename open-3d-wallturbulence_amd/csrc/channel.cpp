@@ -51,12 +51,17 @@ void *Channel::dmalloc(size_t bytes) {
     return p;
 }
 
-Channel::Channel(const Params &p, int device, bool verbose) : P_(p), device_(device), verbose_(verbose) {
+Channel::Channel(const Params &p, int device, bool verbose, hipStream_t stream) : P_(p), device_(device), verbose_(verbose) {
     int ndev = 0;
     OWT_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) throw Error(3, "no HIP device " + std::to_string(device));
     OWT_HIP(hipSetDevice(device));
-    OWT_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    if (stream) {
+        s_ = stream;
+        own_stream_ = false;
+    } else {
+        OWT_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
+    }
     op_ = build_operators((int)P_.NY, P_.alpha_mesh);
     try {
         alloc();
@@ -69,6 +74,8 @@ Channel::Channel(const Params &p, int device, bool verbose) : P_(p), device_(dev
 
 Channel::~Channel() {
     (void)hipSetDevice(device_);
+    tangent_.reset();
+    adjoint_.reset();
     drop_graph();
     free_all();
 }
@@ -77,7 +84,7 @@ void Channel::free_all() {
     if (s_) (void)hipStreamSynchronize(s_);
     for (void *p : allocs_) (void)hipFree(p);
     allocs_.clear();
-    if (s_) (void)hipStreamDestroy(s_);
+    if (s_ && own_stream_) (void)hipStreamDestroy(s_);
     s_ = nullptr;
 }
 
@@ -501,6 +508,7 @@ void Channel::set_velocity(const double *u, const double *v, const double *w) {
     OWT_HIP(hipMemcpyAsync(c_.hist + (size_t)(H_V * 3) * W, v, W * sizeof(double2), hipMemcpyHostToDevice, s_));
     host_steps_ = 0;
     ring_ = 0;
+    tangent_.reset();
     init_post();
 }
 
@@ -512,7 +520,8 @@ void Channel::init_post() {
     const double2 *six[6] = {c_.cur, c_.hist + (size_t)(H_V * 3) * W, c_.cur + W, c_.cur + 2 * W, c_.hist + (size_t)(H_G * 3) * W,
                              c_.cur + 3 * W};
     launch_z_inverse(c_, six, 6, c_.zbuf, s_);
-    launch_x_lamb(c_, s_);
+    if (tangent_base_) launch_x_tangent(c_, (const double2 *)tangent_base_->c_.zbuf, s_);
+    else launch_x_lamb(c_, s_);
     launch_z_forward(c_, s_);
     // Hv, Hg of the initial field land in slot 1 (the new slot of ring phase 0)
     launch_nonlinear(c_, step_ptrs(0), s_);
@@ -561,7 +570,8 @@ void Channel::enqueue_step_body(int ring) {
     const size_t W = c_.W;
     const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
     launch_z_inverse(c_, six, 6, c_.zbuf, s_);
-    launch_x_lamb(c_, s_);
+    if (tangent_base_) launch_x_tangent(c_, (const double2 *)tangent_base_->c_.zbuf, s_);
+    else launch_x_lamb(c_, s_);
     launch_z_forward(c_, s_);
     launch_nonlinear(c_, sp, s_);
 }
@@ -636,6 +646,7 @@ void Channel::step(int n) {
     if (n < 0) throw Error(2, "step count must be non-negative");
     for (int k = 0; k < n; ++k) {
         enqueue_step();
+        if (tangent_) tangent_->enqueue_step();
         if (verbose_) {
             sync();
             double fx, b, u;
@@ -856,6 +867,7 @@ void Channel::checkpoint(const std::string &path) {
 
 void Channel::restore(const std::string &path) {
     sync();
+    tangent_.reset();
     std::ifstream f(path, std::ios::binary);
     if (!f) throw Error(5, "cannot open " + path);
     CkptHeader h{};
@@ -885,6 +897,186 @@ void Channel::restore(const std::string &path) {
     ctrl[C_RING] = 0;
     OWT_HIP(hipMemcpy(c_.ctrl, ctrl, sizeof(ctrl), hipMemcpyHostToDevice));
     load(c_.diag, (size_t)c_.diag_cap * 3 * sizeof(double));
+}
+
+// ---------------------------------------------------------------------------
+// tangent and adjoint
+// ---------------------------------------------------------------------------
+void Channel::save_state(State &st) {
+    const size_t W = c_.W, N = c_.NY;
+    if (!st.hist) {
+        st.hist = (double2 *)dmalloc(12 * W * sizeof(double2));
+        st.d2v = (double2 *)dmalloc(3 * W * sizeof(double2));
+        st.cur = (double2 *)dmalloc(4 * W * sizeof(double2));
+        st.lmb = (double2 *)dmalloc(3 * W * sizeof(double2));
+        st.means = (double2 *)dmalloc(12 * N * sizeof(double2));
+        st.fric = (double *)dmalloc(8 * sizeof(double));
+        st.ctrl = (int *)dmalloc(16 * sizeof(int));
+    }
+    const auto d2d = hipMemcpyDeviceToDevice;
+    OWT_HIP(hipMemcpyAsync(st.hist, c_.hist, 12 * W * sizeof(double2), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(st.d2v, c_.d2v, 3 * W * sizeof(double2), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(st.cur, c_.cur, 4 * W * sizeof(double2), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(st.lmb, c_.lmb, 3 * W * sizeof(double2), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(st.means, c_.means, 12 * N * sizeof(double2), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(st.fric, c_.fric, 8 * sizeof(double), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(st.ctrl, c_.ctrl, 16 * sizeof(int), d2d, s_));
+    st.ring = ring_;
+    st.host_steps = host_steps_;
+}
+
+void Channel::load_state(const State &st) {
+    const size_t W = c_.W, N = c_.NY;
+    const auto d2d = hipMemcpyDeviceToDevice;
+    OWT_HIP(hipMemcpyAsync(c_.hist, st.hist, 12 * W * sizeof(double2), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(c_.d2v, st.d2v, 3 * W * sizeof(double2), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(c_.cur, st.cur, 4 * W * sizeof(double2), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(c_.lmb, st.lmb, 3 * W * sizeof(double2), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(c_.means, st.means, 12 * N * sizeof(double2), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(c_.fric, st.fric, 8 * sizeof(double), d2d, s_));
+    OWT_HIP(hipMemcpyAsync(c_.ctrl, st.ctrl, 16 * sizeof(int), d2d, s_));
+    ring_ = st.ring;
+    host_steps_ = st.host_steps;
+}
+
+void Channel::zero_state() {
+    const size_t W = c_.W, N = c_.NY;
+    OWT_HIP(hipMemsetAsync(c_.hist, 0, 12 * W * sizeof(double2), s_));
+    OWT_HIP(hipMemsetAsync(c_.d2v, 0, 3 * W * sizeof(double2), s_));
+    OWT_HIP(hipMemsetAsync(c_.cur, 0, 4 * W * sizeof(double2), s_));
+    OWT_HIP(hipMemsetAsync(c_.lmb, 0, 3 * W * sizeof(double2), s_));
+    OWT_HIP(hipMemsetAsync(c_.means, 0, 12 * N * sizeof(double2), s_));
+    OWT_HIP(hipMemsetAsync(c_.fric, 0, 8 * sizeof(double), s_));
+}
+
+// the six fields of the level in `slot` (cur holds that level's u, w, omega_x,
+// omega_z) transformed in z into zbuf, as the step leaves them
+void Channel::six_to_zbuf(int slot) {
+    const size_t W = c_.W;
+    const double2 *six[6] = {c_.cur, c_.hist + (size_t)(H_V * 3 + slot) * W, c_.cur + W, c_.cur + 2 * W,
+                             c_.hist + (size_t)(H_G * 3 + slot) * W, c_.cur + 3 * W};
+    launch_z_inverse(c_, six, 6, c_.zbuf, s_);
+}
+
+void Channel::tangent_init(const double *du, const double *dv, const double *dw) {
+    if (c_.precision != 0) throw Error(2, "the tangent needs an FP64 channel");
+    if (steps_done() != 0) throw Error(2, "the tangent starts from an initial field (no steps taken)");
+    if (!tangent_) {
+        Params q = P_;
+        q.precision = 0;
+        q.method = 1;
+        tangent_ = std::make_unique<Channel>(q, device_, false, s_);
+        tangent_->tangent_base_ = this;
+    }
+    six_to_zbuf(ring_);
+    tangent_->set_velocity(du, dv, dw);
+    check_errors();
+}
+
+// the adjoint state shares this channel's stream; its factor set is turned into
+// the transposed form once
+void Channel::adjoint_prepare() {
+    if (transposed_) return;
+    factor_operators();
+    launch_factor_transpose(Fb_, 2 * nfac_, c_.NY, s_);
+    OWT_HIP(hipGetLastError());
+    transposed_ = true;
+}
+
+// transpose of the Lamb vector evaluation: adjoint lmb -> adjoints of the six
+// fields (u, w, omega_x, omega_z in cur; v, omega_y in the history slot), about
+// the base fields in base_zbuf; the adjoint lmb is consumed
+void Channel::adjoint_lamb(const double2 *base_zbuf, int slot) {
+    const size_t W = c_.W, I = c_.I;
+    double2 *lm = (double2 *)c_.lmb, *zb = (double2 *)c_.zbuf, *xb = (double2 *)c_.xbuf;
+    const double2 *in3[3] = {lm, lm + W, lm + 2 * W};
+    launch_z_forward_T(c_, in3, 3, xb, s_);
+    launch_x_adjoint(c_, base_zbuf, xb, zb, s_);
+    double2 *hv = c_.hist + (size_t)(H_V * 3 + slot) * W, *hg = c_.hist + (size_t)(H_G * 3 + slot) * W;
+    double2 *dst[2][3] = {{c_.cur, hv, c_.cur + W}, {c_.cur + 2 * W, hg, c_.cur + 3 * W}};
+    for (int half = 0; half < 2; ++half) {
+        launch_z_inverse_T(c_, zb + (size_t)half * 3 * I, 3, xb, s_);
+        double2 *src[3] = {xb, xb + W, xb + 2 * W};
+        launch_accumulate(dst[half], src, 3, W, s_);
+    }
+    OWT_HIP(hipMemsetAsync(c_.lmb, 0, 3 * W * sizeof(double2), s_));
+}
+
+// transpose of one step at ring phase `ring` (reverse stage order)
+void Channel::adjoint_step(int ring, const double2 *base_zbuf, double seed) {
+    const StepPtrs sp = step_ptrs(ring);
+    launch_nonlinear_adj(c_, sp, s_);
+    adjoint_lamb(base_zbuf, sp.sn);
+    launch_mean_adj(c_, sp, Fb_, Pb_, fmap_host_[0], seed, s_);
+    launch_horizontal_adj(c_, sp, s_);
+    launch_solve_adj(c_, sp, Fb_, Pb_, jobs_, njobs_, s_);
+    OWT_HIP(hipGetLastError());
+}
+
+// transpose of the initial post-processing, down to the initial u, v, w
+void Channel::adjoint_init(const double2 *base_zbuf, double *gu, double *gv, double *gw) {
+    const size_t W = c_.W;
+    launch_init_adj_mean_history(c_, s_);
+    launch_nonlinear_adj(c_, step_ptrs(0), s_);
+    adjoint_lamb(base_zbuf, 0);
+    double2 *g = (double2 *)c_.zbuf;
+    launch_init_vel_adj(c_, g, g + W, g + 2 * W, s_);
+    OWT_HIP(hipGetLastError());
+    OWT_HIP(hipMemcpyAsync(gu, g, W * sizeof(double2), hipMemcpyDeviceToHost, s_));
+    OWT_HIP(hipMemcpyAsync(gv, g + W, W * sizeof(double2), hipMemcpyDeviceToHost, s_));
+    OWT_HIP(hipMemcpyAsync(gw, g + 2 * W, W * sizeof(double2), hipMemcpyDeviceToHost, s_));
+    OWT_HIP(hipStreamSynchronize(s_));
+}
+
+double Channel::adjoint_run(int K, int ckpt, int first, int last, double *gu, double *gv, double *gw) {
+    if (c_.precision != 0) throw Error(2, "the adjoint needs an FP64 channel");
+    if (K < 1 || ckpt < 1 || first < 1 || first > last || last > K)
+        throw Error(2, "adjoint: need K >= 1, ckpt >= 1 and 1 <= first <= last <= K");
+    if (steps_done() != 0) throw Error(2, "the adjoint starts from an initial field (no steps taken)");
+    if (K >= c_.diag_cap) throw Error(2, "adjoint: K exceeds the friction history");
+    tangent_.reset();
+    // forward sweep with checkpoints at steps 0, ckpt, 2 ckpt, ...
+    const int nck = (K - 1) / ckpt + 1;
+    if ((int)ckpts_.size() < nck) ckpts_.resize(nck);
+    for (int n = 0; n < K; ++n) {
+        if (n % ckpt == 0) save_state(ckpts_[n / ckpt]);
+        enqueue_step();
+    }
+    sync();
+    check_errors();
+    std::vector<double> d((size_t)K * 3 + 3);
+    OWT_HIP(hipMemcpy(d.data(), c_.diag, d.size() * sizeof(double), hipMemcpyDeviceToHost));
+    const double wgt = 1.0 / (double)(last - first + 1);
+    double J = 0.0;
+    for (int m = first; m <= last; ++m) J += d[(size_t)m * 3];
+    J *= wgt;
+    if (!adjoint_) {
+        Params q = P_;
+        q.precision = 0;
+        q.method = 1;
+        adjoint_ = std::make_unique<Channel>(q, device_, false, s_);
+    }
+    Channel &A = *adjoint_;
+    A.adjoint_prepare();
+    A.zero_state();
+    // reverse sweep: step n -> n+1 needs the base's six fields of level n+1
+    for (int n = K - 1; n >= 0; --n) {
+        const int c0 = (n / ckpt) * ckpt;
+        load_state(ckpts_[n / ckpt]);
+        for (int m = c0; m <= n; ++m) enqueue_step();
+        const double seed = (n + 1 >= first && n + 1 <= last) ? wgt : 0.0;
+        A.adjoint_step(n % 3, (const double2 *)c_.zbuf, seed);
+    }
+    load_state(ckpts_[0]);
+    six_to_zbuf(0);
+    A.adjoint_init((const double2 *)c_.zbuf, gu, gv, gw);
+    // leave the channel after step K
+    load_state(ckpts_[(K - 1) / ckpt]);
+    for (int m = ((K - 1) / ckpt) * ckpt; m < K; ++m) enqueue_step();
+    sync();
+    check_errors();
+    A.check_errors();
+    return J;
 }
 
 }  // namespace owt

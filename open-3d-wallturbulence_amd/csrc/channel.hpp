@@ -2,6 +2,7 @@
 // solver state (src/setup.hpp:21-83) and its setup/step call sequence
 // (src/main.cpp:63-87).
 #pragma once
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -15,7 +16,8 @@ enum FieldId { F_U = 0, F_V, F_W, F_OMX, F_OMY, F_OMZ, F_LMBX, F_LMBY, F_LMBZ, F
 
 class Channel {
   public:
-    Channel(const Params &p, int device, bool verbose);
+    // stream: a stream to share (the channel does not own it), or null for its own
+    Channel(const Params &p, int device, bool verbose, hipStream_t stream = nullptr);
     ~Channel();
     Channel(const Channel &) = delete;
     Channel &operator=(const Channel &) = delete;
@@ -73,7 +75,37 @@ class Channel {
     unsigned generation() const { return generation_; }
     void finish();
 
+    // Tangent (forward-mode linearization, FP64). tangent_init attaches a tangent
+    // state started from the direction (du, dv, dw) through the linearized
+    // initial post-processing; the channel must hold an initial field. step()
+    // then advances both; tangent() is the tangent state as a channel.
+    void tangent_init(const double *du, const double *dv, const double *dw);
+    Channel *tangent() { return tangent_.get(); }
+    // Discrete adjoint (FP64) of K steps from the initial field held now, for
+    // J = mean of the friction force fx over steps first..last: forward with a
+    // checkpoint every ckpt steps, then the transposed steps in reverse, each
+    // base state recomputed from its checkpoint. Returns J; grad_* get dJ/d(u,
+    // v, w) of the initial field ([NZ][NXH][NY] complex, d/dRe + i d/dIm). The
+    // channel is left after step K.
+    double adjoint_run(int K, int ckpt, int first, int last, double *gu, double *gv, double *gw);
+
   private:
+    // device copy of the whole evolving state (in-memory checkpoint)
+    struct State {
+        double2 *hist = nullptr, *d2v = nullptr, *cur = nullptr, *lmb = nullptr, *means = nullptr;
+        double *fric = nullptr;
+        int *ctrl = nullptr;
+        int ring = 0;
+        long host_steps = 0;
+    };
+    void save_state(State &st);
+    void load_state(const State &st);
+    void zero_state();
+    void six_to_zbuf(int slot);
+    void adjoint_prepare();
+    void adjoint_lamb(const double2 *base_zbuf, int slot);
+    void adjoint_step(int ring, const double2 *base_zbuf, double seed);
+    void adjoint_init(const double2 *base_zbuf, double *gu, double *gv, double *gw);
     void alloc();
     void free_all();
     void upload_operators();
@@ -117,6 +149,14 @@ class Channel {
     hipGraphExec_t gexec_[3] = {nullptr, nullptr, nullptr};
     long host_steps_ = 0;
     unsigned generation_ = 0;
+    bool own_stream_ = true;
+    // tangent: the base this channel linearizes about (its z buffer), or null
+    const Channel *tangent_base_ = nullptr;
+    std::unique_ptr<Channel> tangent_;
+    // adjoint: the adjoint state (its factors hold the transposed form)
+    std::unique_ptr<Channel> adjoint_;
+    bool transposed_ = false;
+    std::vector<State> ckpts_;
 };
 
 FFTPlan make_plan(int L);

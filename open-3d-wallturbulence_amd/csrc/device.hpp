@@ -112,4 +112,28 @@ void launch_init_mean(const Chan &c, hipStream_t s);
 // DY2 v of the initial field into d2v slot 0
 void launch_init_d2v(const Chan &c, hipStream_t s);
 
+// ---- tangent and adjoint (FP64; fft_kernels.hip, lu_kernels.hip, y_kernels.hip) ----
+// x phase of the linearized Lamb vector: c.zbuf (tangent six) with base (base six,
+// z-transformed) -> c.xbuf
+void launch_x_tangent(const Chan &c, const double2 *base, hipStream_t s);
+// transpose of the x phase about base: lam (3 z-padded) -> out (6 z-padded)
+void launch_x_adjoint(const Chan &c, const double2 *base, const double2 *lam, double2 *out, hipStream_t s);
+// transposes of the z forward (nf wave fields -> nf z-padded) and of the z inverse
+// (nf z-padded -> nf wave fields)
+void launch_z_forward_T(const Chan &c, const double2 *const *in, int nf, double2 *out, hipStream_t s);
+void launch_z_inverse_T(const Chan &c, const double2 *in, int nf, double2 *out, hipStream_t s);
+// factor set -> the form solving with A^T (nmat matrices of N x N, in place)
+void launch_factor_transpose(double *F, int nmat, int N, hipStream_t s);
+// transposes of the step's stages on an adjoint state c at the ring phase of sp
+void launch_solve_adj(const Chan &c, const StepPtrs &sp, const double *G, const int *P, const int4 *jobs, int njobs, hipStream_t s);
+void launch_horizontal_adj(const Chan &c, const StepPtrs &sp, hipStream_t s);
+void launch_mean_adj(const Chan &c, const StepPtrs &sp, const double *G, const int *P, int mf, double seed, hipStream_t s);
+void launch_nonlinear_adj(const Chan &c, const StepPtrs &sp, hipStream_t s);
+// transposes of the initial post-processing: mean and history, then the
+// gradient with respect to the initial u, v, w
+void launch_init_adj_mean_history(const Chan &c, hipStream_t s);
+void launch_init_vel_adj(const Chan &c, double2 *gu, double2 *gv, double2 *gw, hipStream_t s);
+// dst[f] += src[f], f < nf (<= 6), n complex values each
+void launch_accumulate(double2 *const *dst, double2 *const *src, int nf, size_t n, hipStream_t s);
+
 }  // namespace owt

@@ -161,8 +161,11 @@ struct PtrTable {
     const double2 *p[6];
 };
 
+// div divides the input (1 in the step); nyq = 1 also feeds the mode at index
+// NZH into the Nyquist slot NZ (0 in the step, which drops it): with div = NZ2
+// and nyq = 1 the kernel is the transpose of k_z_forward.
 template <typename T, int TJ, int TL, int LMAX>
-__global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typename CT<T>::t *out, int nf) {
+__global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typename CT<T>::t *out, int nf, T div, int nyq) {
     using C = typename CT<T>::t;
     extern __shared__ unsigned char smem[];
     C *tw = reinterpret_cast<C *>(smem);
@@ -180,10 +183,10 @@ __global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typen
     const double2 *src = in.p[f];
     for (int n = tl; n < L; n += TL) {
         // slot n <- mode n (n < NZH); slots NZH..NZ are padding and the dropped z-Nyquist
-        const int mode = n < NZH ? n : (n <= NZ ? -1 : n - NZH);
+        const int mode = n < NZH ? n : (n < NZ || (n == NZ && !nyq) ? -1 : n - NZH);
         C v = mk<C>(0, 0);
         if (mode >= 0 && valid) v = load_c<C>(src + ((size_t)mode * NXH + i) * NY + j);
-        buf[n * TJ + jj] = v;
+        buf[n * TJ + jj] = mk<C>(v.x / div, v.y / div);
     }
     __syncthreads();
     fft_lines<C, TJ, TL, LMAX>(buf, tw, c.pz, +1, jj, tl);
@@ -195,8 +198,11 @@ __global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typen
 // ---------------------------------------------------------------------------
 // z forward (src/FFT.cpp:171-190): per (i, y tile, field)
 // ---------------------------------------------------------------------------
+// div divides the output (NZ2 in the step); nyq = 0 sets the mode at index NZH
+// to zero (1 in the step): with div = 1 and nyq = 0 the kernel is the transpose
+// of k_z_inverse.
 template <typename T, int TJ, int TL, int LMAX, typename OutC>
-__global__ __launch_bounds__(TJ *TL) void k_z_forward(Chan c, const typename CT<T>::t *in, OutC *out, int nf) {
+__global__ __launch_bounds__(TJ *TL) void k_z_forward(Chan c, const typename CT<T>::t *in, OutC *out, int nf, T div, int nyq) {
     using C = typename CT<T>::t;
     extern __shared__ unsigned char smem[];
     C *tw = reinterpret_cast<C *>(smem);
@@ -216,15 +222,15 @@ __global__ __launch_bounds__(TJ *TL) void k_z_forward(Chan c, const typename CT<
     __syncthreads();
     fft_lines<C, TJ, TL, LMAX>(buf, tw, c.pz, -1, jj, tl);
     OutC *dst = out + (size_t)f * c.W;
-    const T sc = (T)L;
     if (valid)
         for (int k = tl; k < NZ; k += TL) {
             // mode k <- bin k (k < NZH); mode k >= NZH <- bin k + NZH (negative frequencies)
             const int bin = k < NZH ? k : k + NZH;
             C v = buf[bin * TJ + jj];
+            if (k == NZH && !nyq) v = mk<C>(0, 0);
             OutC o;
-            o.x = v.x / sc;
-            o.y = v.y / sc;
+            o.x = v.x / div;
+            o.y = v.y / div;
             dst[((size_t)k * NXH + i) * NY + j] = o;
         }
 }
@@ -321,6 +327,195 @@ __global__ __launch_bounds__(TJ *TL) void k_x_lamb(Chan c, const typename CT<T>:
                 } else {
                     out[2 * c.I + plane + (size_t)n * NY] = mk<C>(zn.x / sc, zn.y / sc);
                 }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Linearized and adjoint x phase (FP64), for the tangent and adjoint steps.
+// base holds the six z-transformed fields of the state linearized about (u, v,
+// w, omx, omy, omz), in the layout of the step's z buffer.
+//
+// k_x_tangent: tangent six fields (same layout) -> dL = du x omega + u x domega,
+//   transformed as the Lamb vector (three fields, the step's x buffer layout).
+// k_x_adjoint: the transpose. lam holds the adjoint of the three x-phase
+//   outputs; with lam_phys = (its transpose of the forward x transform) the
+//   adjoints of the six inputs are omega x lam_phys (for u) and lam_phys x u
+//   (for omega), taken back through the transpose of the c2r transform.
+//   Transposes, with theta = 2 pi / NX2 and modes m < NXH:
+//     forward  X_m = (1/NX2) sum_j x_j e^{-i theta m j}
+//       -> x*_j = (1/NX2) (Re a_0 + Re sum_{m>=1} a_m e^{i theta m j})
+//          = the c2r transform of (a_0, a_m / 2) divided by NX2
+//     c2r      x_j = Re c_0 + 2 Re sum_{m>=1} c_m e^{i theta m j}
+//       -> c*_0 = sum_j x*_j,  c*_m = 2 sum_j x*_j e^{-i theta m j}
+//          = the unnormalised forward transform, modes m >= 1 doubled
+// ---------------------------------------------------------------------------
+template <typename C>
+__device__ __forceinline__ void load_pair_scaled(C *buf, const C *A, const C *B, int L, int NXH, size_t stride_i, bool valid,
+                                                 int jj, int tl, int TL, int TJ, double s0, double sm) {
+    for (int m = tl; m < NXH; m += TL) {
+        C a = mk<C>(0, 0), b = mk<C>(0, 0);
+        if (valid) {
+            a = A[(size_t)m * stride_i];
+            if (B) b = B[(size_t)m * stride_i];
+        }
+        const double sc = m == 0 ? s0 : sm;
+        a = mk<C>(a.x * sc, a.y * sc);
+        b = mk<C>(b.x * sc, b.y * sc);
+        if (m == 0) {
+            buf[jj] = mk<C>(a.x, b.x);
+        } else {
+            buf[m * TJ + jj] = mk<C>(a.x - b.y, a.y + b.x);
+            buf[(L - m) * TJ + jj] = mk<C>(a.x + b.y, b.x - a.y);
+        }
+    }
+    for (int n = NXH + tl; n <= L - NXH; n += TL) buf[n * TJ + jj] = mk<C>(0, 0);
+}
+
+// c2r of field pairs (f0, f1) of src into physical values held per thread
+template <int TJ, int TL, int LMAX, int EPT>
+__device__ __forceinline__ void x_c2r_pair(double2 *buf, const double2 *tw, const Chan &c, const double2 *A, const double2 *B,
+                                           bool valid, int jj, int tl, double s0, double sm, double2 (&out)[EPT]) {
+    load_pair_scaled<double2>(buf, A, B, c.NX2, c.NXH, (size_t)c.NY, valid, jj, tl, TL, TJ, s0, sm);
+    __syncthreads();
+    fft_lines<double2, TJ, TL, LMAX>(buf, tw, c.px, +1, jj, tl);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int n = tl + e * TL;
+        if (n < c.NX2) out[e] = buf[n * TJ + jj];
+    }
+    __syncthreads();
+}
+
+template <int TJ, int TL, int LMAX>
+__global__ __launch_bounds__(TJ *TL) void k_x_tangent(Chan c, const double2 *base, const double2 *in, double2 *out) {
+    using C = double2;
+    constexpr int EPT = (LMAX + TL - 1) / TL;
+    extern __shared__ unsigned char smem[];
+    C *tw = reinterpret_cast<C *>(smem);
+    C *buf = tw + LMAX;
+    const int L = c.NX2, NXH = c.NXH, NY = c.NY;
+    const int nyt = (NY + TJ - 1) / TJ;
+    const int tile = xcd_tile(nyt * c.NZ2);
+    if (tile < 0) return;
+    const int k2 = tile / nyt;
+    const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
+    const int j = (tile % nyt) * TJ + jj;
+    const bool valid = j < NY;
+    for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = c.twx[n];
+    const size_t plane = (size_t)k2 * NXH * NY + j;
+    // base pairs (u, v), (w, omx), (omy, omz)
+    C b[3][EPT];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        x_c2r_pair<TJ, TL, LMAX, EPT>(buf, tw, c, base + (size_t)(2 * q) * c.I + plane, base + (size_t)(2 * q + 1) * c.I + plane,
+                                      valid, jj, tl, 1.0, 1.0, b[q]);
+    double lx[EPT], ly[EPT], lz[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) lx[e] = ly[e] = lz[e] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        C t[EPT];
+        x_c2r_pair<TJ, TL, LMAX, EPT>(buf, tw, c, in + (size_t)(2 * q) * c.I + plane, in + (size_t)(2 * q + 1) * c.I + plane, valid,
+                                      jj, tl, 1.0, 1.0, t);
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const double u = b[0][e].x, v = b[0][e].y, w = b[1][e].x, ox = b[1][e].y, oy = b[2][e].x, oz = b[2][e].y;
+            if (q == 0) {  // du, dv
+                const double du = t[e].x, dv = t[e].y;
+                lx[e] += dv * oz;
+                ly[e] -= du * oz;
+                lz[e] += du * oy - dv * ox;
+            } else if (q == 1) {  // dw, domx
+                const double dw = t[e].x, dox = t[e].y;
+                lx[e] -= dw * oy;
+                ly[e] += dw * ox + w * dox;
+                lz[e] -= v * dox;
+            } else {  // domy, domz
+                const double doy = t[e].x, doz = t[e].y;
+                lx[e] += v * doz - w * doy;
+                ly[e] -= u * doz;
+                lz[e] += u * doy;
+            }
+        }
+    }
+    const double sc = (double)L;
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int n = tl + e * TL;
+            if (n < L) buf[n * TJ + jj] = pass == 0 ? mk<C>(lx[e], ly[e]) : mk<C>(lz[e], 0.0);
+        }
+        __syncthreads();
+        fft_lines<C, TJ, TL, LMAX>(buf, tw, c.px, -1, jj, tl);
+        if (valid) {
+            for (int n = tl; n < NXH; n += TL) {
+                const C zn = buf[n * TJ + jj];
+                if (pass == 0) {
+                    const C zm = buf[((L - n) % L) * TJ + jj];
+                    out[0 * c.I + plane + (size_t)n * NY] = mk<C>((zn.x + zm.x) * 0.5 / sc, (zn.y - zm.y) * 0.5 / sc);
+                    out[1 * c.I + plane + (size_t)n * NY] = mk<C>((zn.y + zm.y) * 0.5 / sc, (zm.x - zn.x) * 0.5 / sc);
+                } else {
+                    out[2 * c.I + plane + (size_t)n * NY] = mk<C>(zn.x / sc, zn.y / sc);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int TJ, int TL, int LMAX>
+__global__ __launch_bounds__(TJ *TL) void k_x_adjoint(Chan c, const double2 *base, const double2 *lam, double2 *out) {
+    using C = double2;
+    constexpr int EPT = (LMAX + TL - 1) / TL;
+    extern __shared__ unsigned char smem[];
+    C *tw = reinterpret_cast<C *>(smem);
+    C *buf = tw + LMAX;
+    const int L = c.NX2, NXH = c.NXH, NY = c.NY;
+    const int nyt = (NY + TJ - 1) / TJ;
+    const int tile = xcd_tile(nyt * c.NZ2);
+    if (tile < 0) return;
+    const int k2 = tile / nyt;
+    const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
+    const int j = (tile % nyt) * TJ + jj;
+    const bool valid = j < NY;
+    for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = c.twx[n];
+    const size_t plane = (size_t)k2 * NXH * NY + j;
+    C b[3][EPT], lp[2][EPT];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        x_c2r_pair<TJ, TL, LMAX, EPT>(buf, tw, c, base + (size_t)(2 * q) * c.I + plane, base + (size_t)(2 * q + 1) * c.I + plane,
+                                      valid, jj, tl, 1.0, 1.0, b[q]);
+    const double il = 1.0 / (double)L;
+    // (lam_x, lam_y) and (lam_z, 0): the transpose of the forward x transform
+    x_c2r_pair<TJ, TL, LMAX, EPT>(buf, tw, c, lam + plane, lam + (size_t)c.I + plane, valid, jj, tl, il, 0.5 * il, lp[0]);
+    x_c2r_pair<TJ, TL, LMAX, EPT>(buf, tw, c, lam + (size_t)2 * c.I + plane, nullptr, valid, jj, tl, il, 0.5 * il, lp[1]);
+    // output pairs (u*, v*), (w*, omx*), (omy*, omz*)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int n = tl + e * TL;
+            const double u = b[0][e].x, v = b[0][e].y, w = b[1][e].x, ox = b[1][e].y, oy = b[2][e].x, oz = b[2][e].y;
+            const double ax = lp[0][e].x, ay = lp[0][e].y, az = lp[1][e].x;
+            C r;
+            if (q == 0) r = mk<C>(oy * az - oz * ay, oz * ax - ox * az);
+            else if (q == 1) r = mk<C>(ox * ay - oy * ax, ay * w - az * v);
+            else r = mk<C>(az * u - ax * w, ax * v - ay * u);
+            if (n < L) buf[n * TJ + jj] = r;
+        }
+        __syncthreads();
+        fft_lines<C, TJ, TL, LMAX>(buf, tw, c.px, -1, jj, tl);
+        if (valid) {
+            for (int n = tl; n < NXH; n += TL) {
+                const C zn = buf[n * TJ + jj], zm = buf[((L - n) % L) * TJ + jj];
+                // spectra of the two real sequences, modes m >= 1 doubled
+                const double f = n == 0 ? 0.5 : 1.0;
+                out[(size_t)(2 * q) * c.I + plane + (size_t)n * NY] = mk<C>((zn.x + zm.x) * f, (zn.y - zm.y) * f);
+                out[(size_t)(2 * q + 1) * c.I + plane + (size_t)n * NY] = mk<C>((zn.y + zm.y) * f, (zm.x - zn.x) * f);
             }
         }
         __syncthreads();
@@ -473,13 +668,13 @@ void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out
         OWT_Z_DISPATCH(c.NZ2, double, {
             dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * nf));
             hipLaunchKernelGGL((k_z_inverse<double, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, t,
-                               (double2 *)out, nf);
+                               (double2 *)out, nf, 1.0, 0);
         });
     else
         OWT_Z_DISPATCH(c.NZ2, float, {
             dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * nf));
             hipLaunchKernelGGL((k_z_inverse<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c, t,
-                               (float2 *)out, nf);
+                               (float2 *)out, nf, 1.0f, 0);
         });
 }
 
@@ -503,13 +698,13 @@ void launch_z_forward(const Chan &c, hipStream_t s) {
         OWT_Z_DISPATCH(c.NZ2, double, {
             dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * 3));
             hipLaunchKernelGGL((k_z_forward<double, TJ, TL, LMAX, double2>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s,
-                               c, (const double2 *)c.xbuf, (double2 *)c.lmb, 3);
+                               c, (const double2 *)c.xbuf, (double2 *)c.lmb, 3, (double)c.NZ2, 1);
         });
     else
         OWT_Z_DISPATCH(c.NZ2, float, {
             dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * 3));
             hipLaunchKernelGGL((k_z_forward<float, TJ, TL, LMAX, float2>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
-                               (const float2 *)c.xbuf, (float2 *)c.lmb, 3);
+                               (const float2 *)c.xbuf, (float2 *)c.lmb, 3, (float)c.NZ2, 1);
         });
 }
 
@@ -519,7 +714,42 @@ void launch_z_forward_to(const Chan &c, const void *in, int nf, double2 *out, hi
     OWT_Z_DISPATCH(c.NZ2, double, {
         dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * nf));
         hipLaunchKernelGGL((k_z_forward<double, TJ, TL, LMAX, double2>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, d,
-                           (const double2 *)in, out, nf);
+                           (const double2 *)in, out, nf, (double)c.NZ2, 1);
+    });
+}
+
+void launch_x_tangent(const Chan &c, const double2 *base, hipStream_t s) {
+    OWT_X_DISPATCH(c.NX2, double, {
+        dim3 g(xcd_grid(c.NZ2 * ((c.NY + TJ - 1) / TJ)));
+        hipLaunchKernelGGL((k_x_tangent<TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, base,
+                           (const double2 *)c.zbuf, (double2 *)c.xbuf);
+    });
+}
+
+void launch_x_adjoint(const Chan &c, const double2 *base, const double2 *lam, double2 *out, hipStream_t s) {
+    OWT_X_DISPATCH(c.NX2, double, {
+        dim3 g(xcd_grid(c.NZ2 * ((c.NY + TJ - 1) / TJ)));
+        hipLaunchKernelGGL((k_x_adjoint<TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, base, lam, out);
+    });
+}
+
+// transposes of the z transforms of the step (FP64): of k_z_forward (divide by
+// NZ2, the Nyquist mode fed) and of k_z_inverse (no division, Nyquist mode zero)
+void launch_z_forward_T(const Chan &c, const double2 *const *in, int nf, double2 *out, hipStream_t s) {
+    PtrTable t{};
+    for (int f = 0; f < nf; ++f) t.p[f] = in[f];
+    OWT_Z_DISPATCH(c.NZ2, double, {
+        dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * nf));
+        hipLaunchKernelGGL((k_z_inverse<double, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, t, out, nf,
+                           (double)c.NZ2, 1);
+    });
+}
+
+void launch_z_inverse_T(const Chan &c, const double2 *in, int nf, double2 *out, hipStream_t s) {
+    OWT_Z_DISPATCH(c.NZ2, double, {
+        dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * nf));
+        hipLaunchKernelGGL((k_z_forward<double, TJ, TL, LMAX, double2>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, in,
+                           out, nf, 1.0, 0);
     });
 }
 

@@ -356,6 +356,88 @@ __global__ __launch_bounds__(256) void k_psolve(Chan c, const double *F, const i
 }
 
 // ---------------------------------------------------------------------------
+// Transposed solves for the adjoint step. With P A = L U (dgetrf), A^T = U^T L^T P.
+// Writing D = diag(U), U^T = L' D with L' = U^T D^-1 unit lower, so
+//   A^T = L' (D L^T) P,
+// a unit-lower factor times an upper one, solved by the same substitution as A,
+// followed by y[perm[k]] = w[k]. k_factor_transpose turns each stored factor into
+// that form in place: G[j,k] = U[k,j] / U[k,k] and G[k,j] = U[k,k] L[j,k] (j > k).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_factor_transpose(double *F, int N) {
+    double *M = F + (size_t)blockIdx.y * N * N;
+    const int k = blockIdx.x;
+    const double d = M[(size_t)k * N + k];
+    for (int j = k + 1 + threadIdx.x; j < N; j += blockDim.x) {
+        const double l = M[(size_t)k * N + j], u = M[(size_t)j * N + k];
+        M[(size_t)k * N + j] = u / d;
+        M[(size_t)j * N + k] = d * l;
+    }
+}
+
+// Transpose of k_solve (batched jobs): the adjoint of the new level (slot sn)
+// through A^-T, boundary rows zeroed, into the adjoints of the three history
+// levels of v or omega_y, Hv or Hg and (v only) DY2 v. The adjoint of slot sn
+// itself becomes that of the level it held before the step (l = 2).
+template <int NQ>
+__global__ __launch_bounds__(256) void k_solve_adj(Chan c, StepPtrs sp, const double *G, const int *P, const int4 *jobs, int njobs) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int job = blockIdx.x * 2 + (w >> 1), op = w & 1;
+    const int N = c.NY;
+    if (job >= njobs) return;
+    const int4 jb = jobs[job];
+    int pc[4];
+    job_pencils(c, jb, pc);
+    const int kz = jb.y / c.NXH, ix = jb.y % c.NXH;
+    double2 *X[3], *H[3], *D[3];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+        X[l] = const_cast<double2 *>(op == 0 ? sp.v[l] : sp.g[l]);
+        H[l] = const_cast<double2 *>(op == 0 ? sp.hv[l] : sp.hg[l]);
+        D[l] = const_cast<double2 *>(sp.dv[l]);
+    }
+    const double2 *XN = op == 0 ? sp.vn : sp.gn;
+    const double dt = c.dt;
+    const double k2dt = __ddiv_rn(k2_of(c, kz, ix), dt);
+    const double *Go = G + ((size_t)jb.x * 2 + op) * N * N;
+    const int *Po = P + ((size_t)jb.x * 2 + op) * N;
+    double t[8][NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int row = lane + 64 * q;
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) {
+            double2 v = XN[(size_t)(pc[pp] >= 0 ? pc[pp] : pc[0]) * N + (row < N ? row : 0)];
+            if (row >= N || pc[pp] < 0) v = c2(0, 0);
+            t[2 * pp][q] = v.x;
+            t[2 * pp + 1][q] = v.y;
+        }
+    }
+    lu_substitute<NQ, 8>(Go, N, lane, t);
+    const double AL[3] = {AL0, AL1, AL2}, BE[3] = {BE0, BE1, BE2};
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int k = lane + 64 * q;
+        if (k >= N) continue;
+        int r = Po[k];
+        if ((unsigned)r >= (unsigned)N) r = 0;
+        const bool bc = op == 0 ? (r <= 1 || r >= N - 2) : (r == 0 || r == N - 1);
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) {
+            if (pc[pp] < 0) continue;
+            const size_t o = (size_t)pc[pp] * N + r;
+            const double2 mu = bc ? c2(0, 0) : c2(t[2 * pp][q], t[2 * pp + 1][q]);
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                const double2 ax = op == 0 ? cscale(mu, -mul(AL[l], k2dt)) : cdiv(cscale(mu, AL[l]), dt);
+                X[l][o] = l == 2 ? ax : caddc(X[l][o], ax);
+                H[l][o] = caddc(H[l][o], cscale(mu, BE[l]));
+                if (op == 0) D[l][o] = caddc(D[l][o], cdiv(cscale(mu, AL[l]), dt));
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
 #define OWT_FACTOR_DISPATCH(N, BODY)                                                    \
@@ -382,6 +464,17 @@ void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int 
         OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false>), dim3(blocks), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
 }
 
+
+void launch_factor_transpose(double *F, int nmat, int N, hipStream_t s) {
+    for (int m0 = 0; m0 < nmat; m0 += 65535) {
+        const int nm = nmat - m0 < 65535 ? nmat - m0 : 65535;
+        hipLaunchKernelGGL(k_factor_transpose, dim3(N, nm), dim3(256), 0, s, F + (size_t)m0 * N * N, N);
+    }
+}
+
+void launch_solve_adj(const Chan &c, const StepPtrs &sp, const double *G, const int *P, const int4 *jobs, int njobs, hipStream_t s) {
+    OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_solve_adj<NQ>, dim3((njobs + 1) / 2), dim3(256), 0, s, c, sp, G, P, jobs, njobs));
+}
 
 void launch_pressure_solve(const Chan &c, const double *F, const int *P, const int4 *jobs, int njobs, const double2 *R,
                            double2 *out, hipStream_t s) {

@@ -454,6 +454,358 @@ __global__ __launch_bounds__(256) void k_init_mean(Chan c) {
 }
 
 // ---------------------------------------------------------------------------
+// Adjoints (transposes) of the wall-normal stages, FP64. The channel c is an
+// adjoint state: each buffer holds the adjoint of the forward buffer of the same
+// name and slot. A stage that overwrites a forward buffer consumes that buffer's
+// adjoint (reads it, then zeroes it) and adds to the adjoints of its inputs.
+// Complex scalars enter conjugated: the transpose of v -> i s v is v -> -i s v.
+// ---------------------------------------------------------------------------
+
+// (DY^T x)_row = sum_j DY(j, row) x_j: column `row` of the column-major DY
+template <int TP>
+__device__ __forceinline__ void matvec_T(const double *DY, const double2 (*sx)[kMaxNY], int N, int row, double2 (&acc)[TP]) {
+    const double *col = DY + (size_t)row * N;
+    for (int j = 0; j < N; ++j) {
+        const double a = col[j];
+#pragma unroll
+        for (int pp = 0; pp < TP; ++pp) {
+            acc[pp].x = fma(a, sx[pp][j].x, acc[pp].x);
+            acc[pp].y = fma(a, sx[pp][j].y, acc[pp].y);
+        }
+    }
+}
+
+// transpose of k_horizontal: adjoints of u, w, omega_x, omega_z (all pencils
+// but k = 0) and of DY2 v (slot sn) into the adjoints of v and omega_y (slot sn)
+template <int TP>
+__global__ __launch_bounds__(256) void k_horizontal_adj(Chan c, StepPtrs sp) {
+    __shared__ double2 s1v[TP][kMaxNY], s2v[TP][kMaxNY], s1g[TP][kMaxNY];
+    const int N = c.NY;
+    double2 *au = c.cur, *aw = c.cur + c.W, *ax = c.cur + 2 * c.W, *az = c.cur + 3 * c.W;
+    double2 *av = sp.vn, *ag = sp.gn, *ad = sp.dvn;
+    const int p0 = blockIdx.x * TP;
+    for (int e = threadIdx.x; e < TP * N; e += blockDim.x) {
+        const int pp = e / N, j = e % N, p = p0 + pp;
+        double2 d1v = c2(0, 0), d2v = c2(0, 0), d1g = c2(0, 0);
+        if (p < c.npencil) {
+            const int k = p / c.NXH, i = p % c.NXH;
+            const size_t o = (size_t)p * N + j;
+            d2v = ad[o];
+            const double k2 = k2_of(c, k, i);
+            if (k2 >= 1.e-8) {
+                const double kx = c.kx[i], kz = c.kz[k];
+                const double ix = __ddiv_rn(kx, k2), iz = __ddiv_rn(kz, k2), ir = __ddiv_rn(1.0, k2);
+                const double2 U = au[o], Wv = aw[o], X = ax[o], Z = az[o];
+                d1v = caddc(imul(-ix, U), imul(-iz, Wv));
+                d1g = caddc(imul(-mul(ir, kx), X), imul(-mul(ir, kz), Z));
+                d2v = caddc(d2v, caddc(imul(-mul(ir, kz), X), imul(mul(ir, kx), Z)));
+            }
+        }
+        s1v[pp][j] = d1v;
+        s2v[pp][j] = d2v;
+        s1g[pp][j] = d1g;
+    }
+    __syncthreads();
+    const int row = threadIdx.x;
+    if (row >= N) return;
+    double2 gv[TP], gg[TP], tmp[TP];
+#pragma unroll
+    for (int pp = 0; pp < TP; ++pp) gv[pp] = gg[pp] = tmp[pp] = c2(0, 0);
+    matvec_T<TP>(c.DY1, s1v, N, row, gv);
+    matvec_T<TP>(c.DY2, s2v, N, row, tmp);
+    matvec_T<TP>(c.DY1, s1g, N, row, gg);
+#pragma unroll
+    for (int pp = 0; pp < TP; ++pp) {
+        const int p = p0 + pp;
+        if (p >= c.npencil) break;
+        const int k = p / c.NXH, i = p % c.NXH;
+        const size_t o = (size_t)p * N + row;
+        double2 v = caddc(gv[pp], tmp[pp]), g = gg[pp];
+        const double k2 = k2_of(c, k, i);
+        if (k2 >= 1.e-8) {
+            const double kx = c.kx[i], kz = c.kz[k];
+            const double ix = __ddiv_rn(kx, k2), iz = __ddiv_rn(kz, k2), ir = __ddiv_rn(1.0, k2);
+            const double2 U = au[o], Wv = aw[o], X = ax[o], Z = az[o];
+            v = caddc(v, caddc(imul(mul(ir, mul(kz, k2)), X), imul(-mul(ir, mul(kx, k2)), Z)));
+            g = caddc(g, caddc(imul(iz, U), imul(-ix, Wv)));
+            au[o] = aw[o] = ax[o] = az[o] = c2(0, 0);
+        }
+        av[o] = caddc(av[o], v);
+        ag[o] = caddc(ag[o], g);
+        ad[o] = c2(0, 0);
+    }
+}
+
+// transpose of k_nonlinear: adjoints of Hv, Hg (slot sn) and of the mean Lamb
+// history (slot sn) added into the adjoint Lamb vector lmb (FP64)
+template <int TP>
+__global__ __launch_bounds__(256) void k_nonlinear_adj(Chan c, StepPtrs sp) {
+    __shared__ double2 sh[TP][kMaxNY];
+    const int N = c.NY, ns = sp.sn;
+    double2 *lx = reinterpret_cast<double2 *>(c.lmb), *ly = lx + c.W, *lz = lx + 2 * c.W;
+    double2 *Hv = sp.hvn, *Hg = sp.hgn;
+    const int p0 = blockIdx.x * TP;
+    for (int e = threadIdx.x; e < TP * N; e += blockDim.x) {
+        const int pp = e / N, j = e % N, p = p0 + pp;
+        sh[pp][j] = p < c.npencil ? Hv[(size_t)p * N + j] : c2(0, 0);
+    }
+    __syncthreads();
+    const int row = threadIdx.x;
+    if (row >= N) return;
+    double2 t[TP];
+#pragma unroll
+    for (int pp = 0; pp < TP; ++pp) t[pp] = c2(0, 0);
+    matvec_T<TP>(c.DY1, sh, N, row, t);  // DY1^T Hv*; the adjoint of t is its negative
+#pragma unroll
+    for (int pp = 0; pp < TP; ++pp) {
+        const int p = p0 + pp;
+        if (p >= c.npencil) break;
+        const int k = p / c.NXH, i = p % c.NXH;
+        const double kx = c.kx[i], kz = c.kz[k];
+        const double k2 = k2_of(c, k, i);
+        const size_t o = (size_t)p * N + row;
+        const double2 at = c2(-t[pp].x, -t[pp].y), hg = Hg[o];
+        double2 ax = caddc(imul(-kx, at), imul(-kz, hg));
+        double2 az = caddc(imul(-kz, at), imul(kx, hg));
+        if (p == 0) {
+            double2 *mx = c.means + (size_t)(M_LMBX * 3 + ns) * N, *mz = c.means + (size_t)(M_LMBZ * 3 + ns) * N;
+            ax = caddc(ax, mx[row]);
+            az = caddc(az, mz[row]);
+            mx[row] = mz[row] = c2(0, 0);
+        }
+        lx[o] = caddc(lx[o], ax);
+        ly[o] = caddc(ly[o], cscale(sh[pp][row], -k2));
+        lz[o] = caddc(lz[o], az);
+        Hv[o] = Hg[o] = c2(0, 0);
+    }
+}
+
+// transpose of the friction evaluation: adjoints of U (msol) and W from the
+// adjoints of the friction forces fx, fz (real) and of DY1 U, DY1 W (rows)
+__device__ void friction_adj(const Chan &c, double2 *sdu, double2 *sdw, double afx, double afz, double2 *aU, double2 *aW) {
+    const int N = c.NY;
+    const double f = c.nu / c.Ly;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // fx = -nu/Ly (DY1U[N-1] - DY1U[0]) on the real parts
+        sdu[N - 1].x -= f * afx;
+        sdu[0].x += f * afx;
+        sdw[N - 1].x -= f * afz;
+        sdw[0].x += f * afz;
+    }
+    __syncthreads();
+    const int row = threadIdx.x;
+    if (row < N) {
+        double2 u = c2(0, 0), w = c2(0, 0);
+        const double *col = c.DY1 + (size_t)row * N;
+        for (int j = 0; j < N; ++j) {
+            const double a = col[j];
+            u.x = fma(a, sdu[j].x, u.x);
+            u.y = fma(a, sdu[j].y, u.y);
+            w.x = fma(a, sdw[j].x, w.x);
+            w.y = fma(a, sdw[j].y, w.y);
+        }
+        aU[row] = caddc(aU[row], u);
+        aW[row] = caddc(aW[row], w);
+    }
+    __syncthreads();
+}
+
+// transpose of k_mean (one block). seed is added to the adjoint of this step's
+// fx (the objective's weight on it). G, P: the transposed factor set.
+template <int NQ>
+__global__ __launch_bounds__(256) void k_mean_adj(Chan c, StepPtrs sp, const double *G, const int *P, int mf, double seed) {
+    __shared__ double2 sdu[kMaxNY], sdw[kMaxNY], aU[kMaxNY], aW[kMaxNY];
+    __shared__ double S[4][64 * NQ];
+    __shared__ double red[2][256];
+    const int N = c.NY, lane = threadIdx.x & 63, ns = sp.sn;
+    const double dt = c.dt;
+    const double afx = c.fric[ns] + seed, afz = c.fric[3 + ns];
+    double2 *mvx = c.means + (size_t)(M_VELX * 3 + ns) * N, *mvz = c.means + (size_t)(M_VELZ * 3 + ns) * N;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const double2 x = c.cur[2 * c.W + j], z = c.cur[3 * c.W + j];
+        sdu[j] = c2(-z.x, -z.y);  // omega_z(0,0) = -DY1 U
+        sdw[j] = x;               // omega_x(0,0) = DY1 W
+        aU[j] = caddc(c.cur[j], mvx[j]);
+        aW[j] = caddc(c.cur[c.W + j], mvz[j]);
+    }
+    friction_adj(c, sdu, sdw, afx, afz, aU, aW);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        for (int f = 0; f < 4; ++f) c.cur[(size_t)f * c.W + j] = c2(0, 0);
+        mvx[j] = mvz[j] = c2(0, 0);
+    }
+    if (threadIdx.x == 0) c.fric[ns] = c.fric[3 + ns] = 0.0;
+    for (int j = threadIdx.x; j < 64 * NQ; j += blockDim.x) {
+        const bool in = j < N;
+        S[0][j] = in ? aU[j].x : 0.0;
+        S[1][j] = in ? aU[j].y : 0.0;
+        S[2][j] = in ? aW[j].x : 0.0;
+        S[3][j] = in ? aW[j].y : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // A^T y = b with A^T = L' D U' P: substitution in natural order, then y[perm[k]] = w[k]
+        const int *Po = P + ((size_t)mf * 2 + 1) * N;
+        double t[4][NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) t[r][q] = S[r][lane + 64 * q];
+        lu_substitute<NQ>(G + ((size_t)mf * 2 + 1) * N * N, N, lane, t);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int k = lane + 64 * q;
+            if (k < N) {
+                int r = Po[k];
+                if ((unsigned)r >= (unsigned)N) r = 0;
+                const bool bc = r == 0 || r == N - 1;
+                sdu[r] = bc ? c2(0, 0) : c2(t[0][q], t[1][q]);
+                sdw[r] = bc ? c2(0, 0) : c2(t[2][q], t[3][q]);
+            }
+        }
+    }
+    __syncthreads();
+    // mu_U = sdu, mu_W = sdw: into the adjoints of the Lamb, friction and velocity histories
+    const double BE[3] = {BE0, BE1, BE2}, AL[3] = {AL0, AL1, AL2};
+    double ru = 0.0, rw = 0.0;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        const double2 mu = sdu[j], mw = sdw[j];
+        ru += mu.x;
+        rw += mw.x;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            const int sl = sp.s[l];
+            double2 *lmx = c.means + (size_t)(M_LMBX * 3 + sl) * N, *lmz = c.means + (size_t)(M_LMBZ * 3 + sl) * N;
+            double2 *vx = c.means + (size_t)(M_VELX * 3 + sl) * N, *vz = c.means + (size_t)(M_VELZ * 3 + sl) * N;
+            lmx[j] = caddc(lmx[j], cscale(mu, BE[l]));
+            lmz[j] = caddc(lmz[j], cscale(mw, BE[l]));
+            vx[j] = caddc(vx[j], cdiv(cscale(mu, AL[l]), dt));
+            vz[j] = caddc(vz[j], cdiv(cscale(mw, AL[l]), dt));
+        }
+    }
+    red[0][threadIdx.x] = ru;
+    red[1][threadIdx.x] = rw;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int t2 = 0; t2 < (int)blockDim.x; ++t2) {
+            a += red[0][t2];
+            b += red[1][t2];
+        }
+        for (int l = 0; l < 3; ++l) {
+            c.fric[sp.s[l]] += BE[l] * a;
+            c.fric[3 + sp.s[l]] += BE[l] * b;
+        }
+    }
+}
+
+// transposes of the initial post-processing (ring 0). Order of use: init_mean,
+// init_history, nonlinear (slot 1), Lamb, init_d2v, vor_from_vel.
+__global__ __launch_bounds__(256) void k_init_mean_adj(Chan c) {
+    __shared__ double2 sdu[kMaxNY], sdw[kMaxNY], aU[kMaxNY], aW[kMaxNY];
+    const int N = c.NY;
+    double2 *lx = reinterpret_cast<double2 *>(c.lmb), *lz = lx + 2 * c.W;
+    double afx = 0.0, afz = 0.0;
+    for (int l = 0; l < 3; ++l) {
+        afx += c.fric[l];
+        afz += c.fric[3 + l];
+    }
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        double2 a = c2(0, 0), b = c2(0, 0), u = c2(0, 0), w = c2(0, 0);
+        for (int s = 0; s < 3; ++s) {
+            a = caddc(a, c.means[(size_t)(M_LMBX * 3 + s) * N + j]);
+            b = caddc(b, c.means[(size_t)(M_LMBZ * 3 + s) * N + j]);
+            u = caddc(u, c.means[(size_t)(M_VELX * 3 + s) * N + j]);
+            w = caddc(w, c.means[(size_t)(M_VELZ * 3 + s) * N + j]);
+        }
+        lx[j] = caddc(lx[j], a);
+        lz[j] = caddc(lz[j], b);
+        aU[j] = u;
+        aW[j] = w;
+        sdu[j] = sdw[j] = c2(0, 0);
+    }
+    friction_adj(c, sdu, sdw, afx, afz, aU, aW);
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+        c.cur[j] = caddc(c.cur[j], aU[j]);
+        c.cur[c.W + j] = caddc(c.cur[c.W + j], aW[j]);
+        for (int f = 0; f < 12; ++f) c.means[(size_t)f * N + j] = c2(0, 0);
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) c.fric[threadIdx.x] = 0.0;
+}
+
+__global__ void k_init_history_adj(Chan c) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < c.W; e += (size_t)gridDim.x * blockDim.x) {
+        double2 v = c2(0, 0), g = c2(0, 0), hv = c2(0, 0), hg = c2(0, 0), dv = c2(0, 0);
+        for (int s = 0; s < 3; ++s) {
+            v = caddc(v, hist(c, H_V, s)[e]);
+            g = caddc(g, hist(c, H_G, s)[e]);
+            hv = caddc(hv, hist(c, H_HV, s)[e]);
+            hg = caddc(hg, hist(c, H_HG, s)[e]);
+            dv = caddc(dv, c.d2v[(size_t)s * c.W + e]);
+            hist(c, H_V, s)[e] = hist(c, H_G, s)[e] = hist(c, H_HV, s)[e] = hist(c, H_HG, s)[e] = c2(0, 0);
+            c.d2v[(size_t)s * c.W + e] = c2(0, 0);
+        }
+        hist(c, H_V, 0)[e] = v;
+        hist(c, H_G, 0)[e] = g;
+        hist(c, H_HV, 1)[e] = hv;
+        hist(c, H_HG, 1)[e] = hg;
+        c.d2v[e] = dv;
+    }
+}
+
+// transposes of k_init_d2v and k_vor_from_vel: the gradient with respect to the
+// initial u, v, w, written to gu, gv, gw (W each)
+template <int TP>
+__global__ __launch_bounds__(256) void k_init_vel_adj(Chan c, double2 *gu, double2 *gv, double2 *gw) {
+    __shared__ double2 sz[TP][kMaxNY], sx[TP][kMaxNY], sd[TP][kMaxNY];
+    const int N = c.NY;
+    const double2 *au = c.cur, *aw = c.cur + c.W, *ax = c.cur + 2 * c.W, *az = c.cur + 3 * c.W;
+    const double2 *av = hist(c, H_V, 0), *ay = hist(c, H_G, 0);
+    const int p0 = blockIdx.x * TP;
+    for (int e = threadIdx.x; e < TP * N; e += blockDim.x) {
+        const int pp = e / N, j = e % N, p = p0 + pp;
+        const bool in = p < c.npencil;
+        const size_t o = (size_t)p * N + j;
+        sz[pp][j] = in ? az[o] : c2(0, 0);
+        sx[pp][j] = in ? ax[o] : c2(0, 0);
+        sd[pp][j] = in ? c.d2v[o] : c2(0, 0);
+    }
+    __syncthreads();
+    const int row = threadIdx.x;
+    if (row >= N) return;
+    double2 dz[TP], dx[TP], dd[TP];
+#pragma unroll
+    for (int pp = 0; pp < TP; ++pp) dz[pp] = dx[pp] = dd[pp] = c2(0, 0);
+    matvec_T<TP>(c.DY1, sz, N, row, dz);
+    matvec_T<TP>(c.DY1, sx, N, row, dx);
+    matvec_T<TP>(c.DY2, sd, N, row, dd);
+#pragma unroll
+    for (int pp = 0; pp < TP; ++pp) {
+        const int p = p0 + pp;
+        if (p >= c.npencil) break;
+        const int k = p / c.NXH, i = p % c.NXH;
+        const double kx = c.kx[i], kz = c.kz[k];
+        const size_t o = (size_t)p * N + row;
+        const double2 Y = ay[o], X = sx[pp][row], Z = sz[pp][row];
+        // omega_x = DY1 w - i kz v; omega_y = i kz u - i kx w; omega_z = i kx v - DY1 u
+        gu[o] = csubc(caddc(au[o], imul(-kz, Y)), dz[pp]);
+        gw[o] = caddc(caddc(aw[o], imul(kx, Y)), dx[pp]);
+        gv[o] = caddc(caddc(caddc(av[o], dd[pp]), imul(kz, X)), imul(-kx, Z));
+    }
+}
+
+// dst[f] += src[f] for nf fields of n complex values; zeroes src when asked
+struct AccTable {
+    double2 *dst[6];
+    double2 *src[6];
+};
+__global__ void k_accumulate(AccTable t, int nf, size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        for (int f = 0; f < nf; ++f) t.dst[f][e] = caddc(t.dst[f][e], t.src[f][e]);
+}
+
+// ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
 constexpr int kTP = 8;
@@ -483,6 +835,36 @@ void launch_pressure_rhs(const Chan &c, const double2 *dv, double2 *R, hipStream
         hipLaunchKernelGGL((k_pressure_rhs<kTP, double2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, dv, R);
     else
         hipLaunchKernelGGL((k_pressure_rhs<kTP, float2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, dv, R);
+}
+
+void launch_horizontal_adj(const Chan &c, const StepPtrs &sp, hipStream_t s) {
+    hipLaunchKernelGGL(k_horizontal_adj<kTPH>, dim3((c.npencil + kTPH - 1) / kTPH), dim3(256), 0, s, c, sp);
+}
+
+void launch_nonlinear_adj(const Chan &c, const StepPtrs &sp, hipStream_t s) {
+    hipLaunchKernelGGL(k_nonlinear_adj<kTP>, dim3(blocks_tp(c)), dim3(256), 0, s, c, sp);
+}
+
+void launch_mean_adj(const Chan &c, const StepPtrs &sp, const double *G, const int *P, int mf, double seed, hipStream_t s) {
+    OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_mean_adj<NQ>, dim3(1), dim3(256), 0, s, c, sp, G, P, mf, seed));
+}
+
+void launch_init_adj_mean_history(const Chan &c, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_mean_adj, dim3(1), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(k_init_history_adj, dim3(1024), dim3(256), 0, s, c);
+}
+
+void launch_init_vel_adj(const Chan &c, double2 *gu, double2 *gv, double2 *gw, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_vel_adj<kTPH>, dim3((c.npencil + kTPH - 1) / kTPH), dim3(256), 0, s, c, gu, gv, gw);
+}
+
+void launch_accumulate(double2 *const *dst, double2 *const *src, int nf, size_t n, hipStream_t s) {
+    AccTable t{};
+    for (int f = 0; f < nf; ++f) {
+        t.dst[f] = dst[f];
+        t.src[f] = src[f];
+    }
+    hipLaunchKernelGGL(k_accumulate, dim3(1024), dim3(256), 0, s, t, nf, n);
 }
 
 void launch_vor_from_vel(const Chan &c, hipStream_t s) {

@@ -37,6 +37,7 @@ EXPORTS = [
     "owt_time_steps", "owt_time_kernel", "owt_get_factor", "owt_save",
     "owt_ensemble_create", "owt_ensemble_destroy", "owt_ensemble_last_error", "owt_ensemble_member",
     "owt_ensemble_init_fields", "owt_ensemble_step", "owt_ensemble_time_steps",
+    "owt_tangent_init", "owt_tangent_handle", "owt_tangent_step", "owt_adjoint_run",
 ]
 
 
@@ -97,6 +98,11 @@ def lib():
         L.owt_time_steps.argtypes = [P, I, P]
         L.owt_time_kernel.argtypes = [P, I, I, P]
         L.owt_get_factor.argtypes = [P, I, I, P, P]
+        L.owt_tangent_init.argtypes = [P, P, P, P]
+        L.owt_tangent_handle.argtypes = [P]
+        L.owt_tangent_handle.restype = P
+        L.owt_tangent_step.argtypes = [P, I]
+        L.owt_adjoint_run.argtypes = [P, I, I, I, I, I, P, P, P, P]
         L.owt_ensemble_create.argtypes = [I, P, ctypes.POINTER(_Opts), PP]
         L.owt_ensemble_destroy.argtypes = [P]
         L.owt_ensemble_destroy.restype = None
@@ -259,6 +265,23 @@ class Channel:
         ms = ctypes.c_double()
         self._check(lib().owt_time_steps(self._h, int(n), ctypes.byref(ms)))
         return ms.value
+
+    def tangent_init(self, du, dv, dw):
+        """Attach the tangent (forward-mode linearization) started from the
+        direction (du, dv, dw) of the initial velocity; returns it as a Channel."""
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128).reshape(self.wave_shape) for a in (du, dv, dw)]
+        self._check(lib().owt_tangent_init(self._h, *[_p(a) for a in arrs]))
+        return Channel(_member=(ctypes.c_void_p(lib().owt_tangent_handle(self._h)), self))
+
+    def adjoint_run(self, K, ckpt_every=5, first=1, last=None):
+        """J = mean friction force over steps first..last of K steps from the
+        initial field, and dJ/d(u, v, w) of that field by the discrete adjoint."""
+        last = K if last is None else last
+        J = ctypes.c_double()
+        g = [np.empty(self.wave_shape, dtype=np.complex128) for _ in range(3)]
+        self._check(lib().owt_adjoint_run(self._h, int(K), int(ckpt_every), 0, int(first), int(last), ctypes.byref(J),
+                                          *[_p(a) for a in g]))
+        return J.value, g
 
     def time_kernel(self, name, reps):
         ms = ctypes.c_double()

@@ -1,0 +1,113 @@
+"""Tangent and discrete adjoint of the time step (FP64), through the C ABI.
+
+Nothing in the reference differentiates the step, so the checks are the ones the
+mathematics fixes:
+  tangent   central finite differences of the forward step (itself checked
+            against the oracle): fields after K steps and the objective
+  adjoint   the adjoint identity <grad J, d> = dJ[d] against the tangent, which
+            holds to roundoff for an exact transpose, and central finite
+            differences of J; checkpoint spacing does not change the gradient
+The objective is the mean friction force fx_fric (mean wall shear) over steps
+first..last, the gradient taken with respect to the initial u, v, w.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RE180 = dict(nu=1 / 2800, dt=1e-3, dkx=0.5, dkz=1.0, alpha_mesh=1.5)
+
+
+def params(NX, NY, NZ):
+    return dict(RE180, NX=NX, NY=NY, NZ=NZ, precision=0, method=1)
+
+
+def directions(shape, seeds):
+    out = []
+    for s in seeds:
+        rng = np.random.default_rng(s)
+        out.append([0.1 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) for _ in range(3)])
+    return out
+
+
+def run_J(ch, x, K, first, last):
+    ch.set_velocity(*x)
+    ch.step(K)
+    return ch.diag()[first:last + 1, 0].mean(), [ch.field(n) for n in ("u", "v", "w")]
+
+
+@pytest.fixture(scope="module")
+def base(owt_mod):
+    p = params(16, 33, 16)
+    print("[gpu] adjoint base 16x33x16", flush=True)
+    ch = owt_mod.Channel(params=p)
+    ch.init_fields(2, seed=0, amp=0.3)
+    x0 = [ch.field(n) for n in ("u", "v", "w")]
+    yield ch, x0
+    ch.close()
+
+
+def test_tangent_matches_finite_differences(base):
+    ch, x0 = base
+    K, first, last, eps = 6, 2, 6, 1e-6
+    d = directions(x0[0].shape, [11])[0]
+    Jp, Fp = run_J(ch, [a + eps * b for a, b in zip(x0, d)], K, first, last)
+    Jm, Fm = run_J(ch, [a - eps * b for a, b in zip(x0, d)], K, first, last)
+    ch.set_velocity(*x0)
+    t = ch.tangent_init(*d)
+    ch.step(K)
+    for n, name in enumerate(("u", "v", "w")):
+        fd = (Fp[n] - Fm[n]) / (2 * eps)
+        err = np.linalg.norm(t.field(name) - fd) / np.linalg.norm(fd)
+        print(f"[tangent] {name} vs fd {err:.2e}", flush=True)
+        assert err < 1e-6, (name, err)
+    dJ_t = t.diag()[first:last + 1, 0].mean()
+    dJ_fd = (Jp - Jm) / (2 * eps)
+    assert abs(dJ_t - dJ_fd) < 1e-6 * abs(dJ_fd), (dJ_t, dJ_fd)
+
+
+def test_adjoint_identity_and_finite_differences(base):
+    ch, x0 = base
+    K, first, last, eps = 6, 2, 6, 1e-6
+    ch.set_velocity(*x0)
+    J, g = ch.adjoint_run(K, 2, first, last)
+    J0, _ = run_J(ch, x0, K, first, last)
+    assert abs(J - J0) <= 1e-14 * abs(J0)
+    for d in directions(x0[0].shape, [21, 22, 23]):
+        dJ_a = sum(np.vdot(gg, dd).real for gg, dd in zip(g, d))
+        ch.set_velocity(*x0)
+        t = ch.tangent_init(*d)
+        ch.step(K)
+        dJ_t = t.diag()[first:last + 1, 0].mean()
+        Jp, _ = run_J(ch, [a + eps * b for a, b in zip(x0, d)], K, first, last)
+        Jm, _ = run_J(ch, [a - eps * b for a, b in zip(x0, d)], K, first, last)
+        dJ_fd = (Jp - Jm) / (2 * eps)
+        print(f"[adjoint] <g,d>={dJ_a:.15e} tangent={dJ_t:.15e} fd={dJ_fd:.15e}", flush=True)
+        assert abs(dJ_a - dJ_t) <= 1e-10 * abs(dJ_t)
+        assert abs(dJ_a - dJ_fd) <= 1e-6 * abs(dJ_fd)
+
+
+def test_adjoint_checkpoint_spacing_does_not_change_gradient(base):
+    ch, x0 = base
+    ch.set_velocity(*x0)
+    J1, g1 = ch.adjoint_run(7, 1, 1, 7)
+    ch.set_velocity(*x0)
+    J7, g7 = ch.adjoint_run(7, 7, 1, 7)
+    ch.set_velocity(*x0)
+    J3, g3 = ch.adjoint_run(7, 3, 1, 7)
+    assert J1 == J7 == J3
+    for a, b, c in zip(g1, g7, g3):
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+    assert ch.steps_done() == 7
+
+
+def test_adjoint_rejects_mixed_precision_and_bad_ranges(owt_mod):
+    ch = owt_mod.Channel(params=dict(params(16, 17, 16), precision=1))
+    ch.init_fields(2, seed=0, amp=0.15)
+    with pytest.raises(owt_mod.OwtError):
+        ch.adjoint_run(3, 1, 1, 3)
+    ch.set_precision(0)
+    ch.init_fields(2, seed=0, amp=0.15)
+    with pytest.raises(owt_mod.OwtError):
+        ch.adjoint_run(3, 1, 2, 4)
+    ch.close()
