@@ -111,3 +111,38 @@ def test_adjoint_rejects_mixed_precision_and_bad_ranges(owt_mod):
     with pytest.raises(owt_mod.OwtError):
         ch.adjoint_run(3, 1, 2, 4)
     ch.close()
+
+
+def test_torch_autograd_function_harness(base):
+    """owt_adjoint_run behind torch.autograd.Function: torch's backward returns the
+    adjoint gradient (torch's convention for a real loss of complex inputs is
+    d/dRe + i d/dIm, the same as the library's), and a directional derivative
+    through the Function matches central differences of the Function itself."""
+    torch = pytest.importorskip("torch")
+    ch, x0 = base
+    K, first, last = 6, 2, 6
+
+    class MeanWallShear(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, u, v, w):
+            ch.set_velocity(u.detach().numpy(), v.detach().numpy(), w.detach().numpy())
+            J, g = ch.adjoint_run(K, 3, first, last)
+            ctx.save_for_backward(*[torch.from_numpy(a) for a in g])
+            return torch.tensor(J, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            return tuple(gJ * gi for gi in ctx.saved_tensors)
+
+    xs = [torch.tensor(a, requires_grad=True) for a in x0]
+    J = MeanWallShear.apply(*xs)
+    J.backward()
+    d = directions(x0[0].shape, [31])[0]
+    dJ = sum(float((torch.conj(x.grad) * torch.from_numpy(dd)).real.sum()) for x, dd in zip(xs, d))
+    eps = 1e-6
+    with torch.no_grad():
+        Jp = MeanWallShear.apply(*[torch.from_numpy(a + eps * b) for a, b in zip(x0, d)])
+        Jm = MeanWallShear.apply(*[torch.from_numpy(a - eps * b) for a, b in zip(x0, d)])
+    fd = float(Jp - Jm) / (2 * eps)
+    print(f"[torch] dJ={dJ:.15e} fd={fd:.15e}", flush=True)
+    assert abs(dJ - fd) <= 1e-6 * abs(fd)
