@@ -271,6 +271,7 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=100,
                     help="timed steps of the 32-member ensemble workload (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient check")
+    ap.add_argument("--no-autotune", action="store_true", help="time the default launch shapes")
     ap.add_argument("--accuracy-steps", type=int, default=100,
                     help="steps of the mixed-vs-FP64 accuracy comparison (0 skips it)")
     ap.add_argument("--mode", choices=("full", "profile", "copy"), default="full",
@@ -309,6 +310,10 @@ def main():
             ch.step(warm)
         return ch.time_steps(steps) / steps
 
+    tuned = None
+    if not args.no_autotune:
+        print("[bench] autotune", file=sys.stderr, flush=True)
+        tuned = ch.autotune(5)
     print("[bench] mixed batched", file=sys.stderr, flush=True)
     t_mixed = timed(K, Wm)
     if args.mode == "profile":
@@ -320,6 +325,8 @@ def main():
     t_factor_once = ch.time_kernel("factor", 1)
     print("[bench] fp64 batched", file=sys.stderr, flush=True)
     ch.set_precision(owt.FP64)
+    if tuned:
+        ch.autotune(5)
     t_fp64 = timed(K, 1)
     kern64 = {name: ch.time_kernel(name, 3) for name in stages}
     print("[bench] fp64 plain", file=sys.stderr, flush=True)
@@ -378,6 +385,7 @@ def main():
         "kernels_ms_mixed": {k: round(v, 4) for k, v in kern.items()},
         "kernels_ms_fp64": {k: round(v, 4) for k, v in kern64.items()},
         "roofline": roof,
+        "autotune": {k: v["choice"] for k, v in tuned.items()} if tuned else "off (default shapes)",
     }
     if args.ensemble_steps > 0:
         print("[bench] ensemble", file=sys.stderr, flush=True)

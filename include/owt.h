@@ -132,6 +132,19 @@ int owt_get_matrix(owt_handle *h, int which, double *out);
  * factors for validation. */
 int owt_get_factor(owt_handle *h, int sys, int op, double *lu, int *perm);
 
+/* Launch-shape autotuner (no reference counterpart): times every candidate
+ * shape of each tunable launch on the handle's own data (reps launches each),
+ * keeps the fastest and recaptures the step graphs. Knobs (OWT_TUNE_COUNT ints,
+ * candidate index 0..2, 0 = default): solve jobs per block, pencils per block
+ * of the horizontal and the nonlinear kernels, tile shape of the z inverse, x
+ * and z forward transforms. Every candidate computes the same values in the
+ * same order, so results do not depend on the choice. report (may be NULL)
+ * receives a JSON summary of the timings. */
+enum { OWT_TUNE_COUNT = 6 };
+int owt_autotune(owt_handle *h, int reps, char *report, int report_len);
+int owt_get_tune(owt_handle *h, int *knobs);
+int owt_set_tune(owt_handle *h, const int *knobs);
+
 /* Differentiation (FP64; no reference counterpart).
  *
  * Tangent: owt_tangent_init attaches to h a tangent state, the forward-mode

@@ -141,6 +141,26 @@ void owt_destroy(owt_handle *h) {
     }
 }
 
+int owt_autotune(owt_handle *h, int reps, char *report, int report_len) {
+    return guard(h, [&] {
+        const std::string r = h->ch->autotune(reps);
+        if (report && report_len > 0) {
+            std::strncpy(report, r.c_str(), (size_t)report_len - 1);
+            report[report_len - 1] = 0;
+        }
+    });
+}
+
+int owt_get_tune(owt_handle *h, int *knobs) {
+    if (!knobs) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->get_tune(knobs); });
+}
+
+int owt_set_tune(owt_handle *h, const int *knobs) {
+    if (!knobs) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->set_tune(knobs); });
+}
+
 int owt_tangent_init(owt_handle *h, const double *du, const double *dv, const double *dw) {
     if (!du || !dv || !dw) return OWT_E_ARG;
     return guard(h, [&] { h->ch->tangent_init(du, dv, dw); });

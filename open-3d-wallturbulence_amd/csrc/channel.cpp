@@ -1079,4 +1079,74 @@ double Channel::adjoint_run(int K, int ckpt, int first, int last, double *gu, do
     return J;
 }
 
+// ---------------------------------------------------------------------------
+// launch-shape autotuner
+// ---------------------------------------------------------------------------
+void Channel::get_tune(int *knobs) const {
+    for (int k = 0; k < T_COUNT; ++k) knobs[k] = c_.tune[k];
+}
+
+void Channel::set_tune(const int *knobs) {
+    for (int k = 0; k < T_COUNT; ++k)
+        if (knobs[k] < 0 || knobs[k] >= kTuneCandidates) throw Error(2, "tune candidate out of range");
+    sync();
+    for (int k = 0; k < T_COUNT; ++k) c_.tune[k] = knobs[k];
+    drop_graph();
+}
+
+std::string Channel::autotune(int reps) {
+    if (reps < 1) throw Error(2, "autotune: reps must be positive");
+    static const char *names[T_COUNT] = {"solve_jobs_per_block", "horizontal_pencils_per_block", "nonlinear_pencils_per_block",
+                                         "z_inverse_tile", "x_tile", "z_forward_tile"};
+    static const char *shapes[T_COUNT][kTuneCandidates] = {
+        {"2", "1", "4"}, {"4", "2", "8"}, {"8", "4", "16"}, {"default", "alt1", "alt2"}, {"default", "alt1", "alt2"},
+        {"default", "alt1", "alt2"}};
+    // stage kernel of each knob (enqueue_kernel ids)
+    static const int kernel[T_COUNT] = {0, 1, 6, 3, 4, 5};
+    prepare_step();
+    sync();
+    hipEvent_t a, b;
+    OWT_HIP(hipEventCreate(&a));
+    OWT_HIP(hipEventCreate(&b));
+    std::string rep = "{";
+    for (int k = 0; k < T_COUNT; ++k) {
+        // the length class >= 577 has two x/z candidates
+        int ncand = kTuneCandidates;
+        if ((k == T_Z_INVERSE_TILE || k == T_Z_FORWARD_TILE) && c_.NZ2 > 576) ncand = 2;
+        if (k == T_X_TILE && c_.NX2 > 576) ncand = 2;
+        if (k == T_SOLVE_JOBS && P_.method == 0) ncand = 1;  // the plain solve has one shape
+        double best = 1e300;
+        int arg = 0;
+        std::string times;
+        for (int cand = 0; cand < ncand; ++cand) {
+            c_.tune[k] = cand;
+            enqueue_kernel(kernel[k]);
+            OWT_HIP(hipEventRecord(a, s_));
+            for (int r = 0; r < reps; ++r) enqueue_kernel(kernel[k]);
+            OWT_HIP(hipEventRecord(b, s_));
+            OWT_HIP(hipEventSynchronize(b));
+            float t = 0;
+            OWT_HIP(hipEventElapsedTime(&t, a, b));
+            const double ms = t / reps;
+            char buf[64];
+            std::snprintf(buf, sizeof buf, "%s\"%s\": %.5f", cand ? ", " : "", shapes[k][cand], ms);
+            times += buf;
+            if (ms < best) {
+                best = ms;
+                arg = cand;
+            }
+        }
+        c_.tune[k] = arg;
+        char buf[160];
+        std::snprintf(buf, sizeof buf, "%s\"%s\": {\"choice\": %d, \"ms\": {", k ? ", " : "", names[k], arg);
+        rep += buf + times + "}}";
+    }
+    rep += "}";
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    check_errors();
+    drop_graph();
+    return rep;
+}
+
 }  // namespace owt

@@ -55,6 +55,12 @@ class Channel {
     // 0 solve, 1 horizontal, 2 mean, 3 z inverse, 4 x phase, 5 z forward,
     // 6 nonlinear, 7 factorization
     void enqueue_kernel(int id);
+    // Launch-shape autotuner: times every candidate of every knob (TuneKnob) on
+    // this channel's data, keeps the fastest, and recaptures the step graphs.
+    // Returns a JSON report; get/set_tune read or force the choices.
+    std::string autotune(int reps);
+    void get_tune(int *knobs) const;
+    void set_tune(const int *knobs);
     // stored factor of one +-kz system (batched factor set)
     void get_factor(int sys, int op, double *lu, int *perm);
     void sync();

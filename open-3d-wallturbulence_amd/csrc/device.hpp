@@ -34,6 +34,11 @@ enum Ctrl { C_RING = 0, C_STEP = 1, C_ERR = 2 };  // C_RING is written only into
 // error bits
 enum ErrBits { E_SINGULAR = 1 };
 
+// Launch shapes (candidate indices, 0 = the default shape) chosen by the
+// autotuner; every candidate computes the same values in the same order.
+enum TuneKnob { T_SOLVE_JOBS = 0, T_HORIZONTAL_TP, T_NONLINEAR_TP, T_Z_INVERSE_TILE, T_X_TILE, T_Z_FORWARD_TILE, T_COUNT };
+constexpr int kTuneCandidates = 3;
+
 struct Chan {
     int NX, NY, NZ, NXH, NZH, NX2, NZ2;
     int npencil;  // NZ * NXH
@@ -60,6 +65,7 @@ struct Chan {
     const double2 *twx, *twz;  // e^{-2 pi i m / L}, fp64
     const float2 *twxf, *twzf; // same, fp32
     FFTPlan px, pz;
+    int tune[T_COUNT];
 };
 
 // History slots of one step: level 0 is the current field, levels 1 and 2 the

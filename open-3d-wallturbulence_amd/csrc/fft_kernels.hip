@@ -649,6 +649,29 @@ size_t smem_bytes() {
         else if ((L) <= 576) { constexpr int TJ = 4, TL = 64, LMAX = 576; BODY; }    \
         else { constexpr int TJ = 2, TL = 128, LMAX = 1152; BODY; }                  \
     } while (0)
+// Tuned variants for the step's three transform kernels: candidate SEL of the
+// length class (0 the shape above; every candidate has 256 threads and fits 64 KB
+// of LDS in FP64)
+#define OWT_TUNED_DISPATCH(L, SEL, BODY)                                                            \
+    do {                                                                                            \
+        const int sel_ = (SEL);                                                                     \
+        if ((L) <= 96) {                                                                            \
+            if (sel_ == 1) { constexpr int TJ = 8, TL = 32, LMAX = 96; BODY; }                      \
+            else if (sel_ == 2) { constexpr int TJ = 32, TL = 8, LMAX = 96; BODY; }                 \
+            else { constexpr int TJ = 16, TL = 16, LMAX = 96; BODY; }                               \
+        } else if ((L) <= 192) {                                                                    \
+            if (sel_ == 1) { constexpr int TJ = 16, TL = 16, LMAX = 192; BODY; }                    \
+            else if (sel_ == 2) { constexpr int TJ = 4, TL = 64, LMAX = 192; BODY; }                \
+            else { constexpr int TJ = 8, TL = 32, LMAX = 192; BODY; }                               \
+        } else if ((L) <= 576) {                                                                    \
+            if (sel_ == 1) { constexpr int TJ = 2, TL = 128, LMAX = 576; BODY; }                    \
+            else if (sel_ == 2) { constexpr int TJ = 1, TL = 256, LMAX = 576; BODY; }               \
+            else { constexpr int TJ = 4, TL = 64, LMAX = 576; BODY; }                               \
+        } else {                                                                                    \
+            if (sel_ == 1) { constexpr int TJ = 1, TL = 256, LMAX = 1152; BODY; }                   \
+            else { constexpr int TJ = 2, TL = 128, LMAX = 1152; BODY; }                             \
+        }                                                                                           \
+    } while (0)
 // the same shapes for the x phase (spelled out: a forwarding macro would expand
 // the launch in BODY before substitution)
 #define OWT_X_DISPATCH(L, T, BODY)                                                   \
@@ -665,13 +688,13 @@ void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out
     PtrTable t{};
     for (int f = 0; f < nf; ++f) t.p[f] = in[f];
     if (c.precision == 0)
-        OWT_Z_DISPATCH(c.NZ2, double, {
+        OWT_TUNED_DISPATCH(c.NZ2, c.tune[T_Z_INVERSE_TILE], {
             dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * nf));
             hipLaunchKernelGGL((k_z_inverse<double, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c, t,
                                (double2 *)out, nf, 1.0, 0);
         });
     else
-        OWT_Z_DISPATCH(c.NZ2, float, {
+        OWT_TUNED_DISPATCH(c.NZ2, c.tune[T_Z_INVERSE_TILE], {
             dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * nf));
             hipLaunchKernelGGL((k_z_inverse<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c, t,
                                (float2 *)out, nf, 1.0f, 0);
@@ -680,13 +703,13 @@ void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out
 
 void launch_x_lamb(const Chan &c, hipStream_t s) {
     if (c.precision == 0)
-        OWT_X_DISPATCH(c.NX2, double, {
+        OWT_TUNED_DISPATCH(c.NX2, c.tune[T_X_TILE], {
             dim3 g(xcd_grid(c.NZ2 * ((c.NY + TJ - 1) / TJ)));
             hipLaunchKernelGGL((k_x_lamb<double, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s, c,
                                (const double2 *)c.zbuf, (double2 *)c.xbuf);
         });
     else
-        OWT_X_DISPATCH(c.NX2, float, {
+        OWT_TUNED_DISPATCH(c.NX2, c.tune[T_X_TILE], {
             dim3 g(xcd_grid(c.NZ2 * ((c.NY + TJ - 1) / TJ)));
             hipLaunchKernelGGL((k_x_lamb<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
                                (const float2 *)c.zbuf, (float2 *)c.xbuf);
@@ -695,13 +718,13 @@ void launch_x_lamb(const Chan &c, hipStream_t s) {
 
 void launch_z_forward(const Chan &c, hipStream_t s) {
     if (c.precision == 0)
-        OWT_Z_DISPATCH(c.NZ2, double, {
+        OWT_TUNED_DISPATCH(c.NZ2, c.tune[T_Z_FORWARD_TILE], {
             dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * 3));
             hipLaunchKernelGGL((k_z_forward<double, TJ, TL, LMAX, double2>), g, dim3(TJ * TL), (smem_bytes<double, TJ, TL, LMAX>()), s,
                                c, (const double2 *)c.xbuf, (double2 *)c.lmb, 3, (double)c.NZ2, 1);
         });
     else
-        OWT_Z_DISPATCH(c.NZ2, float, {
+        OWT_TUNED_DISPATCH(c.NZ2, c.tune[T_Z_FORWARD_TILE], {
             dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * 3));
             hipLaunchKernelGGL((k_z_forward<float, TJ, TL, LMAX, float2>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
                                (const float2 *)c.xbuf, (float2 *)c.lmb, 3, (float)c.NZ2, 1);

@@ -238,12 +238,12 @@ __device__ __forceinline__ void job_pencils(const Chan &c, int4 jb, int (&pc)[4]
 // row that its position in the pivot order names (dlaswp folded into the
 // loads), so the solve needs no data exchange before the substitution.
 // ---------------------------------------------------------------------------
-template <int NQ, bool PLAIN>
-__global__ __launch_bounds__(256) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P, const int4 *jobs, int njobs) {
+template <int NQ, bool PLAIN, int JPB>
+__global__ __launch_bounds__(128 * JPB) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P, const int4 *jobs, int njobs) {
     constexpr int NP = PLAIN ? 1 : 4;  // pencils per job
     constexpr int NR = 2 * NP;         // real right-hand sides
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int job = blockIdx.x * 2 + (w >> 1), op = w & 1;
+    const int job = blockIdx.x * JPB + (w >> 1), op = w & 1;
     const int N = c.NY;
     if (job >= njobs) return;
     int fac, pc[NP], kz, ix;
@@ -457,11 +457,17 @@ void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, 
 
 void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int4 *jobs, int njobs, bool plain,
                   hipStream_t s) {
-    const int blocks = (njobs + 1) / 2;
-    if (plain)
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, true>), dim3(blocks), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
-    else
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false>), dim3(blocks), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
+    // tuned jobs per block (two waves each): candidates 2, 1, 4
+    const int sel = c.tune[T_SOLVE_JOBS];
+    if (plain) {
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, true, 2>), dim3((njobs + 1) / 2), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
+    } else if (sel == 1) {
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 1>), dim3(njobs), dim3(128), 0, s, c, sp, F, P, jobs, njobs));
+    } else if (sel == 2) {
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 4>), dim3((njobs + 3) / 4), dim3(512), 0, s, c, sp, F, P, jobs, njobs));
+    } else {
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 2>), dim3((njobs + 1) / 2), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
+    }
 }
 
 

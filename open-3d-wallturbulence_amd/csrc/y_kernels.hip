@@ -815,8 +815,20 @@ constexpr int kTPH = 4;
 
 static int blocks_tp(const Chan &c) { return (c.npencil + kTP - 1) / kTP; }
 
+template <typename LC>
+static void launch_nonlinear_tp(const Chan &c, const StepPtrs &sp, hipStream_t s) {
+    const int sel = c.tune[T_NONLINEAR_TP];
+    if (sel == 1) hipLaunchKernelGGL((k_nonlinear<4, LC>), dim3((c.npencil + 3) / 4), dim3(256), 0, s, c, sp);
+    else if (sel == 2) hipLaunchKernelGGL((k_nonlinear<16, LC>), dim3((c.npencil + 15) / 16), dim3(256), 0, s, c, sp);
+    else hipLaunchKernelGGL((k_nonlinear<kTP, LC>), dim3(blocks_tp(c)), dim3(256), 0, s, c, sp);
+}
+
+// tuned pencils per block: candidates kTPH, 2, 8 (horizontal) and kTP, 4, 16 (nonlinear)
 void launch_horizontal(const Chan &c, const StepPtrs &sp, hipStream_t s) {
-    hipLaunchKernelGGL(k_horizontal<kTPH>, dim3((c.npencil + kTPH - 1) / kTPH), dim3(256), 0, s, c, sp);
+    const int sel = c.tune[T_HORIZONTAL_TP];
+    if (sel == 1) hipLaunchKernelGGL(k_horizontal<2>, dim3((c.npencil + 1) / 2), dim3(256), 0, s, c, sp);
+    else if (sel == 2) hipLaunchKernelGGL(k_horizontal<8>, dim3((c.npencil + 7) / 8), dim3(256), 0, s, c, sp);
+    else hipLaunchKernelGGL(k_horizontal<kTPH>, dim3((c.npencil + kTPH - 1) / kTPH), dim3(256), 0, s, c, sp);
 }
 
 void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *P, int mf, hipStream_t s) {
@@ -825,9 +837,9 @@ void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *
 
 void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s) {
     if (c.precision == 0)
-        hipLaunchKernelGGL((k_nonlinear<kTP, double2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, sp);
+        launch_nonlinear_tp<double2>(c, sp, s);
     else
-        hipLaunchKernelGGL((k_nonlinear<kTP, float2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, sp);
+        launch_nonlinear_tp<float2>(c, sp, s);
 }
 
 void launch_pressure_rhs(const Chan &c, const double2 *dv, double2 *R, hipStream_t s) {

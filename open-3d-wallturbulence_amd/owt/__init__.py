@@ -38,6 +38,7 @@ EXPORTS = [
     "owt_ensemble_create", "owt_ensemble_destroy", "owt_ensemble_last_error", "owt_ensemble_member",
     "owt_ensemble_init_fields", "owt_ensemble_step", "owt_ensemble_time_steps",
     "owt_tangent_init", "owt_tangent_handle", "owt_tangent_step", "owt_adjoint_run",
+    "owt_autotune", "owt_get_tune", "owt_set_tune",
 ]
 
 
@@ -98,6 +99,9 @@ def lib():
         L.owt_time_steps.argtypes = [P, I, P]
         L.owt_time_kernel.argtypes = [P, I, I, P]
         L.owt_get_factor.argtypes = [P, I, I, P, P]
+        L.owt_autotune.argtypes = [P, I, ctypes.c_char_p, I]
+        L.owt_get_tune.argtypes = [P, P]
+        L.owt_set_tune.argtypes = [P, P]
         L.owt_tangent_init.argtypes = [P, P, P, P]
         L.owt_tangent_handle.argtypes = [P]
         L.owt_tangent_handle.restype = P
@@ -265,6 +269,25 @@ class Channel:
         ms = ctypes.c_double()
         self._check(lib().owt_time_steps(self._h, int(n), ctypes.byref(ms)))
         return ms.value
+
+    def autotune(self, reps=5):
+        """Time every candidate launch shape on this channel and keep the fastest;
+        returns {knob: {"choice": i, "ms": {shape: ms}}}."""
+        import json
+        buf = ctypes.create_string_buffer(4096)
+        self._check(lib().owt_autotune(self._h, int(reps), buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+    @property
+    def tune(self):
+        k = np.zeros(6, dtype=np.int32)
+        self._check(lib().owt_get_tune(self._h, _p(k)))
+        return [int(v) for v in k]
+
+    @tune.setter
+    def tune(self, knobs):
+        k = np.ascontiguousarray(knobs, dtype=np.int32)
+        self._check(lib().owt_set_tune(self._h, _p(k)))
 
     def tangent_init(self, du, dv, dw):
         """Attach the tangent (forward-mode linearization) started from the

@@ -291,3 +291,33 @@ def test_ensemble_members_equal_standalone_channels(owt_mod):
         ch.close()
     assert ens.time_steps(2) > 0
     ens.close()
+
+
+@pytest.mark.parametrize("shape", [(16, 17, 16), (128, 17, 128), (384, 9, 384), (768, 7, 32)])
+def test_every_tuned_launch_shape_gives_identical_results(owt_mod, shape):
+    """Each candidate launch shape of each tunable kernel (forced one knob at a time),
+    and the autotuner's own choice, step bit for bit like the default shapes; the
+    shapes cover the four transform length classes."""
+    p = params(*shape)
+    print(f"[gpu] tune candidates {shape}", flush=True)
+    ch = gpu(owt_mod, p)
+
+    def run(knobs):
+        ch.tune = knobs
+        ch.init_fields(2, seed=3, amp=0.15)
+        ch.step(2)
+        return [ch.field(n) for n in ("u", "v", "w", "lmbx")]
+
+    ref = run([0] * 6)
+    for k in range(6):
+        for cand in (1, 2):
+            knobs = [0] * 6
+            knobs[k] = cand
+            got = run(knobs)
+            assert all(np.array_equal(a, b) for a, b in zip(got, ref)), (k, cand)
+    ch.init_fields(2, seed=3, amp=0.15)
+    rep = ch.autotune(2)
+    assert len(rep) == 6 and all(0 <= v["choice"] <= 2 for v in rep.values())
+    got = run(ch.tune)
+    assert all(np.array_equal(a, b) for a, b in zip(got, ref))
+    ch.close()
