@@ -24,6 +24,25 @@ namespace owt {
 // operator columns fetched per batch in the row-per-thread mat-vecs
 constexpr int JC = 8;
 
+typedef double owt_d4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Dense wall-normal derivatives on the matrix cores (mixed precision mode; the
+// FP64 mode keeps the row-per-thread kernels, whose summation order follows the
+// reference's column-axpy mat-vec). Across pencils the mat-vecs form one GEMM,
+// Y (N x 2P) = DY (N x N) X (N x 2P), the columns of X being the pencils' real
+// and imaginary parts. A block takes kMP = 16 pencils (32 columns) and every
+// row, so each DY element read from L2 serves 32 columns; the pencils are staged
+// in LDS kKC rows at a time. v_mfma_f64_16x16x4f64 per wave: A = DY rows x 4 k
+// (lane l: row l%16, k l/16), B = 4 k x 16 columns (lane l: k l/16, column
+// l%16), D = 16 x 16 (lane l, register r: row 4r + l/16, column l%16). Wave w
+// owns the row tiles w, w+4, w+8, ...; column c of a column tile is pencil
+// c/2, real (c even) or imaginary part, and the two parts meet by a lane swap.
+// ---------------------------------------------------------------------------
+constexpr int kMP = 16, kKC = 32, kMaxRT = 4;
+
+__device__ __forceinline__ owt_d4 mfma4(double a, double b, owt_d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
 
 // ---------------------------------------------------------------------------
 // horizontal velocity and vorticity (src/solver.cpp:516-539)
@@ -97,6 +116,197 @@ __global__ __launch_bounds__(256) void k_horizontal(Chan c, StepPtrs sp) {
         const double2 Y = caddc(csubc(cscale(d1g[pp], kz), cscale(d2v[pp], kx)), cscale(vv, mul(kx, k2)));
         ox[o] = imul(ir, X);
         oz[o] = imul(ir, Y);
+    }
+}
+
+// horizontal velocity and vorticity (src/solver.cpp:516-539) with the three
+// products DY1 v, DY2 v, DY1 omega_y on the matrix cores. Eight waves; wave w
+// owns row tiles w and w + 8, and loads the DY operands of the next k step
+// before issuing the products of the current one.
+__global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
+    constexpr int RT = 2, NW = 8;
+    __shared__ double2 sv[kMP][kKC + 1], sg[kMP][kKC + 1];
+    const int N = c.NY, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int p0 = blockIdx.x * kMP, ntile = (N + 15) / 16;
+    owt_d4 acc[RT][2][3];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) acc[rt][ct][q] = owt_d4{0.0, 0.0, 0.0, 0.0};
+    const int am = lane & 15, ak = lane >> 4, bq = (lane & 15) >> 1, bc = lane & 1;
+    // DY operands of k step k (this lane's row and k), zero outside the matrix
+    auto loadA = [&](int k, double (&a1)[RT], double (&a2)[RT]) {
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            const int row = (w + NW * rt) * 16 + am;
+            const bool in = row < N && k < N;
+            const size_t e = (size_t)(k < N ? k : 0) * N + (row < N ? row : 0);
+            a1[rt] = in ? c.DY1[e] : 0.0;
+            a2[rt] = in ? c.DY2[e] : 0.0;
+        }
+    };
+    // operands of the current k step and of the next PF - 1 steps, in flight
+    constexpr int PF = 3;
+    double q1[PF][RT], q2[PF][RT];
+#pragma unroll
+    for (int d = 0; d < PF; ++d) loadA(4 * d + ak, q1[d], q2[d]);
+    int kstep = 0;
+    for (int k0 = 0; k0 < N; k0 += kKC) {
+        for (int e = threadIdx.x; e < kMP * kKC; e += blockDim.x) {
+            const int q = e / kKC, kk = e % kKC, p = p0 + q, k = k0 + kk;
+            double2 a = c2(0, 0), b = c2(0, 0);
+            if (p < c.npencil && k < N) {
+                a = sp.vn[(size_t)p * N + k];
+                b = sp.gn[(size_t)p * N + k];
+            }
+            sv[q][kk] = a;
+            sg[q][kk] = b;
+        }
+        __syncthreads();
+        const int kend = N - k0 < kKC ? N - k0 : kKC;
+        for (int kk = 0; kk < kend; kk += 4, ++kstep) {
+            double a1[RT], a2[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                a1[rt] = q1[0][rt];
+                a2[rt] = q2[0][rt];
+            }
+#pragma unroll
+            for (int d = 0; d + 1 < PF; ++d)
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    q1[d][rt] = q1[d + 1][rt];
+                    q2[d][rt] = q2[d + 1][rt];
+                }
+            loadA(4 * (kstep + PF) + ak, q1[PF - 1], q2[PF - 1]);
+            double bv[2], bg[2];
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+                const double2 x = sv[ct * 8 + bq][kk + ak], y = sg[ct * 8 + bq][kk + ak];
+                bv[ct] = bc ? x.y : x.x;
+                bg[ct] = bc ? y.y : y.x;
+            }
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                if (w + NW * rt >= ntile) break;
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    acc[rt][ct][0] = mfma4(a1[rt], bv[ct], acc[rt][ct][0]);
+                    acc[rt][ct][1] = mfma4(a2[rt], bv[ct], acc[rt][ct][1]);
+                    acc[rt][ct][2] = mfma4(a1[rt], bg[ct], acc[rt][ct][2]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double2 *u = c.cur, *wv = c.cur + c.W, *ox = c.cur + 2 * c.W, *oz = c.cur + 3 * c.W;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int tile = w + NW * rt;
+        if (tile >= ntile) break;
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double m1 = acc[rt][ct][0][r], m2 = acc[rt][ct][1][r], m3 = acc[rt][ct][2][r];
+                const double o1 = __shfl_xor(m1, 1), o2 = __shfl_xor(m2, 1), o3 = __shfl_xor(m3, 1);
+                const int row = tile * 16 + 4 * r + (lane >> 4), p = p0 + ct * 8 + bq;
+                if (bc != 0 || row >= N || p >= c.npencil) continue;
+                const double2 d1v = c2(m1, o1), d2v = c2(m2, o2), d1g = c2(m3, o3);
+                const size_t o = (size_t)p * N + row;
+                sp.dvn[o] = d2v;
+                const int k = p / c.NXH, i = p % c.NXH;
+                const double kx = c.kx[i], kz = c.kz[k];
+                const double k2 = k2_of(c, k, i);
+                if (k2 < 1.e-8) continue;  // the mean pencil is set by the mean-flow kernels
+                const double ix = __ddiv_rn(kx, k2), iz = __ddiv_rn(kz, k2), ir = __ddiv_rn(1.0, k2);
+                const double2 vv = sp.vn[o], gg = sp.gn[o];
+                u[o] = csubc(imul(ix, d1v), imul(iz, gg));
+                wv[o] = caddc(imul(ix, gg), imul(iz, d1v));
+                const double2 X = csubc(caddc(cscale(d1g, kx), cscale(d2v, kz)), cscale(vv, mul(kz, k2)));
+                const double2 Yv = caddc(csubc(cscale(d1g, kz), cscale(d2v, kx)), cscale(vv, mul(kx, k2)));
+                ox[o] = imul(ir, X);
+                oz[o] = imul(ir, Yv);
+            }
+        }
+    }
+}
+
+// nonlinear source terms (src/solver.cpp:240-260) with DY1 (i kx Lx + i kz Lz)
+// on the matrix cores; Hg and the mean Lamb history are written while staging
+template <typename LC>
+__global__ __launch_bounds__(256) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
+    __shared__ double2 st[kMP][kKC + 1];
+    const int N = c.NY, ns = sp.sn, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const LC *lx = reinterpret_cast<const LC *>(c.lmb);
+    const LC *ly = lx + c.W, *lz = lx + 2 * c.W;
+    const int p0 = blockIdx.x * kMP, ntile = (N + 15) / 16;
+    owt_d4 acc[kMaxRT][2];
+#pragma unroll
+    for (int rt = 0; rt < kMaxRT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = owt_d4{0.0, 0.0, 0.0, 0.0};
+    const int am = lane & 15, ak = lane >> 4, bq = (lane & 15) >> 1, bc = lane & 1;
+    for (int k0 = 0; k0 < N; k0 += kKC) {
+        for (int e = threadIdx.x; e < kMP * kKC; e += blockDim.x) {
+            const int q = e / kKC, kk = e % kKC, p = p0 + q, k = k0 + kk;
+            double2 t = c2(0, 0);
+            if (p < c.npencil && k < N) {
+                const int kz_ = p / c.NXH, i = p % c.NXH;
+                const double kx = c.kx[i], kz = c.kz[kz_];
+                const size_t o = (size_t)p * N + k;
+                const double2 a = ld2(lx + o), b = ld2(lz + o);
+                t = caddc(imul(kx, a), imul(kz, b));
+                sp.hgn[o] = csubc(imul(kz, a), imul(kx, b));
+                if (p == 0) {
+                    c.means[(size_t)(M_LMBX * 3 + ns) * N + k] = a;
+                    c.means[(size_t)(M_LMBZ * 3 + ns) * N + k] = b;
+                }
+            }
+            st[q][kk] = t;
+        }
+        __syncthreads();
+        const int kend = N - k0 < kKC ? N - k0 : kKC;
+        for (int kk = 0; kk < kend; kk += 4) {
+            const int k = k0 + kk + ak;
+            double bt[2];
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+                const double2 x = st[ct * 8 + bq][kk + ak];
+                bt[ct] = bc ? x.y : x.x;
+            }
+#pragma unroll
+            for (int rt = 0; rt < kMaxRT; ++rt) {
+                const int tile = w + 4 * rt;
+                if (tile >= ntile) break;
+                const int row = tile * 16 + am;
+                const bool in = row < N && k < N;
+                const double a1 = in ? c.DY1[(size_t)(k < N ? k : 0) * N + (row < N ? row : 0)] : 0.0;
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = mfma4(a1, bt[ct], acc[rt][ct]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int rt = 0; rt < kMaxRT; ++rt) {
+        const int tile = w + 4 * rt;
+        if (tile >= ntile) break;
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double m = acc[rt][ct][r], o1 = __shfl_xor(m, 1);
+                const int row = tile * 16 + 4 * r + (lane >> 4), p = p0 + ct * 8 + bq;
+                if (bc != 0 || row >= N || p >= c.npencil) continue;
+                const int k = p / c.NXH, i = p % c.NXH;
+                const double k2 = k2_of(c, k, i);
+                const size_t o = (size_t)p * N + row;
+                sp.hvn[o] = csubc(cscale(ld2(ly + o), -k2), c2(m, o1));
+            }
+        }
     }
 }
 
@@ -825,6 +1035,10 @@ static void launch_nonlinear_tp(const Chan &c, const StepPtrs &sp, hipStream_t s
 
 // tuned pencils per block: candidates kTPH, 2, 8 (horizontal) and kTP, 4, 16 (nonlinear)
 void launch_horizontal(const Chan &c, const StepPtrs &sp, hipStream_t s) {
+    if (c.precision == 1 && c.NY <= 256) {
+        hipLaunchKernelGGL(k_horizontal_mfma, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
+        return;
+    }
     const int sel = c.tune[T_HORIZONTAL_TP];
     if (sel == 1) hipLaunchKernelGGL(k_horizontal<2>, dim3((c.npencil + 1) / 2), dim3(256), 0, s, c, sp);
     else if (sel == 2) hipLaunchKernelGGL(k_horizontal<8>, dim3((c.npencil + 7) / 8), dim3(256), 0, s, c, sp);
@@ -838,6 +1052,8 @@ void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *
 void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s) {
     if (c.precision == 0)
         launch_nonlinear_tp<double2>(c, sp, s);
+    else if (c.NY <= 16 * 4 * kMaxRT)
+        hipLaunchKernelGGL(k_nonlinear_mfma<float2>, dim3((c.npencil + kMP - 1) / kMP), dim3(256), 0, s, c, sp);
     else
         launch_nonlinear_tp<float2>(c, sp, s);
 }
