@@ -1115,6 +1115,8 @@ std::string Channel::autotune(int reps) {
         if ((k == T_Z_INVERSE_TILE || k == T_Z_FORWARD_TILE) && c_.NZ2 > 576) ncand = 2;
         if (k == T_X_TILE && c_.NX2 > 576) ncand = 2;
         if (k == T_SOLVE_JOBS && P_.method == 0) ncand = 1;  // the plain solve has one shape
+        // mixed precision runs these two stages on the matrix-core kernels, which have one shape
+        if ((k == T_HORIZONTAL_TP || k == T_NONLINEAR_TP) && c_.precision == 1 && c_.NY <= 256) ncand = 1;
         double best = 1e300;
         int arg = 0;
         std::string times;

@@ -39,7 +39,8 @@ typedef double owt_d4 __attribute__((ext_vector_type(4)));
 // owns the row tiles w, w+4, w+8, ...; column c of a column tile is pencil
 // c/2, real (c even) or imaginary part, and the two parts meet by a lane swap.
 // ---------------------------------------------------------------------------
-constexpr int kMP = 16, kKC = 32, kMaxRT = 4;
+constexpr int kMP = 16, kKC = 32;  // kMP * kKC = 512, the block size of both kernels
+static_assert(kMP * kKC == 512, "one staged element per thread");
 
 __device__ __forceinline__ owt_d4 mfma4(double a, double b, owt_d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
@@ -129,6 +130,8 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
     const int N = c.NY, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int p0 = blockIdx.x * kMP, ntile = (N + 15) / 16;
     owt_d4 acc[RT][2][3];
+    // staging: this thread's element of a chunk (kMP * kKC == blockDim)
+    const int sq = threadIdx.x / kKC, skk = threadIdx.x % kKC, sp_ = p0 + sq;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -154,15 +157,15 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
     for (int d = 0; d < PF; ++d) loadA(4 * d + ak, q1[d], q2[d]);
     int kstep = 0;
     for (int k0 = 0; k0 < N; k0 += kKC) {
-        for (int e = threadIdx.x; e < kMP * kKC; e += blockDim.x) {
-            const int q = e / kKC, kk = e % kKC, p = p0 + q, k = k0 + kk;
+        {
+            const int k = k0 + skk;
             double2 a = c2(0, 0), b = c2(0, 0);
-            if (p < c.npencil && k < N) {
-                a = sp.vn[(size_t)p * N + k];
-                b = sp.gn[(size_t)p * N + k];
+            if (sp_ < c.npencil && k < N) {
+                a = sp.vn[(size_t)sp_ * N + k];
+                b = sp.gn[(size_t)sp_ * N + k];
             }
-            sv[q][kk] = a;
-            sg[q][kk] = b;
+            sv[sq][skk] = a;
+            sg[sq][skk] = b;
         }
         __syncthreads();
         const int kend = N - k0 < kKC ? N - k0 : kKC;
@@ -235,42 +238,70 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
 }
 
 // nonlinear source terms (src/solver.cpp:240-260) with DY1 (i kx Lx + i kz Lz)
-// on the matrix cores; Hg and the mean Lamb history are written while staging
+// on the matrix cores; Hg and the mean Lamb history are written while staging.
+// Eight waves, wave w owning row tiles w and w + 8, DY1 prefetched as in
+// k_horizontal_mfma.
 template <typename LC>
-__global__ __launch_bounds__(256) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
+__global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
+    constexpr int RT = 2, NW = 8, PF = 3;
     __shared__ double2 st[kMP][kKC + 1];
     const int N = c.NY, ns = sp.sn, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const LC *lx = reinterpret_cast<const LC *>(c.lmb);
     const LC *ly = lx + c.W, *lz = lx + 2 * c.W;
     const int p0 = blockIdx.x * kMP, ntile = (N + 15) / 16;
-    owt_d4 acc[kMaxRT][2];
+    // staging: this thread's element of a chunk (kMP * kKC == blockDim)
+    const int sq = threadIdx.x / kKC, skk = threadIdx.x % kKC, spn = p0 + sq;
+    double skx = 0.0, skz = 0.0;
+    if (spn < c.npencil) {
+        skx = c.kx[spn % c.NXH];
+        skz = c.kz[spn / c.NXH];
+    }
+    auto stage = [&](int k0) {
+        const int k = k0 + skk;
+        double2 t = c2(0, 0);
+        if (spn < c.npencil && k < N) {
+            const size_t o = (size_t)spn * N + k;
+            const double2 a = ld2(lx + o), b = ld2(lz + o);
+            t = caddc(imul(skx, a), imul(skz, b));
+            sp.hgn[o] = csubc(imul(skz, a), imul(skx, b));
+            if (spn == 0) {
+                c.means[(size_t)(M_LMBX * 3 + ns) * N + k] = a;
+                c.means[(size_t)(M_LMBZ * 3 + ns) * N + k] = b;
+            }
+        }
+        st[sq][skk] = t;
+    };
+    owt_d4 acc[RT][2];
 #pragma unroll
-    for (int rt = 0; rt < kMaxRT; ++rt)
+    for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = owt_d4{0.0, 0.0, 0.0, 0.0};
     const int am = lane & 15, ak = lane >> 4, bq = (lane & 15) >> 1, bc = lane & 1;
-    for (int k0 = 0; k0 < N; k0 += kKC) {
-        for (int e = threadIdx.x; e < kMP * kKC; e += blockDim.x) {
-            const int q = e / kKC, kk = e % kKC, p = p0 + q, k = k0 + kk;
-            double2 t = c2(0, 0);
-            if (p < c.npencil && k < N) {
-                const int kz_ = p / c.NXH, i = p % c.NXH;
-                const double kx = c.kx[i], kz = c.kz[kz_];
-                const size_t o = (size_t)p * N + k;
-                const double2 a = ld2(lx + o), b = ld2(lz + o);
-                t = caddc(imul(kx, a), imul(kz, b));
-                sp.hgn[o] = csubc(imul(kz, a), imul(kx, b));
-                if (p == 0) {
-                    c.means[(size_t)(M_LMBX * 3 + ns) * N + k] = a;
-                    c.means[(size_t)(M_LMBZ * 3 + ns) * N + k] = b;
-                }
-            }
-            st[q][kk] = t;
+    auto loadA = [&](int k, double (&a1)[RT]) {
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            const int row = (w + NW * rt) * 16 + am;
+            const bool in = row < N && k < N;
+            a1[rt] = in ? c.DY1[(size_t)(k < N ? k : 0) * N + (row < N ? row : 0)] : 0.0;
         }
+    };
+    double q1[PF][RT];
+#pragma unroll
+    for (int d = 0; d < PF; ++d) loadA(4 * d + ak, q1[d]);
+    int kstep = 0;
+    for (int k0 = 0; k0 < N; k0 += kKC) {
+        stage(k0);
         __syncthreads();
         const int kend = N - k0 < kKC ? N - k0 : kKC;
-        for (int kk = 0; kk < kend; kk += 4) {
-            const int k = k0 + kk + ak;
+        for (int kk = 0; kk < kend; kk += 4, ++kstep) {
+            double a1[RT];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) a1[rt] = q1[0][rt];
+#pragma unroll
+            for (int d = 0; d + 1 < PF; ++d)
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) q1[d][rt] = q1[d + 1][rt];
+            loadA(4 * (kstep + PF) + ak, q1[PF - 1]);
             double bt[2];
 #pragma unroll
             for (int ct = 0; ct < 2; ++ct) {
@@ -278,21 +309,17 @@ __global__ __launch_bounds__(256) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
                 bt[ct] = bc ? x.y : x.x;
             }
 #pragma unroll
-            for (int rt = 0; rt < kMaxRT; ++rt) {
-                const int tile = w + 4 * rt;
-                if (tile >= ntile) break;
-                const int row = tile * 16 + am;
-                const bool in = row < N && k < N;
-                const double a1 = in ? c.DY1[(size_t)(k < N ? k : 0) * N + (row < N ? row : 0)] : 0.0;
+            for (int rt = 0; rt < RT; ++rt) {
+                if (w + NW * rt >= ntile) break;
 #pragma unroll
-                for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = mfma4(a1, bt[ct], acc[rt][ct]);
+                for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = mfma4(a1[rt], bt[ct], acc[rt][ct]);
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int rt = 0; rt < kMaxRT; ++rt) {
-        const int tile = w + 4 * rt;
+    for (int rt = 0; rt < RT; ++rt) {
+        const int tile = w + NW * rt;
         if (tile >= ntile) break;
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct) {
@@ -1052,8 +1079,8 @@ void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *
 void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s) {
     if (c.precision == 0)
         launch_nonlinear_tp<double2>(c, sp, s);
-    else if (c.NY <= 16 * 4 * kMaxRT)
-        hipLaunchKernelGGL(k_nonlinear_mfma<float2>, dim3((c.npencil + kMP - 1) / kMP), dim3(256), 0, s, c, sp);
+    else if (c.NY <= 256)
+        hipLaunchKernelGGL(k_nonlinear_mfma<float2>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
     else
         launch_nonlinear_tp<float2>(c, sp, s);
 }
