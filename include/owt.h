@@ -135,7 +135,7 @@ int owt_get_factor(owt_handle *h, int sys, int op, double *lu, int *perm);
 /* Launch-shape autotuner (no reference counterpart): times every candidate
  * shape of each tunable launch on the handle's own data (reps launches each),
  * keeps the fastest and recaptures the step graphs. Knobs (OWT_TUNE_COUNT ints,
- * candidate index 0..2, 0 = default): solve jobs per block, pencils per block
+ * candidate index 0..3, 0 = default): solve jobs per block, pencils per block
  * of the horizontal and the nonlinear kernels, tile shape of the z inverse, x
  * and z forward transforms. Every candidate computes the same values in the
  * same order, so results do not depend on the choice. report (may be NULL)

@@ -1099,8 +1099,8 @@ std::string Channel::autotune(int reps) {
     static const char *names[T_COUNT] = {"solve_jobs_per_block", "horizontal_pencils_per_block", "nonlinear_pencils_per_block",
                                          "z_inverse_tile", "x_tile", "z_forward_tile"};
     static const char *shapes[T_COUNT][kTuneCandidates] = {
-        {"2", "1", "4"}, {"4", "2", "8"}, {"8", "4", "16"}, {"default", "alt1", "alt2"}, {"default", "alt1", "alt2"},
-        {"default", "alt1", "alt2"}};
+        {"2", "1", "4", ""}, {"4", "2", "8", ""}, {"8", "4", "16", ""}, {"default", "alt1", "alt2", "alt3"},
+        {"default", "alt1", "alt2", "alt3"}, {"default", "alt1", "alt2", "alt3"}};
     // stage kernel of each knob (enqueue_kernel ids)
     static const int kernel[T_COUNT] = {0, 1, 6, 3, 4, 5};
     prepare_step();
@@ -1110,10 +1110,14 @@ std::string Channel::autotune(int reps) {
     OWT_HIP(hipEventCreate(&b));
     std::string rep = "{";
     for (int k = 0; k < T_COUNT; ++k) {
-        // the length class >= 577 has two x/z candidates
-        int ncand = kTuneCandidates;
-        if ((k == T_Z_INVERSE_TILE || k == T_Z_FORWARD_TILE) && c_.NZ2 > 576) ncand = 2;
-        if (k == T_X_TILE && c_.NX2 > 576) ncand = 2;
+        // transform tiles: three candidates per length class (two above 576), and in
+        // FP32 a fourth for lengths above 192; the other knobs have three
+        int ncand = 3;
+        if (k == T_Z_INVERSE_TILE || k == T_Z_FORWARD_TILE || k == T_X_TILE) {
+            const int L = k == T_X_TILE ? c_.NX2 : c_.NZ2;
+            if (c_.precision == 1 && L > 192) ncand = 4;
+            else if (L > 576) ncand = 2;
+        }
         if (k == T_SOLVE_JOBS && P_.method == 0) ncand = 1;  // the plain solve has one shape
         // mixed precision runs these two stages on the matrix-core kernels, which have one shape
         if ((k == T_HORIZONTAL_TP || k == T_NONLINEAR_TP) && c_.precision == 1 && c_.NY <= 256) ncand = 1;

@@ -37,7 +37,7 @@ enum ErrBits { E_SINGULAR = 1 };
 // Launch shapes (candidate indices, 0 = the default shape) chosen by the
 // autotuner; every candidate computes the same values in the same order.
 enum TuneKnob { T_SOLVE_JOBS = 0, T_HORIZONTAL_TP, T_NONLINEAR_TP, T_Z_INVERSE_TILE, T_X_TILE, T_Z_FORWARD_TILE, T_COUNT };
-constexpr int kTuneCandidates = 3;
+constexpr int kTuneCandidates = 4;
 
 struct Chan {
     int NX, NY, NZ, NXH, NZH, NX2, NZ2;

@@ -672,6 +672,30 @@ size_t smem_bytes() {
             else { constexpr int TJ = 2, TL = 128, LMAX = 1152; BODY; }                             \
         }                                                                                           \
     } while (0)
+// FP32 (mixed precision) variant: candidate 3 doubles the y-run of the default
+// shape of the two long length classes, which only fits 64 KB of LDS in FP32
+#define OWT_TUNED_DISPATCH_F32(L, SEL, BODY)                                                        \
+    do {                                                                                            \
+        const int sel_ = (SEL);                                                                     \
+        if ((L) <= 96) {                                                                            \
+            if (sel_ == 1) { constexpr int TJ = 8, TL = 32, LMAX = 96; BODY; }                      \
+            else if (sel_ == 2) { constexpr int TJ = 32, TL = 8, LMAX = 96; BODY; }                 \
+            else { constexpr int TJ = 16, TL = 16, LMAX = 96; BODY; }                               \
+        } else if ((L) <= 192) {                                                                    \
+            if (sel_ == 1) { constexpr int TJ = 16, TL = 16, LMAX = 192; BODY; }                    \
+            else if (sel_ == 2) { constexpr int TJ = 4, TL = 64, LMAX = 192; BODY; }                \
+            else { constexpr int TJ = 8, TL = 32, LMAX = 192; BODY; }                               \
+        } else if ((L) <= 576) {                                                                    \
+            if (sel_ == 1) { constexpr int TJ = 2, TL = 128, LMAX = 576; BODY; }                    \
+            else if (sel_ == 2) { constexpr int TJ = 1, TL = 256, LMAX = 576; BODY; }               \
+            else if (sel_ == 3) { constexpr int TJ = 8, TL = 32, LMAX = 576; BODY; }                \
+            else { constexpr int TJ = 4, TL = 64, LMAX = 576; BODY; }                               \
+        } else {                                                                                    \
+            if (sel_ == 1) { constexpr int TJ = 1, TL = 256, LMAX = 1152; BODY; }                   \
+            else if (sel_ == 3) { constexpr int TJ = 4, TL = 64, LMAX = 1152; BODY; }               \
+            else { constexpr int TJ = 2, TL = 128, LMAX = 1152; BODY; }                             \
+        }                                                                                           \
+    } while (0)
 // the same shapes for the x phase (spelled out: a forwarding macro would expand
 // the launch in BODY before substitution)
 #define OWT_X_DISPATCH(L, T, BODY)                                                   \
@@ -694,7 +718,7 @@ void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out
                                (double2 *)out, nf, 1.0, 0);
         });
     else
-        OWT_TUNED_DISPATCH(c.NZ2, c.tune[T_Z_INVERSE_TILE], {
+        OWT_TUNED_DISPATCH_F32(c.NZ2, c.tune[T_Z_INVERSE_TILE], {
             dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * nf));
             hipLaunchKernelGGL((k_z_inverse<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c, t,
                                (float2 *)out, nf, 1.0f, 0);
@@ -709,7 +733,7 @@ void launch_x_lamb(const Chan &c, hipStream_t s) {
                                (const double2 *)c.zbuf, (double2 *)c.xbuf);
         });
     else
-        OWT_TUNED_DISPATCH(c.NX2, c.tune[T_X_TILE], {
+        OWT_TUNED_DISPATCH_F32(c.NX2, c.tune[T_X_TILE], {
             dim3 g(xcd_grid(c.NZ2 * ((c.NY + TJ - 1) / TJ)));
             hipLaunchKernelGGL((k_x_lamb<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
                                (const float2 *)c.zbuf, (float2 *)c.xbuf);
@@ -724,7 +748,7 @@ void launch_z_forward(const Chan &c, hipStream_t s) {
                                c, (const double2 *)c.xbuf, (double2 *)c.lmb, 3, (double)c.NZ2, 1);
         });
     else
-        OWT_TUNED_DISPATCH(c.NZ2, c.tune[T_Z_FORWARD_TILE], {
+        OWT_TUNED_DISPATCH_F32(c.NZ2, c.tune[T_Z_FORWARD_TILE], {
             dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * 3));
             hipLaunchKernelGGL((k_z_forward<float, TJ, TL, LMAX, float2>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
                                (const float2 *)c.xbuf, (float2 *)c.lmb, 3, (float)c.NZ2, 1);

@@ -293,13 +293,15 @@ def test_ensemble_members_equal_standalone_channels(owt_mod):
     ens.close()
 
 
-@pytest.mark.parametrize("shape", [(16, 17, 16), (128, 17, 128), (384, 9, 384), (768, 7, 32)])
-def test_every_tuned_launch_shape_gives_identical_results(owt_mod, shape):
+@pytest.mark.parametrize("shape,precision", [((16, 17, 16), 0), ((128, 17, 128), 0), ((384, 9, 384), 0),
+                                             ((768, 7, 32), 0), ((384, 9, 384), 1), ((768, 7, 32), 1)])
+def test_every_tuned_launch_shape_gives_identical_results(owt_mod, shape, precision):
     """Each candidate launch shape of each tunable kernel (forced one knob at a time),
     and the autotuner's own choice, step bit for bit like the default shapes; the
-    shapes cover the four transform length classes."""
-    p = params(*shape)
-    print(f"[gpu] tune candidates {shape}", flush=True)
+    shapes cover the four transform length classes, and the FP32 transforms' extra
+    candidates in mixed precision."""
+    p = params(*shape, precision=precision)
+    print(f"[gpu] tune candidates {shape} precision {precision}", flush=True)
     ch = gpu(owt_mod, p)
 
     def run(knobs):
@@ -310,14 +312,14 @@ def test_every_tuned_launch_shape_gives_identical_results(owt_mod, shape):
 
     ref = run([0] * 6)
     for k in range(6):
-        for cand in (1, 2):
+        for cand in (1, 2, 3):
             knobs = [0] * 6
             knobs[k] = cand
             got = run(knobs)
             assert all(np.array_equal(a, b) for a, b in zip(got, ref)), (k, cand)
     ch.init_fields(2, seed=3, amp=0.15)
     rep = ch.autotune(2)
-    assert len(rep) == 6 and all(0 <= v["choice"] <= 2 for v in rep.values())
+    assert len(rep) == 6 and all(0 <= v["choice"] <= 3 for v in rep.values())
     got = run(ch.tune)
     assert all(np.array_equal(a, b) for a, b in zip(got, ref))
     ch.close()
