@@ -16,10 +16,10 @@ FFTPlan make_plan(int L) {
     FFTPlan p;
     p.L = L;
     int n = L;
-    while (n % 4 == 0) p.radix[p.nst++] = 4, n /= 4;
-    while (n % 3 == 0) p.radix[p.nst++] = 3, n /= 3;
-    while (n % 2 == 0) p.radix[p.nst++] = 2, n /= 2;
-    if (n != 1 || p.nst > kMaxStages) throw Error(2, "transform length " + std::to_string(L) + " is not 2^a 3^b");
+    while (n % 4 == 0 && p.nst < kMaxStages) p.radix[p.nst++] = 4, n /= 4;
+    while (n % 3 == 0 && p.nst < kMaxStages) p.radix[p.nst++] = 3, n /= 3;
+    while (n % 2 == 0 && p.nst < kMaxStages) p.radix[p.nst++] = 2, n /= 2;
+    if (n != 1) throw Error(2, "transform length " + std::to_string(L) + " is not 2^a 3^b with at most 8 stages");
     int ns = 1;
     for (int s = 0; s < p.nst; ++s) {
         p.ns[s] = ns;
