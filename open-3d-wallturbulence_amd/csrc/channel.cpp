@@ -1119,8 +1119,8 @@ std::string Channel::autotune(int reps) {
             else if (L > 576) ncand = 2;
         }
         if (k == T_SOLVE_JOBS && P_.method == 0) ncand = 1;  // the plain solve has one shape
-        // mixed precision runs these two stages on the matrix-core kernels, which have one shape
-        if ((k == T_HORIZONTAL_TP || k == T_NONLINEAR_TP) && c_.precision == 1 && c_.NY <= 256) ncand = 1;
+        // up to 256 points in y these two stages run on the matrix-core kernels, which have one shape
+        if ((k == T_HORIZONTAL_TP || k == T_NONLINEAR_TP) && c_.NY <= 256) ncand = 1;
         double best = 1e300;
         int arg = 0;
         std::string times;

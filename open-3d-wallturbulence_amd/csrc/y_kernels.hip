@@ -27,9 +27,9 @@ constexpr int JC = 8;
 typedef double owt_d4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
-// Dense wall-normal derivatives on the matrix cores (mixed precision mode; the
-// FP64 mode keeps the row-per-thread kernels, whose summation order follows the
-// reference's column-axpy mat-vec). Across pencils the mat-vecs form one GEMM,
+// Dense wall-normal derivatives on the matrix cores (both precisions, NY <= 256;
+// the row-per-thread kernels above serve larger NY). Across pencils the mat-vecs
+// form one GEMM,
 // Y (N x 2P) = DY (N x N) X (N x 2P), the columns of X being the pencils' real
 // and imaginary parts. A block takes kMP = 16 pencils (32 columns) and every
 // row, so each DY element read from L2 serves 32 columns; the pencils are staged
@@ -1062,7 +1062,7 @@ static void launch_nonlinear_tp(const Chan &c, const StepPtrs &sp, hipStream_t s
 
 // tuned pencils per block: candidates kTPH, 2, 8 (horizontal) and kTP, 4, 16 (nonlinear)
 void launch_horizontal(const Chan &c, const StepPtrs &sp, hipStream_t s) {
-    if (c.precision == 1 && c.NY <= 256) {
+    if (c.NY <= 256) {
         hipLaunchKernelGGL(k_horizontal_mfma, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
         return;
     }
@@ -1077,7 +1077,9 @@ void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *
 }
 
 void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s) {
-    if (c.precision == 0)
+    if (c.precision == 0 && c.NY <= 256)
+        hipLaunchKernelGGL(k_nonlinear_mfma<double2>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
+    else if (c.precision == 0)
         launch_nonlinear_tp<double2>(c, sp, s);
     else if (c.NY <= 256)
         hipLaunchKernelGGL(k_nonlinear_mfma<float2>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
