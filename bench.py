@@ -260,6 +260,25 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     print(json.dumps({"mode": "copy", "bytes_per_copy": nbytes, "reps": reps}), flush=True)
 
 
+def copy_rate(nbytes=1 << 30, reps=10):
+    """Measured read + write rate of a plain device-to-device copy (GB/s): the
+    practical HBM reference beside the published peak."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    b.synchronize()
+    gbps = 2 * nbytes * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(gbps, 1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -389,8 +408,11 @@ def main():
         "kernels_ms_mixed": {k: round(v, 4) for k, v in kern.items()},
         "kernels_ms_fp64": {k: round(v, 4) for k, v in kern64.items()},
         "roofline": roof,
+        "hbm_copy_gbps": None,
         "autotune": {k: v["choice"] for k, v in tuned.items()} if tuned else "off (default shapes)",
     }
+    out["hbm_copy_gbps"] = copy_rate()
+    out["roofline"]["frac_of_copy_rate"] = round(achieved / out["hbm_copy_gbps"], 4)
     if args.ensemble_steps > 0:
         print("[bench] ensemble", file=sys.stderr, flush=True)
         out["ensemble"] = ensemble_throughput(owt, "re180_32", args.ensemble_steps)

@@ -243,7 +243,17 @@ __global__ __launch_bounds__(128 * JPB) void k_solve(Chan c, StepPtrs sp, const 
     constexpr int NP = PLAIN ? 1 : 4;  // pencils per job
     constexpr int NR = 2 * NP;         // real right-hand sides
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int job = blockIdx.x * JPB + (w >> 1), op = w & 1;
+    // Batched jobs sharing a factor are adjacent in the list. The hardware deals
+    // consecutive blocks to the 8 XCDs in turn, so the grid is numbered to give
+    // each XCD a contiguous range of jobs: the jobs of one factor then run on one
+    // XCD back to back, and the second stream of the factor can hit that L2.
+    int blk = blockIdx.x;
+    if (!PLAIN) {
+        const int nb = (njobs + JPB - 1) / JPB, per = (nb + 7) / 8;
+        blk = (blockIdx.x % 8) * per + blockIdx.x / 8;
+        if (blk >= nb) return;
+    }
+    const int job = blk * JPB + (w >> 1), op = w & 1;
     const int N = c.NY;
     if (job >= njobs) return;
     int fac, pc[NP], kz, ix;
@@ -462,11 +472,16 @@ void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int 
     if (plain) {
         OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, true, 2>), dim3((njobs + 1) / 2), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
     } else if (sel == 1) {
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 1>), dim3(njobs), dim3(128), 0, s, c, sp, F, P, jobs, njobs));
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 1>), dim3(8 * ((njobs + 7) / 8)), dim3(128), 0, s, c, sp, F, P, jobs,
+                                                 njobs));
     } else if (sel == 2) {
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 4>), dim3((njobs + 3) / 4), dim3(512), 0, s, c, sp, F, P, jobs, njobs));
+        const int nb = (njobs + 3) / 4;
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 4>), dim3(8 * ((nb + 7) / 8)), dim3(512), 0, s, c, sp, F, P, jobs,
+                                                 njobs));
     } else {
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 2>), dim3((njobs + 1) / 2), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
+        const int nb = (njobs + 1) / 2;
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 2>), dim3(8 * ((nb + 7) / 8)), dim3(256), 0, s, c, sp, F, P, jobs,
+                                                 njobs));
     }
 }
 
