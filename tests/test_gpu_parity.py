@@ -323,3 +323,27 @@ def test_every_tuned_launch_shape_gives_identical_results(owt_mod, shape, precis
     got = run(ch.tune)
     assert all(np.array_equal(a, b) for a, b in zip(got, ref))
     ch.close()
+
+
+def test_orr_sommerfeld_eigenmode_on_gpu(owt_mod):
+    """Linear dynamics of the HIP step: from the laminar profile plus a 1e-6
+    Orr-Sommerfeld eigenmode (Re 1e4, alpha 1, built from the library's own DY
+    matrices), the mode grows and turns at the discrete eigenvalue to the scheme's
+    O(dt^3), in both methods."""
+    import orr_sommerfeld as os_
+
+    N, dt = 129, 0.01
+    p = dict(params(4, N, 4, nu=os_.NU, dkx=os_.ALPHA, dt=dt))
+    for method in (0, 1):
+        ch = gpu(owt_mod, p, method=method)
+        DY1, DY2, DY4 = ch.matrix("DY1"), ch.matrix("DY2"), ch.matrix("DY4")
+        s = 2.0 * np.arange(N) / (N - 1) - 1.0
+        y = np.tanh(p["alpha_mesh"] * s) / np.tanh(p["alpha_mesh"])
+        lam, phi = os_.least_stable(*os_.textbook_operator(y, DY1, DY2, DY4))
+        phi = phi / phi[np.argmax(np.abs(phi))]
+        ch.set_velocity(*os_.eigenmode_fields(ch.wave_shape, y, DY1, phi, 1e-6))
+        lm = os_.growth(lambda: ch.step(1), lambda: ch.field("v")[0, 1], phi, dt, 100, 600)
+        print(f"[os] method {method}: measured {lm:.10e} discrete {lam:.10e}", flush=True)
+        assert abs(lm.real - lam.real) < 2e-5 * abs(lam.real), (lm, lam)
+        assert abs(lm.imag - lam.imag) < 1e-6 * abs(lam.imag), (lm, lam)
+        ch.close()

@@ -14,6 +14,11 @@ So each part of the oracle is checked here against an independent fact:
   metric, DY_n    derivatives of tanh(a s)/tanh(a) and of smooth functions
   step            laminar Poiseuille balance fx_fric = 3 nu, Hermitian symmetry
                   of the kx = 0 plane, discrete divergence, seeded amplitude
+  linear dynamics Orr-Sommerfeld at Re 1e4, alpha 1: the step's Lamb-vector path
+                  linearized about the laminar profile equals the textbook operator
+                  on the same DY matrices; with DY4 = DY2^2 that operator gives
+                  Orszag's eigenvalue; the time-stepped eigenmode grows at the
+                  discrete eigenvalue
   pressure        (no reference counterpart) laminar P = u^2/2 - mean; a gradient
                   Lamb vector L = grad phi gives P = phi; wall data nu v'' alone
                   gives A cosh(ky) + B sinh(ky); each converges with NY
@@ -419,3 +424,88 @@ def test_seeded_gauss_is_standard_normal(oracle_mod):
     z = np.concatenate([g.real, g.imag])
     assert abs(z.mean()) < 0.06 and abs(z.var() - 1) < 0.08
     assert len(np.unique(np.round(z, 12))) == len(z)
+
+
+# ---------------------------------------------------------------------------
+# Orr-Sommerfeld (linear dynamics of the whole step)
+# ---------------------------------------------------------------------------
+def lamb_path_vs_textbook(oracle_mod, N):
+    import orr_sommerfeld as os_
+
+    a = os_.ALPHA
+    o = make(oracle_mod, 4, N, 4, nu=os_.NU, dkx=a)
+    y, DY1, DY2, DY4 = o.vector("y"), o.matrix("DY1"), o.matrix("DY2"), o.matrix("DY4")
+    shp = o.wave_shape
+    base = np.zeros((6,) + shp, dtype=complex)
+    U = os_.UC * (1 - y ** 2)
+    base[0, 0, 0] = U
+    base[5, 0, 0] = -(DY1 @ U)
+    H = np.zeros((N, N), dtype=complex)
+    for j in range(N):
+        e = np.zeros(N, dtype=complex)
+        e[j] = 1.0
+        p = np.zeros((6,) + shp, dtype=complex)
+        p[1, 0, 1] = e
+        p[0, 0, 1] = (1j / a) * (DY1 @ e)
+        p[5, 0, 1] = -(1j / a) * (DY2 @ e - a * a * e)
+        L = (o.lamb(base + 1e-3 * p) - o.lamb(base - 1e-3 * p)) / 2e-3
+        H[:, j] = -a * a * L[1, 0, 1] - DY1 @ (1j * a * L[0, 0, 1])
+    # the Lamb path forms U'' v as the commutator Omega_z DY1 v - DY1 (Omega_z v), a
+    # dense matrix, so the operators agree in their action on smooth fields and in
+    # their eigenvalues, not entry by entry
+    T = -1j * a * np.diag(U) @ (DY2 - a * a * np.eye(N)) + 1j * a * (-2 * os_.UC) * np.eye(N)
+    A, B = os_.textbook_operator(y, DY1, DY2, DY4)
+    A2 = H + os_.NU * (DY4 - 2 * a * a * DY2 + a ** 4 * np.eye(N))
+    B2 = DY2 - a * a * np.eye(N)
+    os_.boundary_rows(A2, B2, DY1)
+    l1, phi = os_.least_stable(A, B)
+    l2, _ = os_.least_stable(A2, B2)
+    r = np.linalg.norm(((H - T) @ phi)[2:N - 2]) / np.linalg.norm((T @ phi)[2:N - 2])
+    return r, abs(l1 - l2) / abs(l1)
+
+
+def test_orr_sommerfeld_lamb_path_is_the_textbook_operator(oracle_mod):
+    """The v equation's explicit operator as the oracle computes it (Lamb vector of
+    base + perturbation through the dealiased transforms, then Hv), linearized by
+    the exact +-eps difference of a bilinear map, against the textbook
+    -i a U (DY2 - a^2) + i a U'' on the same DY matrices. The Lamb path forms U'' v
+    as the commutator Omega_z DY1 v - DY1 (Omega_z v), so the two differ by that
+    commutator's discretisation error, which falls with NY; a sign or a term wrong
+    would leave an O(1) difference."""
+    r65, e65 = lamb_path_vs_textbook(oracle_mod, 65)
+    r129, e129 = lamb_path_vs_textbook(oracle_mod, 129)
+    print(f"[os] action {r65:.2e} {r129:.2e}, eigenvalue {e65:.2e} {e129:.2e}")
+    assert r65 < 1e-3 and e65 < 1e-6
+    assert r129 < r65 / 4 and e129 < e65 / 8
+
+
+def test_orr_sommerfeld_orszag_eigenvalue(oracle_mod):
+    """With DY4 = DY2^2 the operators reproduce Orszag's eigenvalue (the compact
+    DY2 on the stretched grid); the reference's own DY4, whose top-wall rows carry
+    the B3 overwrite (src/compdiff.hpp:257-258), moves it by about 1 %."""
+    import orr_sommerfeld as os_
+
+    o = make(oracle_mod, 4, 193, 4, nu=os_.NU, dkx=os_.ALPHA)
+    y, DY1, DY2, DY4 = o.vector("y"), o.matrix("DY1"), o.matrix("DY2"), o.matrix("DY4")
+    c_clean = os_.c_of(os_.least_stable(*os_.textbook_operator(y, DY1, DY2, DY2 @ DY2))[0])
+    c_ref = os_.c_of(os_.least_stable(*os_.textbook_operator(y, DY1, DY2, DY4))[0])
+    ref = os_.ORSZAG_C
+    assert abs(c_clean.real - ref.real) < 1e-5 * ref.real, c_clean
+    assert abs(c_clean.imag - ref.imag) < 1e-3 * ref.imag, c_clean
+    assert abs(c_ref.real - ref.real) > 5e-3 * ref.real, c_ref
+
+
+def test_orr_sommerfeld_eigenmode_grows_at_the_discrete_rate(oracle_mod):
+    """The oracle's time step from the laminar profile plus a 1e-6 eigenmode: the
+    mode grows and turns at the discrete eigenvalue to the scheme's O(dt^3)."""
+    import orr_sommerfeld as os_
+
+    N, dt = 65, 0.01
+    o = make(oracle_mod, 4, N, 4, nu=os_.NU, dkx=os_.ALPHA, dt=dt)
+    y, DY1, DY2, DY4 = o.vector("y"), o.matrix("DY1"), o.matrix("DY2"), o.matrix("DY4")
+    lam, phi = os_.least_stable(*os_.textbook_operator(y, DY1, DY2, DY4))
+    phi = phi / phi[np.argmax(np.abs(phi))]
+    o.set_velocity(*os_.eigenmode_fields(o.wave_shape, y, DY1, phi, 1e-6))
+    lm = os_.growth(lambda: o.step(1), lambda: o.field("v")[0, 1], phi, dt, 100, 600)
+    assert abs(lm.real - lam.real) < 2e-5 * abs(lam.real), (lm, lam)
+    assert abs(lm.imag - lam.imag) < 1e-6 * abs(lam.imag), (lm, lam)
