@@ -108,6 +108,22 @@ int owt_set_precision(owt_handle *h, int precision);
  * loaded at run time from $OWT_HDF5_LIB or libhdf5.so on the loader path;
  * OWT_E_IO when it cannot be loaded or the file cannot be written. */
 int owt_save(owt_handle *h, const char *dir);
+/* the same file with the dataset families the reference's writer has enabled
+ * or commented out (src/IO.cpp:237-278), by mask: OWT_SAVE_VEL (velx_real ...),
+ * OWT_SAVE_VOR (vorx_real ...), OWT_SAVE_LAMB (lmbx_real ...) in wave space;
+ * OWT_SAVE_VEL_P (velx_p ...), OWT_SAVE_VOR_P, OWT_SAVE_LAMB_P in physical space,
+ * [NZ2][NX2][NY] doubles (WRITE_3D_dealiased, src/IO.cpp:158-196).
+ * owt_save(h, dir) is owt_save_fields(h, dir, OWT_SAVE_VEL). */
+enum { OWT_SAVE_VEL = 1, OWT_SAVE_VOR = 2, OWT_SAVE_LAMB = 4, OWT_SAVE_VEL_P = 8, OWT_SAVE_VOR_P = 16, OWT_SAVE_LAMB_P = 32 };
+int owt_save_fields(owt_handle *h, const char *dir, int what);
+
+/* Plane statistics of the current field (no reference counterpart; the
+ * reference prints the friction line only): profiles[9][NY] = U, V, W and the
+ * plane averages <u'u'>, <v'v'>, <w'w'>, <u'v'>, <u'w'>, <v'w'> of the
+ * fluctuations, by Parseval over wave space (the z-Nyquist modes excluded, as
+ * the transforms to physical space drop them); re_tau = u_tau h / nu with
+ * u_tau^2 = fx_fric h, h the half height. */
+int owt_statistics(owt_handle *h, double *profiles, double *re_tau);
 
 /* full state: all history levels, mean-flow and friction history, step counter */
 int owt_checkpoint(owt_handle *h, const char *path);

@@ -319,6 +319,16 @@ int owt_save(owt_handle *h, const char *dir) {
     return guard(h, [&] { h->ch->save(dir); });
 }
 
+int owt_save_fields(owt_handle *h, const char *dir, int what) {
+    if (!dir) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->save_fields(dir, what); });
+}
+
+int owt_statistics(owt_handle *h, double *profiles, double *re_tau) {
+    if (!profiles || !re_tau) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->statistics(profiles, re_tau); });
+}
+
 int owt_checkpoint(owt_handle *h, const char *path) {
     if (!path) return OWT_E_ARG;
     return guard(h, [&] { h->ch->checkpoint(path); });

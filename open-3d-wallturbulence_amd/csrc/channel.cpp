@@ -803,29 +803,64 @@ struct CkptHeader {
 };
 }  // namespace
 
-// save_instant_field (src/IO.cpp:198-283): <dir>/F-<id_step, 8 digits>.H5 with
-// the real and imaginary parts of u, v, w as [NZ][NXH][NY] doubles
-void Channel::save(const std::string &dir) {
+// save_instant_field (src/IO.cpp:198-283): <dir>/F-<id_step, 8 digits>.H5. The
+// dataset families are the reference's, enabled (velocity) and commented out
+// (the rest) there: wave fields as real and imaginary [NZ][NXH][NY] doubles
+// (WRITE_3D, src/IO.cpp:118-157), physical fields as [NZ2][NX2][NY] doubles
+// (WRITE_3D_dealiased, src/IO.cpp:158-196), named <field>_real / _imag / _p.
+void Channel::save_fields(const std::string &dir, int what) {
+    if (what <= 0 || what >= 64) throw Error(2, "save: dataset mask must be in 1..63");
     const long id = steps_done();
     char name[40];
     std::snprintf(name, sizeof name, "F-%08ld.H5", id);
     const std::string path = dir.empty() ? std::string(name) : dir + "/" + name;
-    const size_t W = c_.W;
-    const long dims[3] = {c_.NZ, c_.NXH, c_.NY};
-    std::vector<double> z(2 * W), re(W), im(W);
+    const size_t W = c_.W, P = (size_t)c_.NZ2 * c_.NX2 * c_.NY;
+    const long dims[3] = {c_.NZ, c_.NXH, c_.NY}, pdims[3] = {c_.NZ2, c_.NX2, c_.NY};
+    std::vector<double> z(2 * W), re(W), im(W), ph;
     H5Writer f(path);
-    const struct { int id; const char *re, *im; } sets[] = {
-        {F_U, "velx_real", "velx_imag"}, {F_V, "vely_real", "vely_imag"}, {F_W, "velz_real", "velz_imag"}};
-    for (const auto &s : sets) {
-        get_field(s.id, z.data());
-        for (size_t e = 0; e < W; ++e) {
-            re[e] = z[2 * e];
-            im[e] = z[2 * e + 1];
+    struct Set { int id; const char *name; };
+    const Set fam[3][3] = {{{F_U, "velx"}, {F_V, "vely"}, {F_W, "velz"}},
+                           {{F_OMX, "vorx"}, {F_OMY, "vory"}, {F_OMZ, "vorz"}},
+                           {{F_LMBX, "lmbx"}, {F_LMBY, "lmby"}, {F_LMBZ, "lmbz"}}};
+    for (int g = 0; g < 3; ++g) {
+        const bool wave = (what >> g) & 1, phys = (what >> (3 + g)) & 1;
+        if (!wave && !phys) continue;
+        for (const Set &s : fam[g]) {
+            get_field(s.id, z.data());
+            if (wave) {
+                for (size_t e = 0; e < W; ++e) {
+                    re[e] = z[2 * e];
+                    im[e] = z[2 * e + 1];
+                }
+                f.write(std::string(s.name) + "_real", dims, re.data());
+                f.write(std::string(s.name) + "_imag", dims, im.data());
+            }
+            if (phys) {
+                ph.resize(P);
+                f2p(z.data(), 1, ph.data());
+                f.write(std::string(s.name) + "_p", pdims, ph.data());
+            }
         }
-        f.write(s.re, dims, re.data());
-        f.write(s.im, dims, im.data());
     }
     f.close();
+}
+
+void Channel::save(const std::string &dir) { save_fields(dir, 1); }
+
+// Plane statistics of the current level (k_statistics) and the friction Reynolds
+// number Re_tau = u_tau h / nu, u_tau^2 = fx_fric h, h = Ly / 2 (the friction
+// force balances the shear of both walls)
+void Channel::statistics(double *prof, double *re_tau) {
+    sync();
+    double *d = (double *)c_.zbuf;
+    launch_statistics(c_, c_.hist + (size_t)(H_V * 3 + ring_) * c_.W, d, s_);
+    OWT_HIP(hipGetLastError());
+    OWT_HIP(hipMemcpyAsync(prof, d, 9 * (size_t)c_.NY * sizeof(double), hipMemcpyDeviceToHost, s_));
+    OWT_HIP(hipStreamSynchronize(s_));
+    double fx, b, u;
+    wall_shear(&fx, &b, &u);
+    const double h = c_.Ly / 2.0;
+    *re_tau = std::sqrt(std::fabs(fx) * h) * h / c_.nu;
 }
 
 void Channel::checkpoint(const std::string &path) {

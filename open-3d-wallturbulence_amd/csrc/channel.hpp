@@ -39,8 +39,13 @@ class Channel {
     long steps_done();
     void set_method(int method);
     void set_precision(int precision);
-    // save_instant_field: HDF5 snapshot <dir>/F-<id_step>.H5 of u, v, w
+    // save_instant_field: HDF5 snapshot <dir>/F-<id_step>.H5 of u, v, w; save_fields
+    // with a dataset mask (1 velocity, 2 vorticity, 4 Lamb vector in wave space;
+    // 8, 16, 32 the same in physical space)
     void save(const std::string &dir);
+    void save_fields(const std::string &dir, int what);
+    // plane statistics: prof[9][NY] = U, V, W, <u'u'>, <v'v'>, <w'w'>, <u'v'>, <u'w'>, <v'w'>
+    void statistics(double *prof, double *re_tau);
     void checkpoint(const std::string &path);
     void restore(const std::string &path);
 

@@ -139,6 +139,9 @@ void launch_nonlinear_adj(const Chan &c, const StepPtrs &sp, hipStream_t s);
 // gradient with respect to the initial u, v, w
 void launch_init_adj_mean_history(const Chan &c, hipStream_t s);
 void launch_init_vel_adj(const Chan &c, double2 *gu, double2 *gv, double2 *gw, hipStream_t s);
+// plane statistics of the current level (v: its v slot) into out[9][NY]:
+// U, V, W, <u'u'>, <v'v'>, <w'w'>, <u'v'>, <u'w'>, <v'w'>
+void launch_statistics(const Chan &c, const double2 *v, double *out, hipStream_t s);
 // dst[f] += src[f], f < nf (<= 6), n complex values each
 void launch_accumulate(double2 *const *dst, double2 *const *src, int nf, size_t n, hipStream_t s);
 

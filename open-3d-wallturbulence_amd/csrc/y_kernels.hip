@@ -1043,6 +1043,48 @@ __global__ void k_accumulate(AccTable t, int nf, size_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// Plane statistics of the current level by Parseval over wave space (no
+// reference counterpart; the reference prints the friction line only). For each
+// y: the means U, V, W (the k = 0 pencil) and the plane averages of u'u', v'v',
+// w'w', u'v', u'w', v'w' over every other pencil, weight 2 for kx > 0 (its
+// conjugate partner is implicit) and 1 on the kx = 0 plane, the z-Nyquist
+// pencils left out as the transforms to physical space leave them out.
+// One block per y; out[q * NY + y] for q = 0..8.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_statistics(Chan c, const double2 *v, double *out) {
+    __shared__ double red[6][256];
+    const int N = c.NY, y = blockIdx.x;
+    const double2 *u = c.cur, *w = c.cur + c.W;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int p = 1 + threadIdx.x; p < c.npencil; p += blockDim.x) {
+        const int k = p / c.NXH, i = p % c.NXH;
+        if (k == c.NZH) continue;
+        const double wt = i == 0 ? 1.0 : 2.0;
+        const size_t o = (size_t)p * N + y;
+        const double2 a = u[o], b = v[o], d = w[o];
+        s[0] += wt * (a.x * a.x + a.y * a.y);
+        s[1] += wt * (b.x * b.x + b.y * b.y);
+        s[2] += wt * (d.x * d.x + d.y * d.y);
+        s[3] += wt * (a.x * b.x + a.y * b.y);
+        s[4] += wt * (a.x * d.x + a.y * d.y);
+        s[5] += wt * (b.x * d.x + b.y * d.y);
+    }
+    for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0 * N + y] = u[y].x;
+        out[1 * N + y] = v[y].x;
+        out[2 * N + y] = w[y].x;
+        for (int q = 0; q < 6; ++q) out[(3 + q) * N + y] = red[q][0];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
 constexpr int kTP = 8;
@@ -1122,6 +1164,10 @@ void launch_accumulate(double2 *const *dst, double2 *const *src, int nf, size_t 
         t.src[f] = src[f];
     }
     hipLaunchKernelGGL(k_accumulate, dim3(1024), dim3(256), 0, s, t, nf, n);
+}
+
+void launch_statistics(const Chan &c, const double2 *v, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_statistics, dim3(c.NY), dim3(256), 0, s, c, v, out);
 }
 
 void launch_vor_from_vel(const Chan &c, hipStream_t s) {

@@ -38,7 +38,7 @@ EXPORTS = [
     "owt_ensemble_create", "owt_ensemble_destroy", "owt_ensemble_last_error", "owt_ensemble_member",
     "owt_ensemble_init_fields", "owt_ensemble_step", "owt_ensemble_time_steps",
     "owt_tangent_init", "owt_tangent_handle", "owt_tangent_step", "owt_adjoint_run",
-    "owt_autotune", "owt_get_tune", "owt_set_tune",
+    "owt_autotune", "owt_get_tune", "owt_set_tune", "owt_save_fields", "owt_statistics",
 ]
 
 
@@ -90,6 +90,8 @@ def lib():
         L.owt_set_method.argtypes = [P, I]
         L.owt_set_precision.argtypes = [P, I]
         L.owt_save.argtypes = [P, ctypes.c_char_p]
+        L.owt_save_fields.argtypes = [P, ctypes.c_char_p, I]
+        L.owt_statistics.argtypes = [P, P, P]
         L.owt_checkpoint.argtypes = [P, ctypes.c_char_p]
         L.owt_restore.argtypes = [P, ctypes.c_char_p]
         L.owt_f2p.argtypes = [P, P, I, P]
@@ -224,6 +226,21 @@ class Channel:
         returns its path."""
         self._check(lib().owt_save(self._h, os.fsencode(directory)))
         return os.path.join(directory, "F-%08d.H5" % self.steps_done())
+
+    def save_fields(self, directory="DATA", what=1):
+        """Snapshot with a dataset mask (1 velocity, 2 vorticity, 4 Lamb vector in
+        wave space; 8, 16, 32 the same in physical space); returns its path."""
+        self._check(lib().owt_save_fields(self._h, os.fsencode(directory), int(what)))
+        return os.path.join(directory, "F-%08d.H5" % self.steps_done())
+
+    def statistics(self):
+        """Plane statistics: dict of NY-profiles U, V, W, uu, vv, ww, uv, uw, vw, and Re_tau."""
+        prof = np.empty((9, self.NY), dtype=np.float64)
+        rt = ctypes.c_double()
+        self._check(lib().owt_statistics(self._h, _p(prof), ctypes.byref(rt)))
+        out = dict(zip(("U", "V", "W", "uu", "vv", "ww", "uv", "uw", "vw"), prof))
+        out["re_tau"] = rt.value
+        return out
 
     def checkpoint(self, path):
         self._check(lib().owt_checkpoint(self._h, os.fsencode(path)))

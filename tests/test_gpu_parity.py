@@ -347,3 +347,54 @@ def test_orr_sommerfeld_eigenmode_on_gpu(owt_mod):
         assert abs(lm.real - lam.real) < 2e-5 * abs(lam.real), (lm, lam)
         assert abs(lm.imag - lam.imag) < 1e-6 * abs(lam.imag), (lm, lam)
         ch.close()
+
+
+def test_statistics_match_physical_space_averages(owt_mod):
+    """Plane statistics by Parseval over wave space equal the plane averages of the
+    physical fields (the library's dealiased transforms), which fixes the weights
+    (2 for kx > 0, 1 on the kx = 0 plane) and the z-Nyquist exclusion; the laminar
+    profile gives U = 1.5 (1 - y^2), no stresses and Re_tau = sqrt(3 nu) / nu."""
+    p = params(16, 33, 16)
+    ch = gpu(owt_mod, p)
+    ch.init_fields(2, seed=4, amp=0.3)
+    ch.step(3)
+    st = ch.statistics()
+    P = ch.f2p(np.stack([ch.field(n) for n in ("u", "v", "w")]))
+    m = P.mean(axis=(1, 2))
+    f = P - m[:, None, None, :]
+    ref = {"U": m[0], "V": m[1], "W": m[2], "uu": (f[0] * f[0]).mean(axis=(0, 1)), "vv": (f[1] * f[1]).mean(axis=(0, 1)),
+           "ww": (f[2] * f[2]).mean(axis=(0, 1)), "uv": (f[0] * f[1]).mean(axis=(0, 1)),
+           "uw": (f[0] * f[2]).mean(axis=(0, 1)), "vw": (f[1] * f[2]).mean(axis=(0, 1))}
+    # relative to each family's size: the means to |U|, the stresses to <u'u'>
+    for k, r in ref.items():
+        scale = np.abs(ref["U"]).max() if k in ("U", "V", "W") else ref["uu"].max()
+        assert np.abs(st[k] - r).max() <= 1e-12 * scale, k
+    assert st["uu"].max() > 1e-4
+    ch.init_fields(1)
+    lam = ch.statistics()
+    s = 2.0 * np.arange(33) / 32 - 1.0
+    y = np.tanh(1.5 * s) / np.tanh(1.5)
+    assert np.abs(lam["U"] - 1.5 * (1 - y ** 2)).max() < 1e-12
+    fx = ch.wall_shear()[0]
+    assert abs(lam["re_tau"] - np.sqrt(fx) / p["nu"]) < 1e-12 * lam["re_tau"]
+    assert abs(lam["re_tau"] - np.sqrt(3 * p["nu"]) / p["nu"]) < 1e-3 * lam["re_tau"]
+    ch.close()
+
+
+def test_save_fields_writes_every_dataset_family(owt_mod, tmp_path):
+    import h5read
+
+    if h5read.lib() is None:
+        pytest.skip("HDF5 C library not available to read the snapshot back")
+    ch = gpu(owt_mod, params(16, 17, 16))
+    ch.init_fields(2, seed=0, amp=0.15)
+    ch.step(2)
+    path = ch.save_fields(str(tmp_path), 63)
+    for name, key in (("u", "velx"), ("omy", "vory"), ("lmbz", "lmbz")):
+        f = ch.field(name)
+        assert np.array_equal(h5read.read(path, key + "_real"), f.real), key
+        assert np.array_equal(h5read.read(path, key + "_imag"), f.imag), key
+        ph = h5read.read(path, key + "_p")
+        assert ph.shape == ch.phys_shape
+        assert np.array_equal(ph, ch.f2p(f)), key
+    ch.close()
