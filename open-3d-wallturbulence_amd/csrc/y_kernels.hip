@@ -139,68 +139,75 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
 #pragma unroll
             for (int q = 0; q < 3; ++q) acc[rt][ct][q] = owt_d4{0.0, 0.0, 0.0, 0.0};
     const int am = lane & 15, ak = lane >> 4, bq = (lane & 15) >> 1, bc = lane & 1;
-    // DY operands of k step k (this lane's row and k), zero outside the matrix
+    // DY operands of k step k for this lane's row. Indices are clamped into the
+    // matrix instead of masked: staged columns past N are zero, so a clamped k
+    // adds nothing, and rows past N are never stored. The loads are therefore
+    // unconditional, which keeps the count of loads in flight static.
     auto loadA = [&](int k, double (&a1)[RT], double (&a2)[RT]) {
+        const size_t kr = (size_t)(k < N ? k : N - 1) * N;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) {
             const int row = (w + NW * rt) * 16 + am;
-            const bool in = row < N && k < N;
-            const size_t e = (size_t)(k < N ? k : 0) * N + (row < N ? row : 0);
-            a1[rt] = in ? c.DY1[e] : 0.0;
-            a2[rt] = in ? c.DY2[e] : 0.0;
+            const size_t e = kr + (row < N ? row : N - 1);
+            a1[rt] = c.DY1[e];
+            a2[rt] = c.DY2[e];
         }
     };
-    // operands of the current k step and of the next PF - 1 steps, in flight
-    constexpr int PF = 3;
+    // the next chunk's staged element, fetched into registers a chunk ahead of
+    // its LDS write so that the write waits only for loads older than the ring's
+    const size_t so = (size_t)(sp_ < c.npencil ? sp_ : c.npencil - 1) * N;
+    double2 nv, ng;
+    auto fetch = [&](int k0) {
+        const int k = k0 + skk < N ? k0 + skk : N - 1;
+        nv = sp.vn[so + k];
+        ng = sp.gn[so + k];
+    };
+    // a ring of PF slots of DY operands: k step t uses slot t % PF, loaded PF
+    // steps earlier, and reloads it for step t + PF. A chunk's kKC / 4 steps
+    // (a multiple of PF) are unrolled, so every slot is a fixed register set
+    // (a register move of an in-flight load would wait for it).
+    constexpr int PF = 4;
+    static_assert((kKC / 4) % PF == 0, "chunks hold whole rounds of the operand ring");
     double q1[PF][RT], q2[PF][RT];
+    fetch(0);
+    __builtin_amdgcn_sched_barrier(0);  // the ring's loads stay younger than the fetch
 #pragma unroll
     for (int d = 0; d < PF; ++d) loadA(4 * d + ak, q1[d], q2[d]);
-    int kstep = 0;
-    for (int k0 = 0; k0 < N; k0 += kKC) {
-        {
-            const int k = k0 + skk;
-            double2 a = c2(0, 0), b = c2(0, 0);
-            if (sp_ < c.npencil && k < N) {
-                a = sp.vn[(size_t)sp_ * N + k];
-                b = sp.gn[(size_t)sp_ * N + k];
-            }
-            sv[sq][skk] = a;
-            sg[sq][skk] = b;
-        }
+    // the chunk loop is unrolled too (at most kMaxNY / kKC chunks): across a
+    // loop back edge the compiler's wait counting would drain the ring
+#pragma unroll
+    for (int ci = 0; ci < kMaxNY / kKC; ++ci) {
+        const int k0 = ci * kKC;
+        if (k0 >= N) break;
+        const bool live = sp_ < c.npencil && k0 + skk < N;
+        sv[sq][skk] = live ? nv : c2(0, 0);
+        sg[sq][skk] = live ? ng : c2(0, 0);
         __syncthreads();
-        const int kend = N - k0 < kKC ? N - k0 : kKC;
-        for (int kk = 0; kk < kend; kk += 4, ++kstep) {
-            double a1[RT], a2[RT];
+        fetch(k0 + kKC);
+        const int nst = ((N - k0 < kKC ? N - k0 : kKC) + 3) / 4;  // k steps in this chunk
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-                a1[rt] = q1[0][rt];
-                a2[rt] = q2[0][rt];
-            }
-#pragma unroll
-            for (int d = 0; d + 1 < PF; ++d)
-#pragma unroll
-                for (int rt = 0; rt < RT; ++rt) {
-                    q1[d][rt] = q1[d + 1][rt];
-                    q2[d][rt] = q2[d + 1][rt];
-                }
-            loadA(4 * (kstep + PF) + ak, q1[PF - 1], q2[PF - 1]);
-            double bv[2], bg[2];
-#pragma unroll
-            for (int ct = 0; ct < 2; ++ct) {
-                const double2 x = sv[ct * 8 + bq][kk + ak], y = sg[ct * 8 + bq][kk + ak];
-                bv[ct] = bc ? x.y : x.x;
-                bg[ct] = bc ? y.y : y.x;
-            }
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-                if (w + NW * rt >= ntile) break;
+        for (int st = 0; st < kKC / 4; ++st) {
+            const int d = st % PF, kk = 4 * st;
+            if (st < nst) {
+                double bv[2], bg[2];
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) {
-                    acc[rt][ct][0] = mfma4(a1[rt], bv[ct], acc[rt][ct][0]);
-                    acc[rt][ct][1] = mfma4(a2[rt], bv[ct], acc[rt][ct][1]);
-                    acc[rt][ct][2] = mfma4(a1[rt], bg[ct], acc[rt][ct][2]);
+                    const double2 x = sv[ct * 8 + bq][kk + ak], y = sg[ct * 8 + bq][kk + ak];
+                    bv[ct] = bc ? x.y : x.x;
+                    bg[ct] = bc ? y.y : y.x;
+                }
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    if (w + NW * rt >= ntile) break;
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        acc[rt][ct][0] = mfma4(q1[d][rt], bv[ct], acc[rt][ct][0]);
+                        acc[rt][ct][1] = mfma4(q2[d][rt], bv[ct], acc[rt][ct][1]);
+                        acc[rt][ct][2] = mfma4(q1[d][rt], bg[ct], acc[rt][ct][2]);
+                    }
                 }
             }
+            loadA(k0 + kk + 4 * PF + ak, q1[d], q2[d]);
         }
         __syncthreads();
     }
@@ -243,7 +250,8 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
 // k_horizontal_mfma.
 template <typename LC>
 __global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
-    constexpr int RT = 2, NW = 8, PF = 3;
+    constexpr int RT = 2, NW = 8, PF = 4;
+    static_assert((kKC / 4) % PF == 0, "chunks hold whole rounds of the operand ring");
     __shared__ double2 st[kMP][kKC + 1];
     const int N = c.NY, ns = sp.sn, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const LC *lx = reinterpret_cast<const LC *>(c.lmb);
@@ -256,12 +264,21 @@ __global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
         skx = c.kx[spn % c.NXH];
         skz = c.kz[spn / c.NXH];
     }
+    // the next chunk's Lamb components, fetched into registers a chunk ahead of
+    // staging (indices clamped; the staging discards what lies outside)
+    const size_t so = (size_t)(spn < c.npencil ? spn : c.npencil - 1) * N;
+    LC na, nb;
+    auto fetch = [&](int k0) {
+        const int k = k0 + skk < N ? k0 + skk : N - 1;
+        na = lx[so + k];
+        nb = lz[so + k];
+    };
     auto stage = [&](int k0) {
         const int k = k0 + skk;
         double2 t = c2(0, 0);
         if (spn < c.npencil && k < N) {
-            const size_t o = (size_t)spn * N + k;
-            const double2 a = ld2(lx + o), b = ld2(lz + o);
+            const size_t o = so + k;
+            const double2 a = c2(na.x, na.y), b = c2(nb.x, nb.y);
             t = caddc(imul(skx, a), imul(skz, b));
             sp.hgn[o] = csubc(imul(skz, a), imul(skx, b));
             if (spn == 0) {
@@ -277,43 +294,46 @@ __global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
 #pragma unroll
         for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = owt_d4{0.0, 0.0, 0.0, 0.0};
     const int am = lane & 15, ak = lane >> 4, bq = (lane & 15) >> 1, bc = lane & 1;
+    // clamped, unconditional operand loads as in k_horizontal_mfma
     auto loadA = [&](int k, double (&a1)[RT]) {
+        const size_t kr = (size_t)(k < N ? k : N - 1) * N;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) {
             const int row = (w + NW * rt) * 16 + am;
-            const bool in = row < N && k < N;
-            a1[rt] = in ? c.DY1[(size_t)(k < N ? k : 0) * N + (row < N ? row : 0)] : 0.0;
+            a1[rt] = c.DY1[kr + (row < N ? row : N - 1)];
         }
     };
     double q1[PF][RT];
+    fetch(0);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int d = 0; d < PF; ++d) loadA(4 * d + ak, q1[d]);
-    int kstep = 0;
-    for (int k0 = 0; k0 < N; k0 += kKC) {
+#pragma unroll
+    for (int ci = 0; ci < kMaxNY / kKC; ++ci) {
+        const int k0 = ci * kKC;
+        if (k0 >= N) break;
         stage(k0);
         __syncthreads();
-        const int kend = N - k0 < kKC ? N - k0 : kKC;
-        for (int kk = 0; kk < kend; kk += 4, ++kstep) {
-            double a1[RT];
+        fetch(k0 + kKC);
+        const int nst = ((N - k0 < kKC ? N - k0 : kKC) + 3) / 4;
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) a1[rt] = q1[0][rt];
+        for (int stp = 0; stp < kKC / 4; ++stp) {
+            const int d = stp % PF, kk = 4 * stp;
+            if (stp < nst) {
+                double bt[2];
 #pragma unroll
-            for (int d = 0; d + 1 < PF; ++d)
+                for (int ct = 0; ct < 2; ++ct) {
+                    const double2 x = st[ct * 8 + bq][kk + ak];
+                    bt[ct] = bc ? x.y : x.x;
+                }
 #pragma unroll
-                for (int rt = 0; rt < RT; ++rt) q1[d][rt] = q1[d + 1][rt];
-            loadA(4 * (kstep + PF) + ak, q1[PF - 1]);
-            double bt[2];
+                for (int rt = 0; rt < RT; ++rt) {
+                    if (w + NW * rt >= ntile) break;
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct) {
-                const double2 x = st[ct * 8 + bq][kk + ak];
-                bt[ct] = bc ? x.y : x.x;
+                    for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = mfma4(q1[d][rt], bt[ct], acc[rt][ct]);
+                }
             }
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt) {
-                if (w + NW * rt >= ntile) break;
-#pragma unroll
-                for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = mfma4(a1[rt], bt[ct], acc[rt][ct]);
-            }
+            loadA(k0 + kk + 4 * PF + ak, q1[d]);
         }
         __syncthreads();
     }
