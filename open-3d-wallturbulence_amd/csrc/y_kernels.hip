@@ -126,7 +126,9 @@ __global__ __launch_bounds__(256) void k_horizontal(Chan c, StepPtrs sp) {
 // before issuing the products of the current one.
 __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
     constexpr int RT = 2, NW = 8;
-    __shared__ double2 sv[kMP][kKC + 1], sg[kMP][kKC + 1];
+    // the block's whole columns of v and omega_y (2 x 16 x 257 x 16 B = 131.6 KB
+    // of the 160 KB per CU), staged chunk by chunk and read again by the epilogue
+    __shared__ double2 sv[kMP][kMaxNY + 1], sg[kMP][kMaxNY + 1];
     const int N = c.NY, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int p0 = blockIdx.x * kMP, ntile = (N + 15) / 16;
     owt_d4 acc[RT][2][3];
@@ -180,8 +182,8 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
         const int k0 = ci * kKC;
         if (k0 >= N) break;
         const bool live = sp_ < c.npencil && k0 + skk < N;
-        sv[sq][skk] = live ? nv : c2(0, 0);
-        sg[sq][skk] = live ? ng : c2(0, 0);
+        sv[sq][k0 + skk] = live ? nv : c2(0, 0);
+        sg[sq][k0 + skk] = live ? ng : c2(0, 0);
         __syncthreads();
         fetch(k0 + kKC);
         const int nst = ((N - k0 < kKC ? N - k0 : kKC) + 3) / 4;  // k steps in this chunk
@@ -192,7 +194,7 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
                 double bv[2], bg[2];
 #pragma unroll
                 for (int ct = 0; ct < 2; ++ct) {
-                    const double2 x = sv[ct * 8 + bq][kk + ak], y = sg[ct * 8 + bq][kk + ak];
+                    const double2 x = sv[ct * 8 + bq][k0 + kk + ak], y = sg[ct * 8 + bq][k0 + kk + ak];
                     bv[ct] = bc ? x.y : x.x;
                     bg[ct] = bc ? y.y : y.x;
                 }
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
             }
             loadA(k0 + kk + 4 * PF + ak, q1[d], q2[d]);
         }
-        __syncthreads();
+        // no barrier here: the next chunk is staged into rows no wave reads yet
     }
     double2 *u = c.cur, *wv = c.cur + c.W, *ox = c.cur + 2 * c.W, *oz = c.cur + 3 * c.W;
 #pragma unroll
@@ -232,7 +234,7 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
                 const double k2 = k2_of(c, k, i);
                 if (k2 < 1.e-8) continue;  // the mean pencil is set by the mean-flow kernels
                 const double ix = __ddiv_rn(kx, k2), iz = __ddiv_rn(kz, k2), ir = __ddiv_rn(1.0, k2);
-                const double2 vv = sp.vn[o], gg = sp.gn[o];
+                const double2 vv = sv[ct * 8 + bq][row], gg = sg[ct * 8 + bq][row];
                 u[o] = csubc(imul(ix, d1v), imul(iz, gg));
                 wv[o] = caddc(imul(ix, gg), imul(iz, d1v));
                 const double2 X = csubc(caddc(cscale(d1g, kx), cscale(d2v, kz)), cscale(vv, mul(kz, k2)));
