@@ -247,7 +247,11 @@ void Channel::get_factor(int sys, int op, double *lu, int *perm) {
     if (sys < 0 || sys >= c_.nsys || op < 0 || op > 1) throw Error(2, "factor index out of range");
     factor_operators();
     const size_t N = c_.NY, f = (size_t)fmap_host_[sys] * 2 + op;
-    OWT_HIP(hipMemcpy(lu, Fb_ + f * N * N, N * N * sizeof(double), hipMemcpyDeviceToHost));
+    // the device keeps the packed layout of lu_at; hand out dgetrf's column-major one
+    std::vector<double> packed(N * N);
+    OWT_HIP(hipMemcpy(packed.data(), Fb_ + f * N * N, N * N * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t col = 0; col < N; ++col)
+        for (size_t row = 0; row < N; ++row) lu[col * N + row] = packed[lu_at((int)N, (int)row, (int)col)];
     OWT_HIP(hipMemcpy(perm, Pb_ + f * N, N * sizeof(int), hipMemcpyDeviceToHost));
 }
 

@@ -12,6 +12,22 @@ namespace owt {
 constexpr int kMaxStages = 8;
 constexpr int kMaxNY = 256;
 
+// Stored LU factors use N*N doubles in a packed layout: first the strictly lower
+// part of every column (rows k+1..N-1 of column k, columns in order), then the
+// upper part with the diagonal (rows 0..k of column k, columns in order). The
+// forward substitution then streams the first region and the back substitution
+// the second, each cache line fetched once; with plain column-major storage the
+// line holding a column's diagonal and the line shared by two columns are
+// fetched by both passes.
+__host__ __device__ __forceinline__ size_t lu_lower_at(int N, int k) { return (size_t)k * (2 * N - k - 1) / 2; }
+__host__ __device__ __forceinline__ size_t lu_upper_at(int N, int k) {
+    return (size_t)N * (N - 1) / 2 + (size_t)k * (k + 1) / 2;
+}
+// element (row, col) of a packed factor
+__host__ __device__ __forceinline__ size_t lu_at(int N, int row, int col) {
+    return row > col ? lu_lower_at(N, col) + (row - col - 1) : lu_upper_at(N, col) + row;
+}
+
 // 1-D transform plan: length L = prod(radix[s]), radices in {2, 3, 4}.
 // Stage s has span ns[s] (product of the earlier radices), twiddle stride
 // tstep[s] = L / (ns[s] * radix[s]) and mns[s] = ceil(2^32 / ns[s]), with which

@@ -13,7 +13,7 @@
 //           back substitution with the stored factors (dgetrs).
 //
 // A factor set holds, for factor index f and operator op (0 = Av, 1 = Ag),
-// N*N doubles at F + (2f + op) N^2 (column-major) and N pivot rows at
+// N*N doubles at F + (2f + op) N^2 (packed: lu_at in device.hpp) and N pivot rows at
 // P + (2f + op) N. Factor f is the operator of a representative index
 // s = frep[f] (or s = f when frep is null), whose s / NXH and s % NXH index kz
 // and kx. The method decides what the factors are:
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, c
 #pragma unroll
         for (int b = 0; b < CPT; ++b) {
             const int col = w + TC * b;
-            if (col < N) Fo[(size_t)col * N + rk] = M[a][b];
+            if (col < N) Fo[lu_at(N, rk, col)] = M[a][b];
         }
     }
 }
@@ -376,11 +376,12 @@ __global__ __launch_bounds__(256) void k_psolve(Chan c, const double *F, const i
 __global__ __launch_bounds__(256) void k_factor_transpose(double *F, int N) {
     double *M = F + (size_t)blockIdx.y * N * N;
     const int k = blockIdx.x;
-    const double d = M[(size_t)k * N + k];
+    const double d = M[lu_at(N, k, k)];
     for (int j = k + 1 + threadIdx.x; j < N; j += blockDim.x) {
-        const double l = M[(size_t)k * N + j], u = M[(size_t)j * N + k];
-        M[(size_t)k * N + j] = u / d;
-        M[(size_t)j * N + k] = d * l;
+        const size_t jk = lu_at(N, j, k), kj = lu_at(N, k, j);  // L[j,k] and U[k,j]
+        const double l = M[jk], u = M[kj];
+        M[jk] = u / d;
+        M[kj] = d * l;
     }
 }
 

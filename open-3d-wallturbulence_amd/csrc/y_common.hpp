@@ -57,15 +57,15 @@ __device__ __forceinline__ void lu_gather(const double (*S)[64 * NQ], const int 
 }
 
 // Forward substitution with the unit lower factor, then back substitution with
-// the upper factor (dtrsm as called by dgetrs, src/linalg.hpp:660-680). The
-// factor is N x N column-major.
+// the upper factor (dtrsm as called by dgetrs, src/linalg.hpp:660-680), on a
+// packed factor.
 //  - Steps run block by block: step k belongs to row block qk = k / 64, a
 //    compile-time index, so the pivot value is read from a fixed register of
 //    lane k % 64 and only the row blocks that meet the triangle are touched.
 //  - Columns come in chunks of CK, and the loads of the next chunk are issued
 //    before the current one is used (two buffers), so a wave always has factor
 //    loads in flight. Every lane of a touched block loads: a lane whose row lies
-//    outside the triangle reads the column's diagonal element instead (one
+//    outside the triangle reads an element of the column's own run instead (one
 //    cache line for all such lanes) and its value is masked, so no load sits
 //    behind a branch and only the triangle is fetched.
 //  - The back substitution scales by the reciprocal of the diagonal, as
@@ -75,12 +75,14 @@ __device__ __forceinline__ void lu_load_lower(const double *__restrict__ Fo, int
 #pragma unroll
     for (int c = 0; c < CK; ++c) {
         const int kc = kb + c < N ? kb + c : N - 1;
-        const double *col = Fo + (size_t)kc * N;
+        // rows kc+1..N-1 of column kc (for kc = N-1 an empty run: the masked
+        // lanes read the first upper element, still inside the factor)
+        const double *col = Fo + lu_lower_at(N, kc);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             if (q < qk) continue;
             const int row = lane + 64 * q;
-            cb[c][q] = col[(row > kc && row < N) ? row : kc];
+            cb[c][q] = col[(row > kc && row < N) ? row - kc - 1 : 0];
         }
     }
 }
@@ -138,7 +140,7 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
 #pragma unroll
             for (int c = 0; c < CK; ++c) {
                 const int kc = k0 - c >= lo ? k0 - c : lo;
-                const double *col = Fo + (size_t)kc * N;
+                const double *col = Fo + lu_upper_at(N, kc);  // rows 0..kc
 #pragma unroll
                 for (int q = 0; q <= qk; ++q) {
                     const int row = lane + 64 * q;
