@@ -151,8 +151,9 @@ int owt_get_factor(owt_handle *h, int sys, int op, double *lu, int *perm);
 /* Launch-shape autotuner (no reference counterpart): times every candidate
  * shape of each tunable launch on the handle's own data (reps launches each),
  * keeps the fastest and recaptures the step graphs. Knobs (OWT_TUNE_COUNT ints,
- * candidate index 0..3, 0 = default): solve jobs per block, pencils per block
- * of the horizontal and the nonlinear kernels, tile shape of the z inverse, x
+ * candidate index 0..3, 0 = default): solve jobs per block, the horizontal and
+ * the nonlinear kernels (one shape each, the matrix-core kernels, so these two
+ * slots accept any index and are not timed), tile shape of the z inverse, x
  * and z forward transforms. Every candidate computes the same values in the
  * same order, so results do not depend on the choice. report (may be NULL)
  * receives a JSON summary of the timings. */

@@ -1138,7 +1138,7 @@ std::string Channel::autotune(int reps) {
     static const char *names[T_COUNT] = {"solve_jobs_per_block", "horizontal_pencils_per_block", "nonlinear_pencils_per_block",
                                          "z_inverse_tile", "x_tile", "z_forward_tile"};
     static const char *shapes[T_COUNT][kTuneCandidates] = {
-        {"2", "1", "4", ""}, {"4", "2", "8", ""}, {"8", "4", "16", ""}, {"default", "alt1", "alt2", "alt3"},
+        {"2", "1", "4", ""}, {"mfma16", "", "", ""}, {"mfma16", "", "", ""}, {"default", "alt1", "alt2", "alt3"},
         {"default", "alt1", "alt2", "alt3"}, {"default", "alt1", "alt2", "alt3"}};
     // stage kernel of each knob (enqueue_kernel ids)
     static const int kernel[T_COUNT] = {0, 1, 6, 3, 4, 5};
@@ -1158,8 +1158,8 @@ std::string Channel::autotune(int reps) {
             else if (L > 576) ncand = 2;
         }
         if (k == T_SOLVE_JOBS && P_.method == 0) ncand = 1;  // the plain solve has one shape
-        // up to 256 points in y these two stages run on the matrix-core kernels, which have one shape
-        if ((k == T_HORIZONTAL_TP || k == T_NONLINEAR_TP) && c_.NY <= 256) ncand = 1;
+        // the derivative stages run on the matrix-core kernels, which have one shape
+        if (k == T_HORIZONTAL_TP || k == T_NONLINEAR_TP) ncand = 1;
         double best = 1e300;
         int arg = 0;
         std::string times;

@@ -27,98 +27,21 @@ constexpr int JC = 8;
 typedef double owt_d4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
-// Dense wall-normal derivatives on the matrix cores (both precisions, NY <= 256;
-// the row-per-thread kernels above serve larger NY). Across pencils the mat-vecs
-// form one GEMM,
+// Dense wall-normal derivatives on the matrix cores (both precisions, every
+// NY <= kMaxNY). Across pencils the mat-vecs form one GEMM,
 // Y (N x 2P) = DY (N x N) X (N x 2P), the columns of X being the pencils' real
 // and imaginary parts. A block takes kMP = 16 pencils (32 columns) and every
 // row, so each DY element read from L2 serves 32 columns; the pencils are staged
 // in LDS kKC rows at a time. v_mfma_f64_16x16x4f64 per wave: A = DY rows x 4 k
 // (lane l: row l%16, k l/16), B = 4 k x 16 columns (lane l: k l/16, column
 // l%16), D = 16 x 16 (lane l, register r: row 4r + l/16, column l%16). Wave w
-// owns the row tiles w, w+4, w+8, ...; column c of a column tile is pencil
+// of eight owns the row tiles w and w + 8; column c of a column tile is pencil
 // c/2, real (c even) or imaginary part, and the two parts meet by a lane swap.
 // ---------------------------------------------------------------------------
 constexpr int kMP = 16, kKC = 32;  // kMP * kKC = 512, the block size of both kernels
 static_assert(kMP * kKC == 512, "one staged element per thread");
 
 __device__ __forceinline__ owt_d4 mfma4(double a, double b, owt_d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-
-
-// ---------------------------------------------------------------------------
-// horizontal velocity and vorticity (src/solver.cpp:516-539)
-// ---------------------------------------------------------------------------
-template <int TP>
-__global__ __launch_bounds__(256) void k_horizontal(Chan c, StepPtrs sp) {
-    __shared__ double2 sv[TP][kMaxNY];
-    __shared__ double2 sg[TP][kMaxNY];
-    const int N = c.NY;
-    const double2 *v = sp.vn, *g = sp.gn;
-    const int p0 = blockIdx.x * TP;
-    for (int e = threadIdx.x; e < TP * N; e += blockDim.x) {
-        const int pp = e / N, j = e % N, p = p0 + pp;
-        double2 a = c2(0, 0), b = c2(0, 0);
-        if (p < c.npencil) {
-            a = v[(size_t)p * N + j];
-            b = g[(size_t)p * N + j];
-        }
-        sv[pp][j] = a;
-        sg[pp][j] = b;
-    }
-    __syncthreads();
-    const int row = threadIdx.x;
-    if (row >= N) return;
-    double2 d1v[TP], d2v[TP], d1g[TP];
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) d1v[pp] = d2v[pp] = d1g[pp] = c2(0, 0);
-    // operator columns JC at a time, all loads in flight before use
-    for (int j0 = 0; j0 < N; j0 += JC) {
-        double da[JC], db[JC];
-#pragma unroll
-        for (int cc = 0; cc < JC; ++cc) {
-            const size_t e = (size_t)(j0 + cc < N ? j0 + cc : N - 1) * N + row;
-            da[cc] = c.DY1[e];
-            db[cc] = c.DY2[e];
-        }
-#pragma unroll
-        for (int cc = 0; cc < JC; ++cc) {
-            const int j = j0 + cc;
-            if (j >= N) break;
-            const double a = da[cc], b = db[cc];
-#pragma unroll
-            for (int pp = 0; pp < TP; ++pp) {
-                const double2 x = sv[pp][j], y = sg[pp][j];
-                d1v[pp].x = fma(a, x.x, d1v[pp].x);
-                d1v[pp].y = fma(a, x.y, d1v[pp].y);
-                d2v[pp].x = fma(b, x.x, d2v[pp].x);
-                d2v[pp].y = fma(b, x.y, d2v[pp].y);
-                d1g[pp].x = fma(a, y.x, d1g[pp].x);
-                d1g[pp].y = fma(a, y.y, d1g[pp].y);
-            }
-        }
-    }
-    double2 *u = c.cur, *w = c.cur + c.W, *ox = c.cur + 2 * c.W, *oz = c.cur + 3 * c.W;
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) {
-        const int p = p0 + pp;
-        if (p >= c.npencil) break;
-        const size_t o = (size_t)p * N + row;
-        // DY2 v of the new level, kept for the next steps' right-hand sides
-        sp.dvn[o] = d2v[pp];
-        const int k = p / c.NXH, i = p % c.NXH;
-        const double kx = c.kx[i], kz = c.kz[k];
-        const double k2 = k2_of(c, k, i);
-        if (k2 < 1.e-8) continue;  // the mean pencil is set by the mean-flow kernels
-        const double ix = __ddiv_rn(kx, k2), iz = __ddiv_rn(kz, k2), ir = __ddiv_rn(1.0, k2);
-        const double2 vv = sv[pp][row], gg = sg[pp][row];
-        u[o] = csubc(imul(ix, d1v[pp]), imul(iz, gg));
-        w[o] = caddc(imul(ix, gg), imul(iz, d1v[pp]));
-        const double2 X = csubc(caddc(cscale(d1g[pp], kx), cscale(d2v[pp], kz)), cscale(vv, mul(kz, k2)));
-        const double2 Y = caddc(csubc(cscale(d1g[pp], kz), cscale(d2v[pp], kx)), cscale(vv, mul(kx, k2)));
-        ox[o] = imul(ir, X);
-        oz[o] = imul(ir, Y);
-    }
-}
 
 // horizontal velocity and vorticity (src/solver.cpp:516-539) with the three
 // products DY1 v, DY2 v, DY1 omega_y on the matrix cores. Eight waves; wave w
@@ -247,7 +170,8 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
 }
 
 // nonlinear source terms (src/solver.cpp:240-260) with DY1 (i kx Lx + i kz Lz)
-// on the matrix cores; Hg and the mean Lamb history are written while staging.
+// on the matrix cores; Hg and the mean Lamb history (the k = 0 pencil of the
+// Lamb vector, src/solver.cpp:293-303) are written while staging.
 // Eight waves, wave w owning row tiles w and w + 8, DY1 prefetched as in
 // k_horizontal_mfma.
 template <typename LC>
@@ -476,67 +400,6 @@ __global__ __launch_bounds__(256) void k_mean(Chan c, StepPtrs sp, const double 
     mean_friction(c, c.msol, c.msol + N, false, sp.sn, step + 1);
     __syncthreads();
     if (threadIdx.x == 0) c.ctrl[C_STEP] = step + 1;
-}
-
-// ---------------------------------------------------------------------------
-// nonlinear source terms (src/solver.cpp:240-260); the k = 0 pencil of the
-// Lamb vector also enters the mean-flow history (src/solver.cpp:293-303)
-// ---------------------------------------------------------------------------
-template <int TP, typename LC>
-__global__ __launch_bounds__(256) void k_nonlinear(Chan c, StepPtrs sp) {
-    __shared__ double2 st[TP][kMaxNY];
-    const int N = c.NY, ns = sp.sn;
-    const LC *lx = reinterpret_cast<const LC *>(c.lmb);
-    const LC *ly = lx + c.W, *lz = lx + 2 * c.W;
-    double2 *Hv = sp.hvn, *Hg = sp.hgn;
-    const int p0 = blockIdx.x * TP;
-    for (int e = threadIdx.x; e < TP * N; e += blockDim.x) {
-        const int pp = e / N, j = e % N, p = p0 + pp;
-        double2 tt = c2(0, 0);
-        if (p < c.npencil) {
-            const int k = p / c.NXH, i = p % c.NXH;
-            const double kx = c.kx[i], kz = c.kz[k];
-            const size_t o = (size_t)p * N + j;
-            const double2 a = ld2(lx + o), b = ld2(lz + o);
-            tt = caddc(imul(kx, a), imul(kz, b));
-            Hg[o] = csubc(imul(kz, a), imul(kx, b));
-            if (p == 0) {
-                c.means[(size_t)(M_LMBX * 3 + ns) * N + j] = a;
-                c.means[(size_t)(M_LMBZ * 3 + ns) * N + j] = b;
-            }
-        }
-        st[pp][j] = tt;
-    }
-    __syncthreads();
-    const int row = threadIdx.x;
-    if (row >= N) return;
-    double2 acc[TP];
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) acc[pp] = c2(0, 0);
-    for (int j0 = 0; j0 < N; j0 += JC) {
-        double dj[JC];
-#pragma unroll
-        for (int cc = 0; cc < JC; ++cc) dj[cc] = c.DY1[(size_t)(j0 + cc < N ? j0 + cc : N - 1) * N + row];
-#pragma unroll
-        for (int cc = 0; cc < JC; ++cc) {
-            const int j = j0 + cc;
-            if (j >= N) break;
-#pragma unroll
-            for (int pp = 0; pp < TP; ++pp) {
-                acc[pp].x = fma(dj[cc], st[pp][j].x, acc[pp].x);
-                acc[pp].y = fma(dj[cc], st[pp][j].y, acc[pp].y);
-            }
-        }
-    }
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) {
-        const int p = p0 + pp;
-        if (p >= c.npencil) break;
-        const int k = p / c.NXH, i = p % c.NXH;
-        const double k2 = k2_of(c, k, i);
-        const size_t o = (size_t)p * N + row;
-        Hv[o] = csubc(cscale(ld2(ly + o), -k2), acc[pp]);
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1116,24 +979,9 @@ constexpr int kTPH = 4;
 
 static int blocks_tp(const Chan &c) { return (c.npencil + kTP - 1) / kTP; }
 
-template <typename LC>
-static void launch_nonlinear_tp(const Chan &c, const StepPtrs &sp, hipStream_t s) {
-    const int sel = c.tune[T_NONLINEAR_TP];
-    if (sel == 1) hipLaunchKernelGGL((k_nonlinear<4, LC>), dim3((c.npencil + 3) / 4), dim3(256), 0, s, c, sp);
-    else if (sel == 2) hipLaunchKernelGGL((k_nonlinear<16, LC>), dim3((c.npencil + 15) / 16), dim3(256), 0, s, c, sp);
-    else hipLaunchKernelGGL((k_nonlinear<kTP, LC>), dim3(blocks_tp(c)), dim3(256), 0, s, c, sp);
-}
-
-// tuned pencils per block: candidates kTPH, 2, 8 (horizontal) and kTP, 4, 16 (nonlinear)
+// the derivative stages run on the matrix-core kernels at every NY (NY <= kMaxNY)
 void launch_horizontal(const Chan &c, const StepPtrs &sp, hipStream_t s) {
-    if (c.NY <= 256) {
-        hipLaunchKernelGGL(k_horizontal_mfma, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
-        return;
-    }
-    const int sel = c.tune[T_HORIZONTAL_TP];
-    if (sel == 1) hipLaunchKernelGGL(k_horizontal<2>, dim3((c.npencil + 1) / 2), dim3(256), 0, s, c, sp);
-    else if (sel == 2) hipLaunchKernelGGL(k_horizontal<8>, dim3((c.npencil + 7) / 8), dim3(256), 0, s, c, sp);
-    else hipLaunchKernelGGL(k_horizontal<kTPH>, dim3((c.npencil + kTPH - 1) / kTPH), dim3(256), 0, s, c, sp);
+    hipLaunchKernelGGL(k_horizontal_mfma, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
 }
 
 void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *P, int mf, hipStream_t s) {
@@ -1141,14 +989,10 @@ void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *
 }
 
 void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s) {
-    if (c.precision == 0 && c.NY <= 256)
+    if (c.precision == 0)
         hipLaunchKernelGGL(k_nonlinear_mfma<double2>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
-    else if (c.precision == 0)
-        launch_nonlinear_tp<double2>(c, sp, s);
-    else if (c.NY <= 256)
-        hipLaunchKernelGGL(k_nonlinear_mfma<float2>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
     else
-        launch_nonlinear_tp<float2>(c, sp, s);
+        hipLaunchKernelGGL(k_nonlinear_mfma<float2>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
 }
 
 void launch_pressure_rhs(const Chan &c, const double2 *dv, double2 *R, hipStream_t s) {
