@@ -239,17 +239,36 @@ __global__ __launch_bounds__(TJ *TL) void k_z_forward(Chan c, const typename CT<
 // Fused x phase (src/FFT.cpp:101-127, src/solver.cpp:284-286, src/FFT.cpp:147-164)
 // per (z plane k2, y tile). Each thread owns the x positions n = tl + e*TL.
 // ---------------------------------------------------------------------------
-template <typename C>
-__device__ __forceinline__ void load_pair_spectrum(C *buf, const C *A, const C *B, int L, int NXH, size_t stride_i,
-                                                   bool valid, int jj, int tl, int TJ, int TL) {
-    // Z = Aext + i Bext with Hermitian extensions; the kx = 0 entries enter
-    // through their real parts (the reference keeps Re of the backward transform)
-    for (int m = tl; m < NXH; m += TL) {
-        C a = mk<C>(0, 0), b = mk<C>(0, 0);
-        if (valid) {
-            a = A[(size_t)m * stride_i];
-            b = B[(size_t)m * stride_i];
-        }
+// The spectra of a pair of fields, fetched into registers (modes m = tl + e TL)
+// so that the loads of the next pair are in flight during the current transform.
+template <typename C, int MP>
+struct PairRegs {
+    C a[MP], b[MP];
+};
+
+template <typename C, int MP>
+__device__ __forceinline__ void fetch_pair_spectrum(PairRegs<C, MP> &r, const C *A, const C *B, int NXH, size_t stride_i,
+                                                    bool valid, int tl, int TL) {
+#pragma unroll
+    for (int e = 0; e < MP; ++e) {
+        const int m = tl + e * TL;
+        const bool in = valid && m < NXH;
+        const size_t o = (size_t)(in ? m : 0) * stride_i;
+        r.a[e] = in ? A[o] : mk<C>(0, 0);
+        r.b[e] = in ? B[o] : mk<C>(0, 0);
+    }
+}
+
+// Z = Aext + i Bext with Hermitian extensions; the kx = 0 entries enter
+// through their real parts (the reference keeps Re of the backward transform)
+template <typename C, int MP>
+__device__ __forceinline__ void place_pair_spectrum(C *buf, const PairRegs<C, MP> &r, int L, int NXH, int jj, int tl, int TJ,
+                                                    int TL) {
+#pragma unroll
+    for (int e = 0; e < MP; ++e) {
+        const int m = tl + e * TL;
+        if (m >= NXH) break;
+        const C a = r.a[e], b = r.b[e];
         if (m == 0) {
             buf[jj] = mk<C>(a.x, b.x);
         } else {
@@ -278,13 +297,18 @@ __global__ __launch_bounds__(TJ *TL) void k_x_lamb(Chan c, const typename CT<T>:
     const C *twg = tw_of<C>(c, true);
     for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = twg[n];
     const size_t plane = (size_t)k2 * NXH * NY + j;
+    // modes per thread of one spectrum: NXH <= NX2 / 3 + 1 <= LMAX / 3 + 1
+    constexpr int MP = (LMAX / 3 + 1 + TL - 1) / TL;
+    PairRegs<C, MP> pr;
+    fetch_pair_spectrum<C, MP>(pr, in + plane, in + c.I + plane, NXH, (size_t)NY, valid, tl, TL);
     C a[3][EPT];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        const C *A = in + (size_t)(2 * q) * c.I + plane;
-        const C *B = in + (size_t)(2 * q + 1) * c.I + plane;
-        load_pair_spectrum<C>(buf, A, B, L, NXH, (size_t)NY, valid, jj, tl, TJ, TL);
+        place_pair_spectrum<C, MP>(buf, pr, L, NXH, jj, tl, TJ, TL);
         __syncthreads();
+        if (q < 2)
+            fetch_pair_spectrum<C, MP>(pr, in + (size_t)(2 * q + 2) * c.I + plane, in + (size_t)(2 * q + 3) * c.I + plane, NXH,
+                                       (size_t)NY, valid, tl, TL);
         fft_lines<C, TJ, TL, LMAX>(buf, tw, c.px, +1, jj, tl);
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
