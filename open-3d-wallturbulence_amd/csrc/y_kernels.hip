@@ -137,6 +137,21 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
         // no barrier here: the next chunk is staged into rows no wave reads yet
     }
     double2 *u = c.cur, *wv = c.cur + c.W, *ox = c.cur + 2 * c.W, *oz = c.cur + 3 * c.W;
+    // a lane stores rows of two pencils only (one per column tile): their
+    // wavenumbers and reciprocals once, not per row
+    double pkx[2], pkz[2], pk2[2], pix[2], piz[2], pir[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+        const int p = p0 + ct * 8 + bq, pq = p < c.npencil ? p : 0;
+        const int k = pq / c.NXH, i = pq % c.NXH;
+        pkx[ct] = c.kx[i];
+        pkz[ct] = c.kz[k];
+        pk2[ct] = k2_of(c, k, i);
+        const double d = pk2[ct] < 1.e-8 ? 1.0 : pk2[ct];
+        pix[ct] = __ddiv_rn(pkx[ct], d);
+        piz[ct] = __ddiv_rn(pkz[ct], d);
+        pir[ct] = __ddiv_rn(1.0, d);
+    }
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
         const int tile = w + NW * rt;
@@ -152,11 +167,9 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
                 const double2 d1v = c2(m1, o1), d2v = c2(m2, o2), d1g = c2(m3, o3);
                 const size_t o = (size_t)p * N + row;
                 sp.dvn[o] = d2v;
-                const int k = p / c.NXH, i = p % c.NXH;
-                const double kx = c.kx[i], kz = c.kz[k];
-                const double k2 = k2_of(c, k, i);
+                const double kx = pkx[ct], kz = pkz[ct], k2 = pk2[ct];
                 if (k2 < 1.e-8) continue;  // the mean pencil is set by the mean-flow kernels
-                const double ix = __ddiv_rn(kx, k2), iz = __ddiv_rn(kz, k2), ir = __ddiv_rn(1.0, k2);
+                const double ix = pix[ct], iz = piz[ct], ir = pir[ct];
                 const double2 vv = sv[ct * 8 + bq][row], gg = sg[ct * 8 + bq][row];
                 u[o] = csubc(imul(ix, d1v), imul(iz, gg));
                 wv[o] = caddc(imul(ix, gg), imul(iz, d1v));
