@@ -276,6 +276,12 @@ __global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
         }
         __syncthreads();
     }
+    double pk2[2];  // k^2 of this lane's two pencils
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+        const int p = p0 + ct * 8 + bq, pq = p < c.npencil ? p : 0;
+        pk2[ct] = k2_of(c, pq / c.NXH, pq % c.NXH);
+    }
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) {
         const int tile = w + NW * rt;
@@ -287,8 +293,7 @@ __global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
                 const double m = acc[rt][ct][r], o1 = __shfl_xor(m, 1);
                 const int row = tile * 16 + 4 * r + (lane >> 4), p = p0 + ct * 8 + bq;
                 if (bc != 0 || row >= N || p >= c.npencil) continue;
-                const int k = p / c.NXH, i = p % c.NXH;
-                const double k2 = k2_of(c, k, i);
+                const double k2 = pk2[ct];
                 const size_t o = (size_t)p * N + row;
                 sp.hvn[o] = csubc(cscale(ld2(ly + o), -k2), c2(m, o1));
             }
