@@ -45,8 +45,8 @@ __device__ __forceinline__ owt_d4 mfma4(double a, double b, owt_d4 c) { return _
 
 // horizontal velocity and vorticity (src/solver.cpp:516-539) with the three
 // products DY1 v, DY2 v, DY1 omega_y on the matrix cores. Eight waves; wave w
-// owns row tiles w and w + 8, and loads the DY operands of the next k step
-// before issuing the products of the current one.
+// owns row tiles w and w + 8 and keeps the DY operands of the next four k
+// steps in flight.
 __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
     constexpr int RT = 2, NW = 8;
     // the block's whole columns of v and omega_y (2 x 16 x 257 x 16 B = 131.6 KB
