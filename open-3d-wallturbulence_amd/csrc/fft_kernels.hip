@@ -181,12 +181,13 @@ __global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typen
     const C *twg = tw_of<C>(c, false);
     for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = twg[n];
     const double2 *src = in.p[f];
+    const bool scale = div != (T)1;  // x / 1 == x: the step's call skips the divisions
     for (int n = tl; n < L; n += TL) {
         // slot n <- mode n (n < NZH); slots NZH..NZ are padding and the dropped z-Nyquist
         const int mode = n < NZH ? n : (n < NZ || (n == NZ && !nyq) ? -1 : n - NZH);
         C v = mk<C>(0, 0);
         if (mode >= 0 && valid) v = load_c<C>(src + ((size_t)mode * NXH + i) * NY + j);
-        buf[n * TJ + jj] = mk<C>(v.x / div, v.y / div);
+        buf[n * TJ + jj] = scale ? mk<C>(v.x / div, v.y / div) : v;
     }
     __syncthreads();
     fft_lines<C, TJ, TL, LMAX>(buf, tw, c.pz, +1, jj, tl);
