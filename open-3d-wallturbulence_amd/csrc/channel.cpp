@@ -2,8 +2,10 @@
 // hipGraph capture, field access and full-state checkpoints.
 #include "channel.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -19,7 +21,8 @@ FFTPlan make_plan(int L) {
     while (n % 4 == 0 && p.nst < kMaxStages) p.radix[p.nst++] = 4, n /= 4;
     while (n % 3 == 0 && p.nst < kMaxStages) p.radix[p.nst++] = 3, n /= 3;
     while (n % 2 == 0 && p.nst < kMaxStages) p.radix[p.nst++] = 2, n /= 2;
-    if (n != 1) throw Error(2, "transform length " + std::to_string(L) + " is not 2^a 3^b with at most 8 stages");
+    while (n % 5 == 0 && p.nst < kMaxStages) p.radix[p.nst++] = 5, n /= 5;
+    if (n != 1) throw Error(2, "transform length " + std::to_string(L) + " is not 2^a 3^b 5^c with at most 8 stages");
     int ns = 1;
     for (int s = 0; s < p.nst; ++s) {
         p.ns[s] = ns;
@@ -52,6 +55,8 @@ void *Channel::dmalloc(size_t bytes) {
 }
 
 Channel::Channel(const Params &p, int device, bool verbose, hipStream_t stream) : P_(p), device_(device), verbose_(verbose) {
+    const char *fp = std::getenv("OWT_FACTOR_PANEL");
+    force_panel_ = fp && fp[0] == '1';
     int ndev = 0;
     OWT_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) throw Error(3, "no HIP device " + std::to_string(device));
@@ -212,7 +217,7 @@ void Channel::factor_operators() {
         OWT_HIP(hipMemcpy(jobs_, jobs.data(), njobs_ * sizeof(int4), hipMemcpyHostToDevice));
         OWT_HIP(hipMemcpy(frep_, frep.data(), nfac_ * sizeof(int), hipMemcpyHostToDevice));
     }
-    launch_factor(c_, Fb_, Pb_, nfac_, frep_, 0, 2, s_);
+    factor_set(Fb_, Pb_, nfac_, frep_, 0, 2);
     OWT_HIP(hipGetLastError());
     OWT_HIP(hipStreamSynchronize(s_));
     check_errors();
@@ -230,7 +235,7 @@ void Channel::pressure(double *out) {
     if (!Fq_) {
         Fq_ = (double *)dmalloc((size_t)nfac_ * N * N * sizeof(double));
         Pq_ = (int *)dmalloc((size_t)nfac_ * N * sizeof(int));
-        launch_factor(c_, Fq_, Pq_, nfac_, frep_, 2, 1, s_);
+        factor_set(Fq_, Pq_, nfac_, frep_, 2, 1);
         OWT_HIP(hipGetLastError());
         OWT_HIP(hipStreamSynchronize(s_));
         check_errors();
@@ -253,6 +258,21 @@ void Channel::get_factor(int sys, int op, double *lu, int *perm) {
     for (size_t col = 0; col < N; ++col)
         for (size_t row = 0; row < N; ++row) lu[col * N + row] = packed[lu_at((int)N, (int)row, (int)col)];
     OWT_HIP(hipMemcpy(perm, Pb_ + f * N, N * sizeof(int), hipMemcpyDeviceToHost));
+}
+
+// LU factors of nfac operators (launch_factor). Above 256 rows the factorization
+// runs by panels with a scratch copy of each matrix: factor indices go in
+// chunks that keep the scratch at about 1 GiB, allocated on first use. The
+// environment variable OWT_FACTOR_PANEL=1, read when the channel is created,
+// forces the panel kernel at every NY (its factors equal the register kernel's).
+void Channel::factor_set(double *F, int *P, int nfac, const int *frep, int op0, int nops) {
+    const int rows = factor_scratch_rows(c_, force_panel_);
+    if (rows > 0 && !fscratch_) {
+        const size_t per = (size_t)2 * rows * rows * sizeof(double);
+        fchunk_ = (int)std::max<size_t>(1, ((size_t)1 << 30) / per);
+        fscratch_ = (double *)dmalloc((size_t)fchunk_ * per);
+    }
+    launch_factor(c_, F, P, nfac, frep, op0, nops, fscratch_, fchunk_ > 0 ? fchunk_ : 1, force_panel_, s_);
 }
 
 // the plain method's per-pencil factor set (refilled every step)
@@ -527,8 +547,9 @@ void Channel::init_post() {
     if (tangent_base_) launch_x_tangent(c_, (const double2 *)tangent_base_->c_.zbuf, s_);
     else launch_x_lamb(c_, s_);
     launch_z_forward(c_, s_);
-    // Hv, Hg of the initial field land in slot 1 (the new slot of ring phase 0)
-    launch_nonlinear(c_, step_ptrs(0), s_);
+    // Hv, Hg of the initial field land in slot 1 (the new slot of ring phase 0);
+    // in the reference's arithmetic for either method
+    launch_nonlinear(c_, step_ptrs(0), true, s_);
     launch_init_history(c_, s_);
     launch_init_mean(c_, s_);
     OWT_HIP(hipGetLastError());
@@ -566,9 +587,9 @@ void Channel::enqueue_step_body(int ring) {
     const StepPtrs sp = step_ptrs(ring);
     const double *F = plain ? Fp_ : Fb_;
     const int *P = plain ? Pp_ : Pb_;
-    if (plain) launch_factor(c_, Fp_, Pp_, c_.npencil, nullptr, 0, 2, s_);
+    if (plain) factor_set(Fp_, Pp_, c_.npencil, nullptr, 0, 2);
     launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, s_);
-    launch_horizontal(c_, sp, s_);
+    launch_horizontal(c_, sp, plain, s_);
     launch_mean(c_, sp, F, P, plain ? 0 : fmap_host_[0], s_);
     // the six transformed fields: u, v, w, omega_x, omega_y, omega_z of the new level
     const size_t W = c_.W;
@@ -577,7 +598,7 @@ void Channel::enqueue_step_body(int ring) {
     if (tangent_base_) launch_x_tangent(c_, (const double2 *)tangent_base_->c_.zbuf, s_);
     else launch_x_lamb(c_, s_);
     launch_z_forward(c_, s_);
-    launch_nonlinear(c_, sp, s_);
+    launch_nonlinear(c_, sp, plain, s_);
 }
 
 void Channel::build_graph(int ring) {
@@ -619,15 +640,15 @@ void Channel::enqueue_kernel(int id) {
     const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
     switch (id) {
         case 0: launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, s_); break;
-        case 1: launch_horizontal(c_, sp, s_); break;
+        case 1: launch_horizontal(c_, sp, plain, s_); break;
         case 2: launch_mean(c_, sp, F, P, plain ? 0 : fmap_host_[0], s_); break;
         case 3: launch_z_inverse(c_, six, 6, c_.zbuf, s_); break;
         case 4: launch_x_lamb(c_, s_); break;
         case 5: launch_z_forward(c_, s_); break;
-        case 6: launch_nonlinear(c_, sp, s_); break;
+        case 6: launch_nonlinear(c_, sp, plain, s_); break;
         case 7:
-            if (plain) launch_factor(c_, Fp_, Pp_, c_.npencil, nullptr, 0, 2, s_);
-            else launch_factor(c_, Fb_, Pb_, nfac_, frep_, 0, 2, s_);
+            if (plain) factor_set(Fp_, Pp_, c_.npencil, nullptr, 0, 2);
+            else factor_set(Fb_, Pb_, nfac_, frep_, 0, 2);
             break;
         default: throw Error(2, "kernel id must be 0..7");
     }

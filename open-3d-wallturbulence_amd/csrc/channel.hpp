@@ -122,6 +122,7 @@ class Channel {
     void upload_operators();
     void factor_operators();
     void ensure_plain_factors();
+    void factor_set(double *F, int *P, int nfac, const int *frep, int op0, int nops);
     void pressure(double *out);
     void init_post();
     StepPtrs step_ptrs(int ring) const;
@@ -147,6 +148,10 @@ class Channel {
     double *Fq_ = nullptr;
     int *Pq_ = nullptr;
     bool factored_ = false;
+    // scratch of the panel factorization (NY > 256): fchunk_ matrices x 2 operators
+    double *fscratch_ = nullptr;
+    int fchunk_ = 0;
+    bool force_panel_ = false;
     // batched factor tables: one factor per distinct k^2; jobs ({factor, system,
     // system or -1}: systems sharing a factor, two at a time), frep (factor -> a
     // system); fmap_host_ (system -> factor)

@@ -102,7 +102,7 @@ void derive_sizes(Params &p) {
     if (p.NX < 4 || p.NZ < 4 || p.NX % 2 || p.NZ % 2)
         throw Error(2, "NX and NZ must be even and at least 4");
     if (p.NY < 7) throw Error(2, "NY must be at least 7 for the 3rd/4th derivative schemes");
-    if (p.NY > 256) throw Error(2, "NY above 256 is not supported by the wall-normal kernels");
+    if (p.NY > 288) throw Error(2, "NY above 288 is not supported by the wall-normal kernels");
     if (!(p.dt > 0) || !(p.nu > 0)) throw Error(2, "nu and dt must be positive");
     p.NXH = p.NX / 2;
     p.NZH = p.NZ / 2;
@@ -111,10 +111,11 @@ void derive_sizes(Params &p) {
     auto smooth = [](long n) {
         while (n % 2 == 0) n /= 2;
         while (n % 3 == 0) n /= 3;
+        while (n % 5 == 0) n /= 5;
         return n == 1;
     };
     if (!smooth(p.NX2) || !smooth(p.NZ2) || p.NX2 > 1152 || p.NZ2 > 1152)
-        throw Error(2, "3/2-padded lengths NX2, NZ2 must be of the form 2^a 3^b and at most 1152");
+        throw Error(2, "3/2-padded lengths NX2, NZ2 must be of the form 2^a 3^b 5^c and at most 1152");
 }
 
 // load_params (src/IO.cpp:8-34). Missing or malformed keys are an error here,

@@ -10,7 +10,7 @@
 namespace owt {
 
 constexpr int kMaxStages = 8;
-constexpr int kMaxNY = 256;
+constexpr int kMaxNY = 288;
 
 // Stored LU factors use N*N doubles in a packed layout: first the strictly lower
 // part of every column (rows k+1..N-1 of column k, columns in order), then the
@@ -28,7 +28,7 @@ __host__ __device__ __forceinline__ size_t lu_at(int N, int row, int col) {
     return row > col ? lu_lower_at(N, col) + (row - col - 1) : lu_upper_at(N, col) + row;
 }
 
-// 1-D transform plan: length L = prod(radix[s]), radices in {2, 3, 4}.
+// 1-D transform plan: length L = prod(radix[s]), radices in {2, 3, 4, 5}.
 // Stage s has span ns[s] (product of the earlier radices), twiddle stride
 // tstep[s] = L / (ns[s] * radix[s]) and mns[s] = ceil(2^32 / ns[s]), with which
 // j / ns[s] = umulhi(j, mns[s]) exactly for j < 2^16 (mns = 0 marks ns = 1).
@@ -112,7 +112,13 @@ void launch_peak3(const double *phys, size_t n, unsigned long long *out, hipStre
 // LU factors for nfac factor indices (see lu_kernels.hip): operators op0 ..
 // op0 + nops - 1 (0 Av, 1 Ag, 2 pressure), stored nops per factor index;
 // frep[f]: representative system or pencil of factor f (null: f itself)
-void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, int op0, int nops, hipStream_t s);
+// Above 256 rows (or with force_panel) the factorization runs by panels and
+// needs scratch for chunk x nops matrices of NY x NY doubles; launches are
+// issued per chunk of factor indices. factor_scratch_rows: NY when scratch is
+// needed, else 0.
+int factor_scratch_rows(const Chan &c, bool force_panel);
+void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, int op0, int nops, double *scratch,
+                   int chunk, bool force_panel, hipStream_t s);
 // dynamic pressure of the current level: right-hand sides into R (W) from the
 // Lamb vector (in the transform precision) and dv = DY2 v of the level; then the
 // solves with the pressure factors (op 2 of the batched factor tables) into out
@@ -123,10 +129,12 @@ void launch_pressure_solve(const Chan &c, const double *F, const int *P, const i
 // batched: job j = {factor, system, system or -1}
 void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int4 *jobs, int njobs, bool plain,
                   hipStream_t s);
-void launch_horizontal(const Chan &c, const StepPtrs &sp, hipStream_t s);
+// exact: the plain method's kernels in the reference's arithmetic (column-order
+// mat-vecs, separate roundings); otherwise the matrix-core kernels
+void launch_horizontal(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s);
 // mean flow, friction force and the k = 0 pencil (factor mf holds Ag(0))
 void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *P, int mf, hipStream_t s);
-void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s);
+void launch_nonlinear(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s);
 // initialisation helpers (src/solver.cpp:379-425)
 void launch_vor_from_vel(const Chan &c, hipStream_t s);
 void launch_init_history(const Chan &c, hipStream_t s);

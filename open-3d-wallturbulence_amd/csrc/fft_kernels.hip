@@ -48,6 +48,15 @@ __device__ __forceinline__ C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y)
 template <typename C>
 __device__ __forceinline__ C cmul(C a, C b) { return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
+// a b - c d: in FP64 each product and the difference rounded separately, as
+// the reference's pointwise Field operations do (src/linalg.hpp:756-816; no
+// contraction into a fused multiply-add); in FP32 contracted
+__device__ __forceinline__ double cross_term(double a, double b, double c, double d) {
+#pragma clang fp contract(off)
+    return a * b - c * d;
+}
+__device__ __forceinline__ float cross_term(float a, float b, float c, float d) { return a * b - c * d; }
+
 template <typename C, int R>
 __device__ __forceinline__ void dft(C (&x)[R], int sign) {
     using T = decltype(C::x);
@@ -62,13 +71,30 @@ __device__ __forceinline__ void dft(C (&x)[R], int sign) {
         x[0] = cadd(x[0], t);
         x[1] = mk<C>(m.x - sn * d.y, m.y + sn * d.x);
         x[2] = mk<C>(m.x + sn * d.y, m.y - sn * d.x);
-    } else {
+    } else if constexpr (R == 4) {
         C a = cadd(x[0], x[2]), b = csub(x[0], x[2]), c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
         const T s = (T)sign;
         x[0] = cadd(a, c);
         x[2] = csub(a, c);
         x[1] = mk<C>(b.x - s * d.y, b.y + s * d.x);
         x[3] = mk<C>(b.x + s * d.y, b.y - s * d.x);
+    } else {
+        // radix 5 with w = e^{sign 2 pi i / 5} = c1 + i sign s1, w^2 = c2 + i sign s2:
+        //   X1, X4 = x0 + c1 (x1 + x4) + c2 (x2 + x3) +- i sign (s1 (x1 - x4) + s2 (x2 - x3))
+        //   X2, X3 = x0 + c2 (x1 + x4) + c1 (x2 + x3) +- i sign (s2 (x1 - x4) - s1 (x2 - x3))
+        const T c1 = (T)0.30901699437494742410229341718281906, c2 = (T)-0.80901699437494742410229341718281906;
+        const T s1 = (T)sign * (T)0.95105651629515357211643933337938214,
+                s2 = (T)sign * (T)0.58778525229247312916870595463907277;
+        const C t1 = cadd(x[1], x[4]), t2 = cadd(x[2], x[3]), t3 = csub(x[1], x[4]), t4 = csub(x[2], x[3]);
+        const C a1 = mk<C>(x[0].x + c1 * t1.x + c2 * t2.x, x[0].y + c1 * t1.y + c2 * t2.y);
+        const C a2 = mk<C>(x[0].x + c2 * t1.x + c1 * t2.x, x[0].y + c2 * t1.y + c1 * t2.y);
+        const C b1 = mk<C>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+        const C b2 = mk<C>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+        x[0] = mk<C>(x[0].x + t1.x + t2.x, x[0].y + t1.y + t2.y);
+        x[1] = mk<C>(a1.x - b1.y, a1.y + b1.x);
+        x[4] = mk<C>(a1.x + b1.y, a1.y - b1.x);
+        x[2] = mk<C>(a2.x - b2.y, a2.y + b2.x);
+        x[3] = mk<C>(a2.x + b2.y, a2.y - b2.x);
     }
 }
 
@@ -120,6 +146,8 @@ __device__ void fft_lines(C *buf, const C *tw, const FFTPlan &P, int sign, int j
         const int R = P.radix[s];
         if (R == 4)
             stage<C, 4, TJ, TL, (LMAX / 4 + TL - 1) / TL>(buf, tw, P.L, P.ns[s], P.mns[s], P.tstep[s], sign, jj, tl);
+        else if (R == 5)
+            stage<C, 5, TJ, TL, (LMAX / 5 + TL - 1) / TL>(buf, tw, P.L, P.ns[s], P.mns[s], P.tstep[s], sign, jj, tl);
         else if (R == 3)
             stage<C, 3, TJ, TL, (LMAX / 3 + TL - 1) / TL>(buf, tw, P.L, P.ns[s], P.mns[s], P.tstep[s], sign, jj, tl);
         else
@@ -323,9 +351,9 @@ __global__ __launch_bounds__(TJ *TL) void k_x_lamb(Chan c, const typename CT<T>:
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
         const T u = a[0][e].x, v = a[0][e].y, w = a[1][e].x, ox = a[1][e].y, oy = a[2][e].x, oz = a[2][e].y;
-        const T lx = v * oz - w * oy;
-        const T ly = w * ox - u * oz;
-        const T lz = u * oy - v * ox;
+        const T lx = cross_term(v, oz, w, oy);
+        const T ly = cross_term(w, ox, u, oz);
+        const T lz = cross_term(u, oy, v, ox);
         a[0][e] = mk<C>(lx, ly);
         a[1][e] = mk<C>(lz, (T)0);
     }

@@ -34,6 +34,42 @@
 
 namespace owt {
 
+// Coefficients of the operators at one k^2 (src/solver.cpp:577-578)
+struct OpCoef {
+    double k2, s1, s2, sg;
+};
+__device__ __forceinline__ OpCoef op_coef(const Chan &c, double k2) {
+    const double nu = c.nu, dt = c.dt;
+    OpCoef o;
+    o.k2 = k2;
+    o.s1 = add(__ddiv_rn(G0, dt), mul(mul(2.0, nu), k2));
+    o.s2 = add(mul(nu, mul(k2, k2)), __ddiv_rn(mul(G0, k2), dt));
+    o.sg = add(mul(nu, k2), __ddiv_rn(G0, dt));
+    return o;
+}
+
+// entry (r, col) of operator op: 0 Av, 1 Ag (src/solver.cpp:577-591, 601-603),
+// 2 the pressure operator (Neumann wall rows; the k = 0 pencil's top row is the gauge)
+__device__ __forceinline__ double op_entry(const Chan &c, int op, const OpCoef &o, int r, int col) {
+    const int N = c.NY;
+    const size_t e = (size_t)col * N + r;
+    if (op == 2) {
+        if (r == 0) return c.DY1[(size_t)col * N];
+        if (r == N - 1) return o.k2 < 1.e-8 ? c.qw[col] : c.DY1[(size_t)col * N + N - 1];
+        const double v = c.DY2[e];
+        return r == col ? sub(v, o.k2) : v;
+    }
+    if (r == 0 || r == N - 1) return (r == col) ? 1.0 : 0.0;
+    if (op == 0) {
+        if (r == 1) return c.DY1[(size_t)col * N];
+        if (r == N - 2) return c.DY1[(size_t)col * N + N - 1];
+        const double v = add(mul(-c.nu, c.DY4[e]), mul(o.s1, c.DY2[e]));
+        return r == col ? sub(v, o.s2) : v;
+    }
+    const double v = mul(-c.nu, c.DY2[e]);
+    return r == col ? add(v, o.sg) : v;
+}
+
 // ---------------------------------------------------------------------------
 // Factorization. TC waves per block; lane l holds rows l + 64a (a < RPT) and
 // wave w holds columns w + TC*b (b < CPT), so every wave holds whole columns
@@ -54,13 +90,9 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, c
     const int f = blockIdx.x, op = op0 + blockIdx.y;
     const int rep = frep ? frep[f] : f;
     const double k2 = k2_of(c, rep / c.NXH, rep % c.NXH);
-    const double nu = c.nu, dt = c.dt;
-    const double k4 = mul(k2, k2);
-    const double s1 = add(__ddiv_rn(G0, dt), mul(mul(2.0, nu), k2));
-    const double s2 = add(mul(nu, k4), __ddiv_rn(mul(G0, k2), dt));
-    const double sg = add(mul(nu, k2), __ddiv_rn(G0, dt));
 
     // assemble (src/solver.cpp:577-591, 601-603)
+    const OpCoef oc = op_coef(c, k2);
     double M[RPT][CPT];
 #pragma unroll
     for (int a = 0; a < RPT; ++a) {
@@ -68,32 +100,7 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, c
 #pragma unroll
         for (int b = 0; b < CPT; ++b) {
             const int col = w + TC * b;
-            double v = 0.0;
-            if (r < N && col < N) {
-                const size_t e = (size_t)col * N + r;
-                if (op == 2) {
-                    // pressure: Neumann wall rows; the k = 0 pencil's top row is the gauge
-                    if (r == 0) v = c.DY1[(size_t)col * N];
-                    else if (r == N - 1) v = k2 < 1.e-8 ? c.qw[col] : c.DY1[(size_t)col * N + N - 1];
-                    else {
-                        v = c.DY2[e];
-                        if (r == col) v = sub(v, k2);
-                    }
-                } else if (r == 0 || r == N - 1) {
-                    v = (r == col) ? 1.0 : 0.0;
-                } else if (op == 0) {
-                    if (r == 1) v = c.DY1[(size_t)col * N];
-                    else if (r == N - 2) v = c.DY1[(size_t)col * N + N - 1];
-                    else {
-                        v = add(mul(-nu, c.DY4[e]), mul(s1, c.DY2[e]));
-                        if (r == col) v = sub(v, s2);
-                    }
-                } else {
-                    v = mul(-nu, c.DY2[e]);
-                    if (r == col) v = add(v, sg);
-                }
-            }
-            M[a][b] = v;
+            M[a][b] = (r < N && col < N) ? op_entry(c, op, oc, r, col) : 0.0;
         }
     }
     for (int r = threadIdx.x; r < NR; r += 64 * TC) {
@@ -181,7 +188,8 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, c
             double l[RPT];
 #pragma unroll
             for (int a = 0; a < RPT; ++a) l[a] = ((active >> a) & 1u) ? lbuf[sel][lane + 64 * a] : 0.0;
-            // rank-1 update of the trailing columns (dger)
+            // rank-1 update of the trailing columns (dger), each product and
+            // difference rounded separately as the reference BLAS does
 #pragma unroll
             for (int b = bk; b < CPT; ++b) {
                 const int col = w + TC * b;
@@ -194,7 +202,7 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, c
                 u = readlane_d(u, pl);
 #pragma unroll
                 for (int a = 0; a < RPT; ++a)
-                    if ((active >> a) & 1u) M[a][b] = fma(-l[a], u, M[a][b]);
+                    if ((active >> a) & 1u) M[a][b] = sub(M[a][b], mul(l[a], u));
             }
         }
     }
@@ -215,6 +223,211 @@ __global__ __launch_bounds__(64 * TC) void k_factor(Chan c, double *F, int *P, c
     }
 }
 
+// ---------------------------------------------------------------------------
+// Factorization of operators larger than one CU's registers hold (NY > 256):
+// left-looking by panels of PW = NWV * CPW columns. Wave w holds columns
+// j0 + w + NWV*b (b < CPW) of the panel, lane l rows l + 64a (a < RPT). Each
+// panel is assembled, receives the updates of every earlier pivot step k in
+// ascending k (l_rk from the scratch copy S of the finished columns, u_kj from
+// the pivot row's register by a lane read within the wave), then is factored
+// column by column as k_factor does. Every element thus receives the same
+// updates in the same order and with the same roundings as in k_factor, so the
+// factors are identical. S holds the block's matrix column-major with rows in
+// their original order; the finished factor is packed from it at the end.
+// ---------------------------------------------------------------------------
+template <int RPT, int CPW, int NWV>
+__global__ __launch_bounds__(64 * NWV) void k_factor_panel(Chan c, double *F, int *P, const int *frep, int op0, int nops, int f0,
+                                                           double *S) {
+    constexpr int NR = 64 * RPT, PW = NWV * CPW;
+    __shared__ double lbuf[2][NR];
+    __shared__ int pos[NR], rowAt[NR], pstep[NR], piv[NR];
+    __shared__ int pivrow[2];
+    __shared__ int sing;
+
+    const int N = c.NY;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int f = f0 + blockIdx.x, op = op0 + blockIdx.y;
+    const int rep = frep ? frep[f] : f;
+    const OpCoef oc = op_coef(c, k2_of(c, rep / c.NXH, rep % c.NXH));
+    double *Sm = S + ((size_t)blockIdx.x * nops + blockIdx.y) * N * N;
+    double *Fo = F + ((size_t)f * nops + blockIdx.y) * N * N;
+    int *Po = P + ((size_t)f * nops + blockIdx.y) * N;
+    constexpr int kNone = 1 << 30;  // pivot step of a row not pivoted yet
+    for (int r = threadIdx.x; r < NR; r += 64 * NWV) {
+        pos[r] = r;
+        rowAt[r] = r;
+        pstep[r] = kNone;
+    }
+    if (threadIdx.x == 0) sing = 0;
+    __syncthreads();
+
+    for (int j0 = 0; j0 < N; j0 += PW) {
+        double M[RPT][CPW];
+        int ps[RPT];
+#pragma unroll
+        for (int a = 0; a < RPT; ++a) {
+            const int r = lane + 64 * a;
+            ps[a] = r < N ? pstep[r] : -1;  // rows past N take no updates
+#pragma unroll
+            for (int b = 0; b < CPW; ++b) {
+                const int col = j0 + w + NWV * b;
+                M[a][b] = (r < N && col < N) ? op_entry(c, op, oc, r, col) : 0.0;
+            }
+        }
+        // updates of the earlier pivot steps, l of step k + 1 fetched while step k runs
+        double ln[RPT];
+#pragma unroll
+        for (int a = 0; a < RPT; ++a) {
+            const int r = lane + 64 * a;
+            ln[a] = (j0 > 0 && r < N) ? Sm[r] : 0.0;
+        }
+        for (int k = 0; k < j0; ++k) {
+            double l[RPT];
+#pragma unroll
+            for (int a = 0; a < RPT; ++a) l[a] = ln[a];
+            if (k + 1 < j0) {
+#pragma unroll
+                for (int a = 0; a < RPT; ++a) {
+                    const int r = lane + 64 * a;
+                    ln[a] = r < N ? Sm[(size_t)(k + 1) * N + r] : 0.0;
+                }
+            }
+            const int p = piv[k], pl = p & 63, pa = p >> 6;
+#pragma unroll
+            for (int b = 0; b < CPW; ++b) {
+                double u = M[0][b];
+#pragma unroll
+                for (int a = 1; a < RPT; ++a)
+                    if (a == pa) u = M[a][b];
+                u = readlane_d(u, pl);
+#pragma unroll
+                for (int a = 0; a < RPT; ++a)
+                    if (ps[a] > k) M[a][b] = sub(M[a][b], mul(l[a], u));
+            }
+        }
+        unsigned active = 0;
+#pragma unroll
+        for (int a = 0; a < RPT; ++a)
+            if (ps[a] == kNone) active |= 1u << a;
+        // factor the panel's columns
+        for (int kk = 0; kk < PW; ++kk) {
+            const int k = j0 + kk;
+            if (k >= N) break;
+            const int wk = kk % NWV, bk = kk / NWV, sel = k & 1;
+            if (w == wk) {
+                double best = -1.0;
+                int bpos = INT_MAX, brow = -1;
+#pragma unroll
+                for (int b = 0; b < CPW; ++b) {
+                    if (b != bk) continue;
+#pragma unroll
+                    for (int a = 0; a < RPT; ++a) {
+                        if ((active >> a) & 1u) {
+                            const int r = lane + 64 * a;
+                            const double m = fabs(M[a][b]);
+                            const int pq = pos[r];
+                            if (m > best || (m == best && pq < bpos)) {
+                                best = m;
+                                bpos = pq;
+                                brow = r;
+                            }
+                        }
+                    }
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ob = __shfl_xor(best, off);
+                    const int opos = __shfl_xor(bpos, off), orow = __shfl_xor(brow, off);
+                    if (ob > best || (ob == best && opos < bpos)) {
+                        best = ob;
+                        bpos = opos;
+                        brow = orow;
+                    }
+                }
+                int p = __builtin_amdgcn_readfirstlane(brow);
+                if (p < 0) p = __builtin_amdgcn_readfirstlane(rowAt[k]);
+                const int pl = p & 63, pa = p >> 6;
+                double pv = 0.0;
+#pragma unroll
+                for (int b = 0; b < CPW; ++b) {
+                    if (b != bk) continue;
+                    pv = M[0][b];
+#pragma unroll
+                    for (int a = 1; a < RPT; ++a)
+                        if (a == pa) pv = M[a][b];
+                }
+                pv = readlane_d(pv, pl);
+                if (lane == 0) {
+                    const int q = pos[p], rk = rowAt[k];
+                    rowAt[q] = rk;
+                    pos[rk] = q;
+                    rowAt[k] = p;
+                    pos[p] = k;
+                    pivrow[sel] = p;
+                    piv[k] = p;
+                    pstep[p] = k;
+                    Po[k] = p;
+                    if (!(pv != 0.0) || brow < 0) sing = 1;
+                }
+                if (lane == pl) active &= ~(1u << pa);
+                const bool recip = fabs(pv) >= DBL_MIN;
+                const double rinv = 1.0 / pv;
+#pragma unroll
+                for (int b = 0; b < CPW; ++b) {
+                    if (b != bk) continue;
+#pragma unroll
+                    for (int a = 0; a < RPT; ++a) {
+                        if ((active >> a) & 1u) {
+                            const double l = recip ? M[a][b] * rinv : M[a][b] / pv;
+                            M[a][b] = l;
+                            lbuf[sel][lane + 64 * a] = l;
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            const int p = __builtin_amdgcn_readfirstlane(pivrow[sel]);
+            const int pl = p & 63, pa = p >> 6;
+            if (lane == pl) active &= ~(1u << pa);
+            double l[RPT];
+#pragma unroll
+            for (int a = 0; a < RPT; ++a) l[a] = ((active >> a) & 1u) ? lbuf[sel][lane + 64 * a] : 0.0;
+#pragma unroll
+            for (int b = 0; b < CPW; ++b) {
+                if (w + NWV * b <= kk) continue;
+                if (j0 + w + NWV * b >= N) continue;
+                double u = M[0][b];
+#pragma unroll
+                for (int a = 1; a < RPT; ++a)
+                    if (a == pa) u = M[a][b];
+                u = readlane_d(u, pl);
+#pragma unroll
+                for (int a = 0; a < RPT; ++a)
+                    if ((active >> a) & 1u) M[a][b] = sub(M[a][b], mul(l[a], u));
+            }
+        }
+        // the panel is final: into the scratch copy
+#pragma unroll
+        for (int a = 0; a < RPT; ++a) {
+            const int r = lane + 64 * a;
+            if (r >= N) continue;
+#pragma unroll
+            for (int b = 0; b < CPW; ++b) {
+                const int col = j0 + w + NWV * b;
+                if (col < N) Sm[(size_t)col * N + r] = M[a][b];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && sing) atomicOr(c.ctrl + C_ERR, E_SINGULAR);
+    // pack: row r of the interchanged factor is the row pivoted at step pstep[r]
+    for (size_t e = threadIdx.x; e < (size_t)N * N; e += 64 * NWV) {
+        const int col = (int)(e / N), r = (int)(e % N);
+        int rk = pstep[r];
+        if ((unsigned)rk >= (unsigned)N) rk = 0;
+        Fo[lu_at(N, rk, col)] = Sm[e];
+    }
+}
+
 // The pencils of batched job jb = {factor, system, system or -1}: each system
 // s = mz*NXH + i is the +kz pencil s and, for 0 < mz < NZH, the -kz pencil
 // (NZ - mz)*NXH + i; missing ones are -1.
@@ -228,6 +441,13 @@ __device__ __forceinline__ void job_pencils(const Chan &c, int4 jb, int (&pc)[4]
     }
 }
 
+// LDS written by some lanes of a wave and then read by others of the same wave
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // Right-hand sides (src/solver.cpp:568-574, 593-599) and dgetrs.
 // Four waves per block: wave w takes job 2*block + w/2 and operator w%2.
@@ -237,11 +457,20 @@ __device__ __forceinline__ void job_pencils(const Chan &c, int4 jb, int (&pc)[4]
 // its own factor. Each lane forms the right-hand side directly at the original
 // row that its position in the pivot order names (dlaswp folded into the
 // loads), so the solve needs no data exchange before the substitution.
+//
+// The plain method keeps the reference's arithmetic: DY2 (sum alpha v) / dt
+// with the mat-vec in column order (src/solver.cpp:571, src/linalg.hpp:434-453),
+// every product and sum rounded separately, and the substitution of dgetrs with
+// separate roundings and divisions by the diagonal. The batched method forms
+// that term from the stored DY2 v of each level (computed by the horizontal
+// kernel when the level was new) and substitutes with fused multiply-adds.
 // ---------------------------------------------------------------------------
 template <int NQ, bool PLAIN, int JPB>
 __global__ __launch_bounds__(128 * JPB) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P, const int4 *jobs, int njobs) {
     constexpr int NP = PLAIN ? 1 : 4;  // pencils per job
     constexpr int NR = 2 * NP;         // real right-hand sides
+    // plain: sum alpha v of the v-wave's pencil in natural row order
+    __shared__ double2 ssa[PLAIN ? JPB : 1][PLAIN ? 64 * NQ : 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // Batched jobs sharing a factor are adjacent in the list. The hardware deals
     // consecutive blocks to the 8 XCDs in turn, so the grid is numbered to give
@@ -282,9 +511,39 @@ __global__ __launch_bounds__(128 * JPB) void k_solve(Chan c, StepPtrs sp, const 
     const double *Fo = F + ((size_t)fac * 2 + op) * N * N;
     const int *Po = P + ((size_t)fac * 2 + op) * N;
 
-    // right-hand sides at the pivot rows. The v equation's DY2 (sum alpha v) / dt
-    // is formed from the stored DY2 v of each level, which the horizontal kernel
-    // computed when that level was new. Loads read valid addresses (missing
+    // plain, v equation: DY2 (sum alpha v) at this lane's pivot rows, the column
+    // loop in the reference's order
+    double2 d2s[PLAIN ? NQ : 1];
+    if (PLAIN && op == 0) {
+        double2 *sa = ssa[(w >> 1) % (PLAIN ? JPB : 1)];
+        const size_t o0 = (size_t)pc[0] * N;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int j = lane + 64 * q;
+            if (j < N)
+                sa[j] = caddc(caddc(cscale(X0[o0 + j], AL0), cscale(X1[o0 + j], AL1)), cscale(X2[o0 + j], AL2));
+        }
+        wave_sync();
+        int rq[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int pos = lane + 64 * q;
+            int r = pos < N ? Po[pos] : 0;
+            rq[q] = (unsigned)r < (unsigned)N ? r : 0;
+            d2s[q] = c2(0, 0);
+        }
+        for (int j = 0; j < N; ++j) {
+            const double2 x = sa[j];
+            const double *col = c.DY2 + (size_t)j * N;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const double a = col[rq[q]];
+                d2s[q] = c2(add(d2s[q].x, mul(a, x.x)), add(d2s[q].y, mul(a, x.y)));
+            }
+        }
+    }
+
+    // right-hand sides at the pivot rows. Loads read valid addresses (missing
     // pencils read the first one) and are masked afterwards.
     double t[NR][NQ];
 #pragma unroll
@@ -300,7 +559,9 @@ __global__ __launch_bounds__(128 * JPB) void k_solve(Chan c, StepPtrs sp, const 
             const double2 h = caddc(caddc(cscale(H0[o], BE0), cscale(H1[o], BE1)), cscale(H2[o], BE2));
             double2 rhs;
             if (op == 0) {
-                const double2 d = caddc(caddc(cscale(D0[o], AL0), cscale(D1[o], AL1)), cscale(D2[o], AL2));
+                double2 d;
+                if (PLAIN) d = d2s[PLAIN ? q : 0];
+                else d = caddc(caddc(cscale(D0[o], AL0), cscale(D1[o], AL1)), cscale(D2[o], AL2));
                 rhs = caddc(csubc(h, cscale(a, k2dt)), cdiv(d, dt));
             } else {
                 rhs = caddc(h, cdiv(a, dt));
@@ -310,7 +571,7 @@ __global__ __launch_bounds__(128 * JPB) void k_solve(Chan c, StepPtrs sp, const 
             t[2 * pp + 1][q] = rhs.y;
         }
     }
-    lu_substitute<NQ, NR>(Fo, N, lane, t);
+    lu_substitute<NQ, NR, 4, PLAIN>(Fo, N, lane, t);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int row = lane + 64 * q;
@@ -354,7 +615,7 @@ __global__ __launch_bounds__(256) void k_psolve(Chan c, const double *F, const i
             t[2 * pp + 1][q] = v.y;
         }
     }
-    lu_substitute<NQ, 8>(Fo, N, lane, t);
+    lu_substitute<NQ, 8, 4, true>(Fo, N, lane, t);
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int row = lane + 64 * q;
@@ -461,9 +722,24 @@ __global__ __launch_bounds__(256) void k_solve_adj(Chan c, StepPtrs sp, const do
         else { constexpr int RPT = 4, CPT = 16, TC = 16; BODY; }                        \
     } while (0)
 
-void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, int op0, int nops, hipStream_t s) {
-    OWT_FACTOR_DISPATCH(c.NY, hipLaunchKernelGGL((k_factor<RPT, CPT, TC>), dim3(nfac, nops), dim3(64 * TC), 0, s, c, F, P, frep,
-                                                 op0, nops));
+// rows of the register-resident factorization; above, the panel factorization
+constexpr int kFactorRegMaxNY = 256;
+
+int factor_scratch_rows(const Chan &c, bool force_panel) { return (c.NY > kFactorRegMaxNY || force_panel) ? c.NY : 0; }
+
+void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, int op0, int nops, double *scratch,
+                   int chunk, bool force_panel, hipStream_t s) {
+    if (c.NY <= kFactorRegMaxNY && !force_panel) {
+        OWT_FACTOR_DISPATCH(c.NY, hipLaunchKernelGGL((k_factor<RPT, CPT, TC>), dim3(nfac, nops), dim3(64 * TC), 0, s, c, F, P,
+                                                     frep, op0, nops));
+        return;
+    }
+    // NY <= kMaxNY = 288: five row blocks per lane, panels of 8 waves x 4 columns
+    static_assert(kMaxNY <= 5 * 64, "panel factorization holds at most 320 rows");
+    for (int f0 = 0; f0 < nfac; f0 += chunk) {
+        const int nc = nfac - f0 < chunk ? nfac - f0 : chunk;
+        hipLaunchKernelGGL((k_factor_panel<5, 4, 8>), dim3(nc, nops), dim3(512), 0, s, c, F, P, frep, op0, nops, f0, scratch);
+    }
 }
 
 void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int4 *jobs, int njobs, bool plain,

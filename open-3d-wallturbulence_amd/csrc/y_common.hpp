@@ -10,10 +10,22 @@ namespace owt {
 constexpr double G0 = 11.0 / 6.0, AL0 = 3.0, AL1 = -1.5, AL2 = 1.0 / 3.0, BE0 = 3.0, BE1 = -3.0, BE2 = 1.0;
 
 // exactly rounded elementwise arithmetic, as the reference's separate vector
-// operations round each product and sum
-__device__ __forceinline__ double mul(double a, double b) { return __dmul_rn(a, b); }
-__device__ __forceinline__ double add(double a, double b) { return __dadd_rn(a, b); }
-__device__ __forceinline__ double sub(double a, double b) { return __dsub_rn(a, b); }
+// operations round each product and sum. HIP's __dmul_rn / __dadd_rn are plain
+// operators that hipcc contracts into fused multiply-adds; contraction is
+// switched off inside these helpers, so a product and the sum it feeds stay two
+// roundings (v_mul_f64 + v_add_f64).
+__device__ __forceinline__ double mul(double a, double b) {
+#pragma clang fp contract(off)
+    return a * b;
+}
+__device__ __forceinline__ double add(double a, double b) {
+#pragma clang fp contract(off)
+    return a + b;
+}
+__device__ __forceinline__ double sub(double a, double b) {
+#pragma clang fp contract(off)
+    return a - b;
+}
 __device__ __forceinline__ double2 c2(double a, double b) { return make_double2(a, b); }
 __device__ __forceinline__ double2 cscale(double2 v, double s) { return c2(mul(v.x, s), mul(v.y, s)); }
 __device__ __forceinline__ double2 caddc(double2 a, double2 b) { return c2(add(a.x, b.x), add(a.y, b.y)); }
@@ -58,7 +70,9 @@ __device__ __forceinline__ void lu_gather(const double (*S)[64 * NQ], const int 
 
 // Forward substitution with the unit lower factor, then back substitution with
 // the upper factor (dtrsm as called by dgetrs, src/linalg.hpp:660-680), on a
-// packed factor.
+// packed factor. EXACT rounds every product and difference separately and
+// divides by the diagonal, as the reference's BLAS does (b_i - b_k l_ik, then
+// b_k / u_kk); otherwise fused multiply-adds and one reciprocal per column.
 //  - Steps run block by block: step k belongs to row block qk = k / 64, a
 //    compile-time index, so the pivot value is read from a fixed register of
 //    lane k % 64 and only the row blocks that meet the triangle are touched.
@@ -68,8 +82,8 @@ __device__ __forceinline__ void lu_gather(const double (*S)[64 * NQ], const int 
 //    outside the triangle reads an element of the column's own run instead (one
 //    cache line for all such lanes) and its value is masked, so no load sits
 //    behind a branch and only the triangle is fetched.
-//  - The back substitution scales by the reciprocal of the diagonal, as
-//    optimized trsm kernels do (one division per column).
+//  - Without EXACT the back substitution scales by the reciprocal of the
+//    diagonal, as optimized trsm kernels do (one division per column).
 template <int NQ, int CK>
 __device__ __forceinline__ void lu_load_lower(const double *__restrict__ Fo, int N, int lane, int qk, int kb, double (&cb)[CK][NQ]) {
 #pragma unroll
@@ -87,7 +101,7 @@ __device__ __forceinline__ void lu_load_lower(const double *__restrict__ Fo, int
     }
 }
 
-template <int NR, int NQ, int CK>
+template <int NR, int NQ, int CK, bool EXACT>
 __device__ __forceinline__ void lu_apply_lower(int N, int lane, int qk, int kb, double (&cb)[CK][NQ], double (&t)[NR][NQ]) {
 #pragma unroll
     for (int c = 0; c < CK; ++c) {
@@ -102,12 +116,12 @@ __device__ __forceinline__ void lu_apply_lower(int N, int lane, int qk, int kb, 
             const int row = lane + 64 * q;
             const double m = (row > k && row < N) ? cb[c][q] : 0.0;
 #pragma unroll
-            for (int r = 0; r < NR; ++r) t[r][q] = fma(-m, yk[r], t[r][q]);
+            for (int r = 0; r < NR; ++r) t[r][q] = EXACT ? sub(t[r][q], mul(yk[r], m)) : fma(-m, yk[r], t[r][q]);
         }
     }
 }
 
-template <int NQ, int NR = 4, int CK = 4>
+template <int NQ, int NR = 4, int CK = 4, bool EXACT = false>
 __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int N, int lane, double (&t)[NR][NQ]) {
     // forward: chunks of CK columns inside each row block, next chunk in flight
 #pragma unroll
@@ -120,12 +134,12 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
         while (kb < kend) {
             const bool more = kb + CK < kend;
             if (more) lu_load_lower<NQ, CK>(Fo, N, lane, qk, kb + CK, cc);
-            lu_apply_lower<NR, NQ, CK>(N, lane, qk, kb, ca, t);
+            lu_apply_lower<NR, NQ, CK, EXACT>(N, lane, qk, kb, ca, t);
             kb += CK;
             if (!more) break;
             const bool more2 = kb + CK < kend;
             if (more2) lu_load_lower<NQ, CK>(Fo, N, lane, qk, kb + CK, ca);
-            lu_apply_lower<NR, NQ, CK>(N, lane, qk, kb, cc, t);
+            lu_apply_lower<NR, NQ, CK, EXACT>(N, lane, qk, kb, cc, t);
             kb += CK;
             if (!more2) break;
         }
@@ -154,10 +168,11 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
                 const int k = k0 - c;
                 if (k < lo) break;
                 const int lk = k & 63;
-                const double rinv = 1.0 / readlane_d(cb[c][qk], lk);
+                const double dk = readlane_d(cb[c][qk], lk);
+                const double rinv = EXACT ? 0.0 : 1.0 / dk;
                 double xk[NR];
 #pragma unroll
-                for (int r = 0; r < NR; ++r) xk[r] = readlane_d(t[r][qk], lk) * rinv;
+                for (int r = 0; r < NR; ++r) xk[r] = EXACT ? __ddiv_rn(readlane_d(t[r][qk], lk), dk) : readlane_d(t[r][qk], lk) * rinv;
                 if (lane == lk) {
 #pragma unroll
                     for (int r = 0; r < NR; ++r) t[r][qk] = xk[r];
@@ -167,7 +182,7 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
                     const int row = lane + 64 * q;
                     const double m = row < k ? cb[c][q] : 0.0;
 #pragma unroll
-                    for (int r = 0; r < NR; ++r) t[r][q] = fma(-m, xk[r], t[r][q]);
+                    for (int r = 0; r < NR; ++r) t[r][q] = EXACT ? sub(t[r][q], mul(xk[r], m)) : fma(-m, xk[r], t[r][q]);
                 }
             }
         };
@@ -195,7 +210,8 @@ __device__ __forceinline__ void lu_substitute(const double *__restrict__ Fo, int
         if (nq_ <= 1) { constexpr int NQ = 1; BODY; }                 \
         else if (nq_ == 2) { constexpr int NQ = 2; BODY; }            \
         else if (nq_ == 3) { constexpr int NQ = 3; BODY; }            \
-        else { constexpr int NQ = 4; BODY; }                          \
+        else if (nq_ == 4) { constexpr int NQ = 4; BODY; }            \
+        else { constexpr int NQ = 5; BODY; }                          \
     } while (0)
 
 }  // namespace owt

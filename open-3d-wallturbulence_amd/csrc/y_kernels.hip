@@ -45,11 +45,12 @@ __device__ __forceinline__ owt_d4 mfma4(double a, double b, owt_d4 c) { return _
 
 // horizontal velocity and vorticity (src/solver.cpp:516-539) with the three
 // products DY1 v, DY2 v, DY1 omega_y on the matrix cores. Eight waves; wave w
-// owns row tiles w and w + 8 and keeps the DY operands of the next four k
-// steps in flight.
+// owns row tiles w, w + 8, ... (RT of them: 2 up to NY = 256, 3 above) and
+// keeps the DY operands of the next four k steps in flight.
+template <int RT>
 __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
-    constexpr int RT = 2, NW = 8;
-    // the block's whole columns of v and omega_y (2 x 16 x 257 x 16 B = 131.6 KB
+    constexpr int NW = 8;
+    // the block's whole columns of v and omega_y (2 x 16 x 289 x 16 B = 148 KB
     // of the 160 KB per CU), staged chunk by chunk and read again by the epilogue
     __shared__ double2 sv[kMP][kMaxNY + 1], sg[kMP][kMaxNY + 1];
     const int N = c.NY, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -187,9 +188,9 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
 // Lamb vector, src/solver.cpp:293-303) are written while staging.
 // Eight waves, wave w owning row tiles w and w + 8, DY1 prefetched as in
 // k_horizontal_mfma.
-template <typename LC>
+template <int RT, typename LC>
 __global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
-    constexpr int RT = 2, NW = 8, PF = 4;
+    constexpr int NW = 8, PF = 4;
     static_assert((kKC / 4) % PF == 0, "chunks hold whole rounds of the operand ring");
     __shared__ double2 st[kMP][kKC + 1];
     const int N = c.NY, ns = sp.sn, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -302,6 +303,105 @@ __global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
 }
 
 // ---------------------------------------------------------------------------
+// The plain method's derivative stages in the reference's arithmetic: each
+// mat-vec in column order (src/linalg.hpp:434-453) with every product and sum
+// rounded separately, TP pencils per block staged in LDS, rows looped over the
+// block's threads.
+// ---------------------------------------------------------------------------
+// solve_horizontal (src/solver.cpp:516-539) and DY2 v of the new level
+template <int TP>
+__global__ __launch_bounds__(256) void k_horizontal_exact(Chan c, StepPtrs sp) {
+    __shared__ double2 sv[TP][kMaxNY], sg[TP][kMaxNY];
+    const int N = c.NY, p0 = blockIdx.x * TP;
+    for (int e = threadIdx.x; e < TP * N; e += blockDim.x) {
+        const int pp = e / N, j = e % N, p = p0 + pp;
+        const bool in = p < c.npencil;
+        sv[pp][j] = in ? sp.vn[(size_t)p * N + j] : c2(0, 0);
+        sg[pp][j] = in ? sp.gn[(size_t)p * N + j] : c2(0, 0);
+    }
+    __syncthreads();
+    double2 *u = c.cur, *wv = c.cur + c.W, *ox = c.cur + 2 * c.W, *oz = c.cur + 3 * c.W;
+    for (int row = threadIdx.x; row < N; row += blockDim.x) {
+        double2 d1v[TP], d2v[TP], d1g[TP];
+#pragma unroll
+        for (int pp = 0; pp < TP; ++pp) d1v[pp] = d2v[pp] = d1g[pp] = c2(0, 0);
+        for (int j = 0; j < N; ++j) {
+            const double a1 = c.DY1[(size_t)j * N + row], a2 = c.DY2[(size_t)j * N + row];
+#pragma unroll
+            for (int pp = 0; pp < TP; ++pp) {
+                const double2 x = sv[pp][j], g = sg[pp][j];
+                d1v[pp] = c2(add(d1v[pp].x, mul(a1, x.x)), add(d1v[pp].y, mul(a1, x.y)));
+                d2v[pp] = c2(add(d2v[pp].x, mul(a2, x.x)), add(d2v[pp].y, mul(a2, x.y)));
+                d1g[pp] = c2(add(d1g[pp].x, mul(a1, g.x)), add(d1g[pp].y, mul(a1, g.y)));
+            }
+        }
+#pragma unroll
+        for (int pp = 0; pp < TP; ++pp) {
+            const int p = p0 + pp;
+            if (p >= c.npencil) break;
+            const size_t o = (size_t)p * N + row;
+            sp.dvn[o] = d2v[pp];
+            const int k = p / c.NXH, i = p % c.NXH;
+            const double kx = c.kx[i], kz = c.kz[k], k2 = k2_of(c, k, i);
+            if (k2 < 1.e-8) continue;  // the mean pencil is set by the mean-flow kernel
+            const double ix = __ddiv_rn(kx, k2), iz = __ddiv_rn(kz, k2), ir = __ddiv_rn(1.0, k2);
+            const double2 vv = sv[pp][row], gg = sg[pp][row];
+            u[o] = csubc(imul(ix, d1v[pp]), imul(iz, gg));
+            wv[o] = caddc(imul(ix, gg), imul(iz, d1v[pp]));
+            const double2 X = csubc(caddc(cscale(d1g[pp], kx), cscale(d2v[pp], kz)), cscale(vv, mul(kz, k2)));
+            const double2 Yv = caddc(csubc(cscale(d1g[pp], kz), cscale(d2v[pp], kx)), cscale(vv, mul(kx, k2)));
+            ox[o] = imul(ir, X);
+            oz[o] = imul(ir, Yv);
+        }
+    }
+}
+
+// nonlinear_source (src/solver.cpp:240-260) and the mean Lamb history
+// (src/solver.cpp:293-303)
+template <int TP, typename LC>
+__global__ __launch_bounds__(256) void k_nonlinear_exact(Chan c, StepPtrs sp) {
+    __shared__ double2 st[TP][kMaxNY];
+    const int N = c.NY, ns = sp.sn, p0 = blockIdx.x * TP;
+    const LC *lx = reinterpret_cast<const LC *>(c.lmb);
+    const LC *ly = lx + c.W, *lz = lx + 2 * c.W;
+    for (int e = threadIdx.x; e < TP * N; e += blockDim.x) {
+        const int pp = e / N, j = e % N, p = p0 + pp;
+        double2 t = c2(0, 0);
+        if (p < c.npencil) {
+            const size_t o = (size_t)p * N + j;
+            const double kx = c.kx[p % c.NXH], kz = c.kz[p / c.NXH];
+            const double2 a = ld2(lx + o), b = ld2(lz + o);
+            t = caddc(imul(kx, a), imul(kz, b));
+            sp.hgn[o] = csubc(imul(kz, a), imul(kx, b));
+            if (p == 0) {
+                c.means[(size_t)(M_LMBX * 3 + ns) * N + j] = a;
+                c.means[(size_t)(M_LMBZ * 3 + ns) * N + j] = b;
+            }
+        }
+        st[pp][j] = t;
+    }
+    __syncthreads();
+    for (int row = threadIdx.x; row < N; row += blockDim.x) {
+        double2 d[TP];
+#pragma unroll
+        for (int pp = 0; pp < TP; ++pp) d[pp] = c2(0, 0);
+        for (int j = 0; j < N; ++j) {
+            const double a = c.DY1[(size_t)j * N + row];
+#pragma unroll
+            for (int pp = 0; pp < TP; ++pp) d[pp] = c2(add(d[pp].x, mul(a, st[pp][j].x)), add(d[pp].y, mul(a, st[pp][j].y)));
+        }
+#pragma unroll
+        for (int pp = 0; pp < TP; ++pp) {
+            const int p = p0 + pp;
+            if (p >= c.npencil) break;
+            const size_t o = (size_t)p * N + row;
+            const double k2 = k2_of(c, p / c.NXH, p % c.NXH);
+            sp.hvn[o] = csubc(cscale(ld2(ly + o), -k2), d[pp]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // mean flow (src/solver.cpp:428-514) and friction force (src/solver.cpp:306-327)
 // ---------------------------------------------------------------------------
 // dudy rows 0 and N-1, the friction force, the new mean history and the mean
@@ -315,15 +415,12 @@ __device__ void mean_friction(const Chan &c, const double2 *U, const double2 *Wm
         sW[j] = Wm[j];
     }
     __syncthreads();
-    const int row = threadIdx.x;
-    if (row < N) {
+    for (int row = threadIdx.x; row < N; row += blockDim.x) {
         double2 du = c2(0, 0), dw = c2(0, 0);
         for (int j = 0; j < N; ++j) {
             const double a = c.DY1[(size_t)j * N + row];
-            du.x = fma(a, sU[j].x, du.x);
-            du.y = fma(a, sU[j].y, du.y);
-            dw.x = fma(a, sW[j].x, dw.x);
-            dw.y = fma(a, sW[j].y, dw.y);
+            du = c2(add(du.x, mul(a, sU[j].x)), add(du.y, mul(a, sU[j].y)));
+            dw = c2(add(dw.x, mul(a, sW[j].x)), add(dw.y, mul(a, sW[j].y)));
         }
         if (!init) {
             c.cur[row] = sU[row];                                 // u(0,0)
@@ -404,7 +501,7 @@ __global__ __launch_bounds__(256) void k_mean(Chan c, StepPtrs sp, const double 
     if (threadIdx.x < 64) {
         double t[4][NQ];
         lu_gather<NQ>(S, P + ((size_t)mf * 2 + 1) * N, N, lane, t);
-        lu_substitute<NQ>(F + ((size_t)mf * 2 + 1) * N * N, N, lane, t);
+        lu_substitute<NQ, 4, 4, true>(F + ((size_t)mf * 2 + 1) * N * N, N, lane, t);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int row = lane + 64 * q;
@@ -442,40 +539,37 @@ __global__ __launch_bounds__(256) void k_pressure_rhs(Chan c, const double2 *dv,
         st[pp][j] = p < c.npencil ? ld2(ly + (size_t)p * N + j) : c2(0, 0);
     }
     __syncthreads();
-    const int row = threadIdx.x;
-    if (row >= N) return;
-    double2 acc[TP];
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) acc[pp] = c2(0, 0);
-    for (int j0 = 0; j0 < N; j0 += JC) {
-        double dj[JC];
-#pragma unroll
-        for (int cc = 0; cc < JC; ++cc) dj[cc] = c.DY1[(size_t)(j0 + cc < N ? j0 + cc : N - 1) * N + row];
-#pragma unroll
-        for (int cc = 0; cc < JC; ++cc) {
-            const int j = j0 + cc;
-            if (j >= N) break;
-#pragma unroll
-            for (int pp = 0; pp < TP; ++pp) {
-                acc[pp].x = fma(dj[cc], st[pp][j].x, acc[pp].x);
-                acc[pp].y = fma(dj[cc], st[pp][j].y, acc[pp].y);
+    for (int row = threadIdx.x; row < N; row += blockDim.x) {
+        double2 acc[TP];
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp) acc[pp] = c2(0, 0);
+        for (int j0 = 0; j0 < N; j0 += JC) {
+            double dj[JC];
+    #pragma unroll
+            for (int cc = 0; cc < JC; ++cc) dj[cc] = c.DY1[(size_t)(j0 + cc < N ? j0 + cc : N - 1) * N + row];
+    #pragma unroll
+            for (int cc = 0; cc < JC; ++cc) {
+                const int j = j0 + cc;
+                if (j >= N) break;
+    #pragma unroll
+                for (int pp = 0; pp < TP; ++pp) acc[pp] = c2(add(acc[pp].x, mul(dj[cc], st[pp][j].x)), add(acc[pp].y, mul(dj[cc], st[pp][j].y)));
             }
         }
-    }
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) {
-        const int p = p0 + pp;
-        if (p >= c.npencil) break;
-        const int k = p / c.NXH, i = p % c.NXH;
-        const size_t o = (size_t)p * N + row;
-        double2 r;
-        if (row == 0 || row == N - 1) {
-            r = caddc(st[pp][row], cscale(dv[o], c.nu));
-            if (row == N - 1 && k2_of(c, k, i) < 1.e-8) r = c2(0, 0);
-        } else {
-            r = caddc(caddc(imul(c.kx[i], ld2(lx + o)), imul(c.kz[k], ld2(lz + o))), acc[pp]);
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp) {
+            const int p = p0 + pp;
+            if (p >= c.npencil) break;
+            const int k = p / c.NXH, i = p % c.NXH;
+            const size_t o = (size_t)p * N + row;
+            double2 r;
+            if (row == 0 || row == N - 1) {
+                r = caddc(st[pp][row], cscale(dv[o], c.nu));
+                if (row == N - 1 && k2_of(c, k, i) < 1.e-8) r = c2(0, 0);
+            } else {
+                r = caddc(caddc(imul(c.kx[i], ld2(lx + o)), imul(c.kz[k], ld2(lz + o))), acc[pp]);
+            }
+            R[o] = r;
         }
-        R[o] = r;
     }
 }
 
@@ -501,32 +595,30 @@ __global__ __launch_bounds__(256) void k_vor_from_vel(Chan c) {
         sw[pp][j] = b;
     }
     __syncthreads();
-    const int row = threadIdx.x;
-    if (row >= N) return;
-    double2 d1u[TP], d1w[TP];
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) d1u[pp] = d1w[pp] = c2(0, 0);
-    for (int j = 0; j < N; ++j) {
-        const double a = c.DY1[(size_t)j * N + row];
-#pragma unroll
-        for (int pp = 0; pp < TP; ++pp) {
-            d1u[pp].x = fma(a, su[pp][j].x, d1u[pp].x);
-            d1u[pp].y = fma(a, su[pp][j].y, d1u[pp].y);
-            d1w[pp].x = fma(a, sw[pp][j].x, d1w[pp].x);
-            d1w[pp].y = fma(a, sw[pp][j].y, d1w[pp].y);
+    for (int row = threadIdx.x; row < N; row += blockDim.x) {
+        double2 d1u[TP], d1w[TP];
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp) d1u[pp] = d1w[pp] = c2(0, 0);
+        for (int j = 0; j < N; ++j) {
+            const double a = c.DY1[(size_t)j * N + row];
+    #pragma unroll
+            for (int pp = 0; pp < TP; ++pp) {
+                d1u[pp] = c2(add(d1u[pp].x, mul(a, su[pp][j].x)), add(d1u[pp].y, mul(a, su[pp][j].y)));
+                d1w[pp] = c2(add(d1w[pp].x, mul(a, sw[pp][j].x)), add(d1w[pp].y, mul(a, sw[pp][j].y)));
+            }
         }
-    }
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) {
-        const int p = p0 + pp;
-        if (p >= c.npencil) break;
-        const int k = p / c.NXH, i = p % c.NXH;
-        const double kx = c.kx[i], kz = c.kz[k];
-        const size_t o = (size_t)p * N + row;
-        const double2 vv = v[o];
-        ox[o] = csubc(d1w[pp], imul(kz, vv));
-        oy[o] = csubc(imul(kz, su[pp][row]), imul(kx, sw[pp][row]));
-        oz[o] = csubc(imul(kx, vv), d1u[pp]);
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp) {
+            const int p = p0 + pp;
+            if (p >= c.npencil) break;
+            const int k = p / c.NXH, i = p % c.NXH;
+            const double kx = c.kx[i], kz = c.kz[k];
+            const size_t o = (size_t)p * N + row;
+            const double2 vv = v[o];
+            ox[o] = csubc(d1w[pp], imul(kz, vv));
+            oy[o] = csubc(imul(kz, su[pp][row]), imul(kx, sw[pp][row]));
+            oz[o] = csubc(imul(kx, vv), d1u[pp]);
+        }
     }
 }
 
@@ -542,22 +634,19 @@ __global__ __launch_bounds__(256) void k_init_d2v(Chan c) {
         sv[pp][j] = p < c.npencil ? v[(size_t)p * N + j] : c2(0, 0);
     }
     __syncthreads();
-    const int row = threadIdx.x;
-    if (row >= N) return;
-    double2 acc[TP];
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) acc[pp] = c2(0, 0);
-    for (int j = 0; j < N; ++j) {
-        const double b = c.DY2[(size_t)j * N + row];
-#pragma unroll
-        for (int pp = 0; pp < TP; ++pp) {
-            acc[pp].x = fma(b, sv[pp][j].x, acc[pp].x);
-            acc[pp].y = fma(b, sv[pp][j].y, acc[pp].y);
+    for (int row = threadIdx.x; row < N; row += blockDim.x) {
+        double2 acc[TP];
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp) acc[pp] = c2(0, 0);
+        for (int j = 0; j < N; ++j) {
+            const double b = c.DY2[(size_t)j * N + row];
+    #pragma unroll
+            for (int pp = 0; pp < TP; ++pp) acc[pp] = c2(add(acc[pp].x, mul(b, sv[pp][j].x)), add(acc[pp].y, mul(b, sv[pp][j].y)));
         }
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp)
+            if (p0 + pp < c.npencil) c.d2v[(size_t)(p0 + pp) * N + row] = acc[pp];
     }
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp)
-        if (p0 + pp < c.npencil) c.d2v[(size_t)(p0 + pp) * N + row] = acc[pp];
 }
 
 // v, omega_y and DY2 v sit in slot 0 and Hv, Hg in slot 1 (written with ring 0):
@@ -646,33 +735,33 @@ __global__ __launch_bounds__(256) void k_horizontal_adj(Chan c, StepPtrs sp) {
         s1g[pp][j] = d1g;
     }
     __syncthreads();
-    const int row = threadIdx.x;
-    if (row >= N) return;
-    double2 gv[TP], gg[TP], tmp[TP];
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) gv[pp] = gg[pp] = tmp[pp] = c2(0, 0);
-    matvec_T<TP>(c.DY1, s1v, N, row, gv);
-    matvec_T<TP>(c.DY2, s2v, N, row, tmp);
-    matvec_T<TP>(c.DY1, s1g, N, row, gg);
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) {
-        const int p = p0 + pp;
-        if (p >= c.npencil) break;
-        const int k = p / c.NXH, i = p % c.NXH;
-        const size_t o = (size_t)p * N + row;
-        double2 v = caddc(gv[pp], tmp[pp]), g = gg[pp];
-        const double k2 = k2_of(c, k, i);
-        if (k2 >= 1.e-8) {
-            const double kx = c.kx[i], kz = c.kz[k];
-            const double ix = __ddiv_rn(kx, k2), iz = __ddiv_rn(kz, k2), ir = __ddiv_rn(1.0, k2);
-            const double2 U = au[o], Wv = aw[o], X = ax[o], Z = az[o];
-            v = caddc(v, caddc(imul(mul(ir, mul(kz, k2)), X), imul(-mul(ir, mul(kx, k2)), Z)));
-            g = caddc(g, caddc(imul(iz, U), imul(-ix, Wv)));
-            au[o] = aw[o] = ax[o] = az[o] = c2(0, 0);
+    for (int row = threadIdx.x; row < N; row += blockDim.x) {
+        double2 gv[TP], gg[TP], tmp[TP];
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp) gv[pp] = gg[pp] = tmp[pp] = c2(0, 0);
+        matvec_T<TP>(c.DY1, s1v, N, row, gv);
+        matvec_T<TP>(c.DY2, s2v, N, row, tmp);
+        matvec_T<TP>(c.DY1, s1g, N, row, gg);
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp) {
+            const int p = p0 + pp;
+            if (p >= c.npencil) break;
+            const int k = p / c.NXH, i = p % c.NXH;
+            const size_t o = (size_t)p * N + row;
+            double2 v = caddc(gv[pp], tmp[pp]), g = gg[pp];
+            const double k2 = k2_of(c, k, i);
+            if (k2 >= 1.e-8) {
+                const double kx = c.kx[i], kz = c.kz[k];
+                const double ix = __ddiv_rn(kx, k2), iz = __ddiv_rn(kz, k2), ir = __ddiv_rn(1.0, k2);
+                const double2 U = au[o], Wv = aw[o], X = ax[o], Z = az[o];
+                v = caddc(v, caddc(imul(mul(ir, mul(kz, k2)), X), imul(-mul(ir, mul(kx, k2)), Z)));
+                g = caddc(g, caddc(imul(iz, U), imul(-ix, Wv)));
+                au[o] = aw[o] = ax[o] = az[o] = c2(0, 0);
+            }
+            av[o] = caddc(av[o], v);
+            ag[o] = caddc(ag[o], g);
+            ad[o] = c2(0, 0);
         }
-        av[o] = caddc(av[o], v);
-        ag[o] = caddc(ag[o], g);
-        ad[o] = c2(0, 0);
     }
 }
 
@@ -690,33 +779,33 @@ __global__ __launch_bounds__(256) void k_nonlinear_adj(Chan c, StepPtrs sp) {
         sh[pp][j] = p < c.npencil ? Hv[(size_t)p * N + j] : c2(0, 0);
     }
     __syncthreads();
-    const int row = threadIdx.x;
-    if (row >= N) return;
-    double2 t[TP];
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) t[pp] = c2(0, 0);
-    matvec_T<TP>(c.DY1, sh, N, row, t);  // DY1^T Hv*; the adjoint of t is its negative
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) {
-        const int p = p0 + pp;
-        if (p >= c.npencil) break;
-        const int k = p / c.NXH, i = p % c.NXH;
-        const double kx = c.kx[i], kz = c.kz[k];
-        const double k2 = k2_of(c, k, i);
-        const size_t o = (size_t)p * N + row;
-        const double2 at = c2(-t[pp].x, -t[pp].y), hg = Hg[o];
-        double2 ax = caddc(imul(-kx, at), imul(-kz, hg));
-        double2 az = caddc(imul(-kz, at), imul(kx, hg));
-        if (p == 0) {
-            double2 *mx = c.means + (size_t)(M_LMBX * 3 + ns) * N, *mz = c.means + (size_t)(M_LMBZ * 3 + ns) * N;
-            ax = caddc(ax, mx[row]);
-            az = caddc(az, mz[row]);
-            mx[row] = mz[row] = c2(0, 0);
+    for (int row = threadIdx.x; row < N; row += blockDim.x) {
+        double2 t[TP];
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp) t[pp] = c2(0, 0);
+        matvec_T<TP>(c.DY1, sh, N, row, t);  // DY1^T Hv*; the adjoint of t is its negative
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp) {
+            const int p = p0 + pp;
+            if (p >= c.npencil) break;
+            const int k = p / c.NXH, i = p % c.NXH;
+            const double kx = c.kx[i], kz = c.kz[k];
+            const double k2 = k2_of(c, k, i);
+            const size_t o = (size_t)p * N + row;
+            const double2 at = c2(-t[pp].x, -t[pp].y), hg = Hg[o];
+            double2 ax = caddc(imul(-kx, at), imul(-kz, hg));
+            double2 az = caddc(imul(-kz, at), imul(kx, hg));
+            if (p == 0) {
+                double2 *mx = c.means + (size_t)(M_LMBX * 3 + ns) * N, *mz = c.means + (size_t)(M_LMBZ * 3 + ns) * N;
+                ax = caddc(ax, mx[row]);
+                az = caddc(az, mz[row]);
+                mx[row] = mz[row] = c2(0, 0);
+            }
+            lx[o] = caddc(lx[o], ax);
+            ly[o] = caddc(ly[o], cscale(sh[pp][row], -k2));
+            lz[o] = caddc(lz[o], az);
+            Hv[o] = Hg[o] = c2(0, 0);
         }
-        lx[o] = caddc(lx[o], ax);
-        ly[o] = caddc(ly[o], cscale(sh[pp][row], -k2));
-        lz[o] = caddc(lz[o], az);
-        Hv[o] = Hg[o] = c2(0, 0);
     }
 }
 
@@ -734,8 +823,7 @@ __device__ void friction_adj(const Chan &c, double2 *sdu, double2 *sdw, double a
         sdw[0].x += f * afz;
     }
     __syncthreads();
-    const int row = threadIdx.x;
-    if (row < N) {
+    for (int row = threadIdx.x; row < N; row += blockDim.x) {
         double2 u = c2(0, 0), w = c2(0, 0);
         const double *col = c.DY1 + (size_t)row * N;
         for (int j = 0; j < N; ++j) {
@@ -912,26 +1000,26 @@ __global__ __launch_bounds__(256) void k_init_vel_adj(Chan c, double2 *gu, doubl
         sd[pp][j] = in ? c.d2v[o] : c2(0, 0);
     }
     __syncthreads();
-    const int row = threadIdx.x;
-    if (row >= N) return;
-    double2 dz[TP], dx[TP], dd[TP];
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) dz[pp] = dx[pp] = dd[pp] = c2(0, 0);
-    matvec_T<TP>(c.DY1, sz, N, row, dz);
-    matvec_T<TP>(c.DY1, sx, N, row, dx);
-    matvec_T<TP>(c.DY2, sd, N, row, dd);
-#pragma unroll
-    for (int pp = 0; pp < TP; ++pp) {
-        const int p = p0 + pp;
-        if (p >= c.npencil) break;
-        const int k = p / c.NXH, i = p % c.NXH;
-        const double kx = c.kx[i], kz = c.kz[k];
-        const size_t o = (size_t)p * N + row;
-        const double2 Y = ay[o], X = sx[pp][row], Z = sz[pp][row];
-        // omega_x = DY1 w - i kz v; omega_y = i kz u - i kx w; omega_z = i kx v - DY1 u
-        gu[o] = csubc(caddc(au[o], imul(-kz, Y)), dz[pp]);
-        gw[o] = caddc(caddc(aw[o], imul(kx, Y)), dx[pp]);
-        gv[o] = caddc(caddc(caddc(av[o], dd[pp]), imul(kz, X)), imul(-kx, Z));
+    for (int row = threadIdx.x; row < N; row += blockDim.x) {
+        double2 dz[TP], dx[TP], dd[TP];
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp) dz[pp] = dx[pp] = dd[pp] = c2(0, 0);
+        matvec_T<TP>(c.DY1, sz, N, row, dz);
+        matvec_T<TP>(c.DY1, sx, N, row, dx);
+        matvec_T<TP>(c.DY2, sd, N, row, dd);
+    #pragma unroll
+        for (int pp = 0; pp < TP; ++pp) {
+            const int p = p0 + pp;
+            if (p >= c.npencil) break;
+            const int k = p / c.NXH, i = p % c.NXH;
+            const double kx = c.kx[i], kz = c.kz[k];
+            const size_t o = (size_t)p * N + row;
+            const double2 Y = ay[o], X = sx[pp][row], Z = sz[pp][row];
+            // omega_x = DY1 w - i kz v; omega_y = i kz u - i kx w; omega_z = i kx v - DY1 u
+            gu[o] = csubc(caddc(au[o], imul(-kz, Y)), dz[pp]);
+            gw[o] = caddc(caddc(aw[o], imul(kx, Y)), dx[pp]);
+            gv[o] = caddc(caddc(caddc(av[o], dd[pp]), imul(kz, X)), imul(-kx, Z));
+        }
     }
 }
 
@@ -997,20 +1085,37 @@ constexpr int kTPH = 4;
 
 static int blocks_tp(const Chan &c) { return (c.npencil + kTP - 1) / kTP; }
 
-// the derivative stages run on the matrix-core kernels at every NY (NY <= kMaxNY)
-void launch_horizontal(const Chan &c, const StepPtrs &sp, hipStream_t s) {
-    hipLaunchKernelGGL(k_horizontal_mfma, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
+// the batched method's derivative stages run on the matrix-core kernels at every
+// NY (NY <= kMaxNY); the plain method's in the reference's arithmetic
+void launch_horizontal(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s) {
+    if (exact)
+        hipLaunchKernelGGL(k_horizontal_exact<kTPH>, dim3((c.npencil + kTPH - 1) / kTPH), dim3(256), 0, s, c, sp);
+    else if (c.NY <= 256)
+        hipLaunchKernelGGL(k_horizontal_mfma<2>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
+    else
+        hipLaunchKernelGGL(k_horizontal_mfma<3>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
 }
 
 void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *P, int mf, hipStream_t s) {
     OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_mean<NQ>, dim3(1), dim3(256), 0, s, c, sp, F, P, mf));
 }
 
-void launch_nonlinear(const Chan &c, const StepPtrs &sp, hipStream_t s) {
-    if (c.precision == 0)
-        hipLaunchKernelGGL(k_nonlinear_mfma<double2>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
-    else
-        hipLaunchKernelGGL(k_nonlinear_mfma<float2>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
+void launch_nonlinear(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s) {
+    if (exact) {
+        if (c.precision == 0)
+            hipLaunchKernelGGL((k_nonlinear_exact<kTP, double2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, sp);
+        else
+            hipLaunchKernelGGL((k_nonlinear_exact<kTP, float2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, sp);
+    } else {
+        const dim3 g((c.npencil + kMP - 1) / kMP);
+        if (c.NY <= 256) {
+            if (c.precision == 0) hipLaunchKernelGGL((k_nonlinear_mfma<2, double2>), g, dim3(512), 0, s, c, sp);
+            else hipLaunchKernelGGL((k_nonlinear_mfma<2, float2>), g, dim3(512), 0, s, c, sp);
+        } else {
+            if (c.precision == 0) hipLaunchKernelGGL((k_nonlinear_mfma<3, double2>), g, dim3(512), 0, s, c, sp);
+            else hipLaunchKernelGGL((k_nonlinear_mfma<3, float2>), g, dim3(512), 0, s, c, sp);
+        }
+    }
 }
 
 void launch_pressure_rhs(const Chan &c, const double2 *dv, double2 *R, hipStream_t s) {

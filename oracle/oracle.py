@@ -53,6 +53,9 @@ def lib():
         L.orc_lu_solve.argtypes = [I, P, I, P]
         L.orc_lu_factor.argtypes = [I, P, P]
         L.orc_seeded_gauss.argtypes = [U64, I, I, I, I, P]
+        L.orc_set_noise.argtypes = [P, U64, D, I]
+        L.orc_set_mutation.argtypes = [P, I]
+        L.orc_set_fft_variant.argtypes = [P, I]
         _lib = L
     return _lib
 
@@ -108,6 +111,23 @@ class Oracle:
 
     def step(self, n=1):
         self._check(lib().orc_step(self._h, int(n)))
+
+    # test instruments (owt_oracle.cpp, Oracle::perturb and Oracle::mutate)
+    NOISE_LAMB, NOISE_SOLVE = 1, 2
+    MUT_HG_SIGN, MUT_FRICTION_SIGN, MUT_U_OMY_SIGN, MUT_HG_KX_SIGN = 1, 2, 4, 8
+
+    def set_noise(self, seed, rel, where):
+        """Multiply the fields named by `where` (NOISE_LAMB | NOISE_SOLVE) by (1 + rel g),
+        g ~ N(0, 1), at every step: the spread that rounding alone causes."""
+        self._check(lib().orc_set_noise(self._h, int(seed), float(rel), int(where)))
+
+    def set_fft_variant(self, variant):
+        """1: transforms by a second, equally valid FFT factorization (radix 2 first)."""
+        self._check(lib().orc_set_fft_variant(self._h, int(variant)))
+
+    def set_mutation(self, bits):
+        """Planted mistakes (MUT_*) that the oracle's pins must detect."""
+        self._check(lib().orc_set_mutation(self._h, int(bits)))
 
     def field(self, name):
         out = np.empty(self.wave_shape, dtype=np.complex128)

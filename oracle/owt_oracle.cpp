@@ -33,6 +33,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -294,9 +295,12 @@ struct FFTPlan {
     }
 };
 
-void fft_rec(int n, const cd *x, int stride, cd *out, const std::vector<cd> &W, int N0, cd *tmp) {
+// variant 0 splits off radix 4, then 2, then 3; variant 1 (a test instrument: a
+// second, equally valid factorization whose rounding differs) radix 2, then 3.
+void fft_rec(int n, const cd *x, int stride, cd *out, const std::vector<cd> &W, int N0, cd *tmp, int variant = 0) {
     if (n == 1) { out[0] = x[0]; return; }
-    int p = (n % 4 == 0) ? 4 : (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : n;
+    int p = variant == 1 ? ((n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : n)
+                         : ((n % 4 == 0) ? 4 : (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : n);
     int m = n / p;
     if (p == n) {  // plain DFT for a prime length
         for (int k = 0; k < n; ++k) {
@@ -307,7 +311,7 @@ void fft_rec(int n, const cd *x, int stride, cd *out, const std::vector<cd> &W, 
         return;
     }
     // sub-transforms of the p decimated sequences, stored contiguously in tmp
-    for (int r = 0; r < p; ++r) fft_rec(m, x + (size_t)r * stride, stride * p, tmp + (size_t)r * m, W, N0, out);
+    for (int r = 0; r < p; ++r) fft_rec(m, x + (size_t)r * stride, stride * p, tmp + (size_t)r * m, W, N0, out, variant);
     int step = N0 / n;
     for (int k = 0; k < m; ++k) {
         cd y[4];
@@ -367,6 +371,36 @@ struct Oracle {
     double last_bottom = 0, last_upper = 0;
     long id_step = 0;
     std::vector<double> diag;  // (fx_fric, bottom, upper) per friction evaluation
+
+    // Test instruments (no reference counterpart; off by default):
+    //  noise: every value of the fields named by noise_where is multiplied by
+    //    (1 + noise_rel g), g ~ N(0, 1) from a stream keyed by (noise_seed, step,
+    //    field): bit 1 the Lamb vector after each evaluation (the transforms'
+    //    rounding), bit 2 v and omega_y after each solve (the solves' rounding).
+    //    Runs with noise give the spread that rounding alone causes, the floor
+    //    below which two correct implementations cannot agree.
+    //  mutate: planted mistakes that the pins in tests/ must detect: bit 1 the
+    //    sign of Hg, bit 2 the sign of the friction forcing of the mean flow,
+    //    bit 4 the sign of the omega_y term of the horizontal recovery of u,
+    //    bit 8 the sign of the i kx Lz term of Hg.
+    //  fft_variant: 1 transforms with the second factorization of fft_rec.
+    int fft_variant = 0;
+    uint64_t noise_seed = 0;
+    double noise_rel = 0.0;
+    int noise_where = 0;
+    int mutate = 0;
+
+    void perturb(std::vector<cd> &f, int tag) {
+        uint64_t h = splitmix(noise_seed ^ 0x243F6A8885A308D3ULL);
+        h = splitmix(h ^ (uint64_t)id_step);
+        h = splitmix(h ^ (uint64_t)tag);
+        std::mt19937_64 gen(h);
+        std::normal_distribution<double> g(0.0, 1.0);
+        for (cd &z : f) {
+            const double a = g(gen), b = g(gen);
+            z = cd(z.real() * (1.0 + noise_rel * a), z.imag() * (1.0 + noise_rel * b));
+        }
+    }
 
     size_t at(int k, int i) const { return ((size_t)k * NXH + i) * NY; }
 
@@ -445,6 +479,43 @@ struct Oracle {
     }
 
     // ---------------- transforms, src/FFT.cpp:49-194 ----------------
+    // z inverse of one wave field into the z-padded [NZ2][NXH][NY] (src/FFT.cpp:68-95)
+    void zinv(const cd *u, std::vector<cd> &I) const {
+#pragma omp parallel
+        {
+            std::vector<cd> a(NZ2), b(NZ2), tmp(NZ2);
+#pragma omp for collapse(2) schedule(static)
+            for (int i = 0; i < NXH; ++i)
+                for (int j = 0; j < NY; ++j) {
+                    for (int k = 0; k < NZ2; ++k) a[k] = 0.0;
+                    for (int k = 0; k < NZH; ++k) {
+                        a[k] = u[at(k, i) + j];
+                        a[k + NZ] = u[at(k + NZH, i) + j];
+                    }
+                    a[NZ] = 0.0;
+                    fft_rec(NZ2, a.data(), 1, b.data(), pz.w_bwd, NZ2, tmp.data(), fft_variant == 1);
+                    for (int k = 0; k < NZ2; ++k) I[((size_t)k * NXH + i) * NY + j] = b[k];
+                }
+        }
+    }
+    // z forward of a z-padded field into a wave field (src/FFT.cpp:171-190)
+    void zfwd(const std::vector<cd> &I, cd *u) const {
+#pragma omp parallel
+        {
+            std::vector<cd> a(NZ2), b(NZ2), tmp(NZ2);
+#pragma omp for collapse(2) schedule(static)
+            for (int i = 0; i < NXH; ++i)
+                for (int j = 0; j < NY; ++j) {
+                    for (int k = 0; k < NZ2; ++k) a[k] = I[((size_t)k * NXH + i) * NY + j];
+                    fft_rec(NZ2, a.data(), 1, b.data(), pz.w_fwd, NZ2, tmp.data(), fft_variant == 1);
+                    for (int k = 0; k < NZH; ++k) {
+                        u[at(k, i) + j] = b[k] / (double)NZ2;
+                        u[at(k + NZH, i) + j] = b[k + NZ] / (double)NZ2;
+                    }
+                }
+        }
+    }
+
     // wave [NZ][NXH][NY] complex -> physical [NZ2][NX2][NY] real
     void f2p(const cd *u, double *up) const {
         std::vector<cd> I((size_t)NZ2 * NXH * NY);
@@ -460,7 +531,7 @@ struct Oracle {
                         a[k + NZ] = u[at(k + NZH, i) + j];
                     }
                     a[NZ] = 0.0;  // z-Nyquist dropped (src/FFT.cpp:86-87)
-                    fft_rec(NZ2, a.data(), 1, b.data(), pz.w_bwd, NZ2, tmp.data());
+                    fft_rec(NZ2, a.data(), 1, b.data(), pz.w_bwd, NZ2, tmp.data(), fft_variant == 1);
                     for (int k = 0; k < NZ2; ++k) I[((size_t)k * NXH + i) * NY + j] = b[k];
                 }
             std::vector<cd> c(NX2), d(NX2);
@@ -472,7 +543,7 @@ struct Oracle {
                         c[i] = I[((size_t)k * NXH + i) * NY + j];
                         if (i > 0) c[i + NX] = std::conj(I[((size_t)k * NXH + (NXH - i)) * NY + j]);
                     }
-                    fft_rec(NX2, c.data(), 1, d.data(), px.w_bwd, NX2, tmp.data());
+                    fft_rec(NX2, c.data(), 1, d.data(), px.w_bwd, NX2, tmp.data(), fft_variant == 1);
                     for (int i = 0; i < NX2; ++i) up[((size_t)k * NX2 + i) * NY + j] = d[i].real();
                 }
         }
@@ -488,7 +559,7 @@ struct Oracle {
             for (int k = 0; k < NZ2; ++k)
                 for (int j = 0; j < NY; ++j) {
                     for (int i = 0; i < NX2; ++i) c[i] = cd(up[((size_t)k * NX2 + i) * NY + j], 0.0);
-                    fft_rec(NX2, c.data(), 1, d.data(), px.w_fwd, NX2, tmp.data());
+                    fft_rec(NX2, c.data(), 1, d.data(), px.w_fwd, NX2, tmp.data(), fft_variant == 1);
                     for (int i = 0; i < NXH; ++i) I[((size_t)k * NXH + i) * NY + j] = d[i] / (double)NX2;
                 }
             std::vector<cd> a(NZ2), b(NZ2);
@@ -496,7 +567,7 @@ struct Oracle {
             for (int i = 0; i < NXH; ++i)
                 for (int j = 0; j < NY; ++j) {
                     for (int k = 0; k < NZ2; ++k) a[k] = I[((size_t)k * NXH + i) * NY + j];
-                    fft_rec(NZ2, a.data(), 1, b.data(), pz.w_fwd, NZ2, tmp.data());
+                    fft_rec(NZ2, a.data(), 1, b.data(), pz.w_fwd, NZ2, tmp.data(), fft_variant == 1);
                     for (int k = 0; k < NZH; ++k) {
                         u[at(k, i) + j] = b[k] / (double)NZ2;
                         u[at(k + NZH, i) + j] = b[k + NZ] / (double)NZ2;
@@ -541,16 +612,91 @@ struct Oracle {
                     matvec(DY1, t.data(), d.data());
                     for (int j = 0; j < NY; ++j) Hv[o + j] = (-k2) * lmby[o + j] - d[j];
                     for (int j = 0; j < NY; ++j) Hg[o + j] = ikz * lmbx[o + j] - ikx * lmbz[o + j];
+                    if (mutate & 8)
+                        for (int j = 0; j < NY; ++j) Hg[o + j] = ikz * lmbx[o + j] + ikx * lmbz[o + j];
+                    if (mutate & 1)
+                        for (int j = 0; j < NY; ++j) Hg[o + j] = -Hg[o + j];
                 }
         }
     }
 
     // src/solver.cpp:262-304
+    // Test instrument: the x transforms with two real fields per complex
+    // transform (Z = A + i B with Hermitian extensions; the forward split
+    // X_m = (Z_m + conj Z_{L-m}) / 2, Y_m = (Z_m - conj Z_{L-m}) / 2i), whose
+    // rounding is relative to the larger field of a pair. dc: the kx = 0 modes
+    // are left out of the packed inverse transform and added to its output.
+    void f2p_pair(const cd *A, const cd *B, double *ap, double *bp, bool dc) const {
+        std::vector<cd> IA((size_t)NZ2 * NXH * NY), IB(IA.size());
+        zinv(A, IA);
+        zinv(B, IB);
+#pragma omp parallel
+        {
+            std::vector<cd> c(NX2), d(NX2), tmp(NX2);
+#pragma omp for collapse(2) schedule(static)
+            for (int j = 0; j < NY; ++j)
+                for (int k = 0; k < NZ2; ++k) {
+                    for (int i = 0; i < NX2; ++i) c[i] = 0.0;
+                    const size_t b0 = ((size_t)k * NXH) * NY + j;
+                    const double a0 = IA[b0].real(), c0 = IB[b0].real();
+                    c[0] = dc ? cd(0.0, 0.0) : cd(a0, c0);
+                    for (int i = 1; i < NXH; ++i) {
+                        const cd a = IA[b0 + (size_t)i * NY], b = IB[b0 + (size_t)i * NY];
+                        c[i] = cd(a.real() - b.imag(), a.imag() + b.real());
+                        c[NX2 - i] = cd(a.real() + b.imag(), b.real() - a.imag());
+                    }
+                    fft_rec(NX2, c.data(), 1, d.data(), px.w_bwd, NX2, tmp.data(), 0);
+                    for (int i = 0; i < NX2; ++i) {
+                        ap[((size_t)k * NX2 + i) * NY + j] = dc ? d[i].real() + a0 : d[i].real();
+                        bp[((size_t)k * NX2 + i) * NY + j] = dc ? d[i].imag() + c0 : d[i].imag();
+                    }
+                }
+        }
+    }
+    void p2f_pair(const double *ap, const double *bp, cd *A, cd *B) const {
+        std::vector<cd> IA((size_t)NZ2 * NXH * NY), IB(IA.size());
+#pragma omp parallel
+        {
+            std::vector<cd> c(NX2), d(NX2), tmp(NX2);
+#pragma omp for collapse(2) schedule(static)
+            for (int k = 0; k < NZ2; ++k)
+                for (int j = 0; j < NY; ++j) {
+                    for (int i = 0; i < NX2; ++i)
+                        c[i] = cd(ap[((size_t)k * NX2 + i) * NY + j], bp ? bp[((size_t)k * NX2 + i) * NY + j] : 0.0);
+                    fft_rec(NX2, c.data(), 1, d.data(), px.w_fwd, NX2, tmp.data(), 0);
+                    const double sc = (double)NX2;
+                    for (int i = 0; i < NXH; ++i) {
+                        const cd zn = d[i], zm = d[(NX2 - i) % NX2];
+                        const size_t o = ((size_t)k * NXH + i) * NY + j;
+                        if (bp) {
+                            IA[o] = cd((zn.real() + zm.real()) * 0.5 / sc, (zn.imag() - zm.imag()) * 0.5 / sc);
+                            IB[o] = cd((zn.imag() + zm.imag()) * 0.5 / sc, (zm.real() - zn.real()) * 0.5 / sc);
+                        } else {
+                            IA[o] = zn / sc;
+                        }
+                    }
+                }
+        }
+        zfwd(IA, A);
+        if (bp) zfwd(IB, B);
+    }
+
     void lamb_vec() {
         size_t P = (size_t)NZ2 * NX2 * NY;
         std::vector<double> up(P), vp(P), wp(P), oxp(P), oyp(P), ozp(P);
-        f2p(velx.data(), up.data()); f2p(vely.data(), vp.data()); f2p(velz.data(), wp.data());
-        f2p(vorx.data(), oxp.data()); f2p(vory.data(), oyp.data()); f2p(vorz.data(), ozp.data());
+        if (fft_variant == 2) {  // pairs (u, v), (w, omx), (omy, omz); (lx, ly), (lz)
+            f2p_pair(velx.data(), vely.data(), up.data(), vp.data(), false);
+            f2p_pair(velz.data(), vorx.data(), wp.data(), oxp.data(), false);
+            f2p_pair(vory.data(), vorz.data(), oyp.data(), ozp.data(), false);
+        } else if (fft_variant == 3 || fft_variant == 4) {  // pairs (u, omz), (v, w), (omx, omy); (lx, lz), (ly)
+            const bool dc = fft_variant == 4;
+            f2p_pair(velx.data(), vorz.data(), up.data(), ozp.data(), dc);
+            f2p_pair(vely.data(), velz.data(), vp.data(), wp.data(), dc);
+            f2p_pair(vorx.data(), vory.data(), oxp.data(), oyp.data(), dc);
+        } else {
+            f2p(velx.data(), up.data()); f2p(vely.data(), vp.data()); f2p(velz.data(), wp.data());
+            f2p(vorx.data(), oxp.data()); f2p(vory.data(), oyp.data()); f2p(vorz.data(), ozp.data());
+        }
         std::vector<double> lx(P), ly(P), lz(P);
 #pragma omp parallel for schedule(static)
         for (size_t e = 0; e < P; ++e) {
@@ -558,7 +704,16 @@ struct Oracle {
             ly[e] = wp[e] * oxp[e] - up[e] * ozp[e];
             lz[e] = up[e] * oyp[e] - vp[e] * oxp[e];
         }
-        p2f(lx.data(), lmbx.data()); p2f(ly.data(), lmby.data()); p2f(lz.data(), lmbz.data());
+        if (fft_variant == 2) {
+            p2f_pair(lx.data(), ly.data(), lmbx.data(), lmby.data());
+            p2f_pair(lz.data(), nullptr, lmbz.data(), nullptr);
+        } else if (fft_variant == 3 || fft_variant == 4) {
+            p2f_pair(lx.data(), lz.data(), lmbx.data(), lmbz.data());
+            p2f_pair(ly.data(), nullptr, lmby.data(), nullptr);
+        } else {
+            p2f(lx.data(), lmbx.data()); p2f(ly.data(), lmby.data()); p2f(lz.data(), lmbz.data());
+        }
+        if (noise_where & 1) { perturb(lmbx, 1); perturb(lmby, 2); perturb(lmbz, 3); }
         lmbx_mean[2] = lmbx_mean[1]; lmbx_mean[1] = lmbx_mean[0];
         lmbx_mean[0].assign(lmbx.begin(), lmbx.begin() + NY);
         lmbz_mean[2] = lmbz_mean[1]; lmbz_mean[1] = lmbz_mean[0];
@@ -587,15 +742,16 @@ struct Oracle {
         for (size_t e = 0; e < Am.a.size(); ++e) Am.a[e] = (-nu) * DY2.a[e];
         for (int i = 0; i < NY; ++i) Am(i, i) = Am(i, i) + g0 / dt;
         std::vector<cd> su(NY), sw(NY);
+        const double fs = (mutate & 2) ? -1.0 : 1.0;
         for (int j = 0; j < NY; ++j) {
-            cd t0 = b0 * (lmbx_mean[0][j] + fx_h[0]);
-            cd t1 = b1 * (lmbx_mean[1][j] + fx_h[1]);
-            cd t2 = b2 * (lmbx_mean[2][j] + fx_h[2]);
+            cd t0 = b0 * (lmbx_mean[0][j] + fs * fx_h[0]);
+            cd t1 = b1 * (lmbx_mean[1][j] + fs * fx_h[1]);
+            cd t2 = b2 * (lmbx_mean[2][j] + fs * fx_h[2]);
             cd va = (a0 * velx_mean[0][j] + a1 * velx_mean[1][j]) + a2 * velx_mean[2][j];
             su[j] = ((t0 + t1) + t2) + cd(va.real() / dt, va.imag() / dt);
-            t0 = b0 * (lmbz_mean[0][j] + fz_h[0]);
-            t1 = b1 * (lmbz_mean[1][j] + fz_h[1]);
-            t2 = b2 * (lmbz_mean[2][j] + fz_h[2]);
+            t0 = b0 * (lmbz_mean[0][j] + fs * fz_h[0]);
+            t1 = b1 * (lmbz_mean[1][j] + fs * fz_h[1]);
+            t2 = b2 * (lmbz_mean[2][j] + fs * fz_h[2]);
             va = (a0 * velz_mean[0][j] + a1 * velz_mean[1][j]) + a2 * velz_mean[2][j];
             sw[j] = ((t0 + t1) + t2) + cd(va.real() / dt, va.imag() / dt);
         }
@@ -632,7 +788,7 @@ struct Oracle {
                     matvec(DY1, &vory[o], d1g.data());
                     cd ix = cd(0.0, 1.0) * kxx / k2, iz = cd(0.0, 1.0) * kzz / k2, ir = cd(0.0, 1.0) / k2;
                     for (int j = 0; j < NY; ++j) {
-                        velx[o + j] = ix * d1v[j] - iz * vory[o + j];
+                        velx[o + j] = (mutate & 4) ? ix * d1v[j] + iz * vory[o + j] : ix * d1v[j] - iz * vory[o + j];
                         velz[o + j] = ix * vory[o + j] + iz * d1v[j];
                         vorx[o + j] = ir * ((kxx * d1g[j] + kzz * d2v[j]) - (kzz * k2) * vely[o + j]);
                         vorz[o + j] = ir * ((kzz * d1g[j] - kxx * d2v[j]) + (kxx * k2) * vely[o + j]);
@@ -752,6 +908,7 @@ struct Oracle {
                 }
         }
         if (errflag) throw std::runtime_error("LU factorization failed. Matrix A may be singular.");
+        if (noise_where & 2) { perturb(vely, 4); perturb(vory, 5); }
         solve_horizontal();
         lamb_vec();
         nonlinear_source();
@@ -952,6 +1109,23 @@ int orc_step(void *h, int n) {
         g_err = e.what();
         return -1;
     }
+}
+
+// test instruments: rounding noise (see Oracle::perturb) and planted mistakes
+int orc_set_noise(void *h, uint64_t seed, double rel, int where) {
+    Oracle *o = (Oracle *)h;
+    o->noise_seed = seed;
+    o->noise_rel = rel;
+    o->noise_where = where;
+    return 0;
+}
+int orc_set_fft_variant(void *h, int variant) {
+    ((Oracle *)h)->fft_variant = variant;
+    return 0;
+}
+int orc_set_mutation(void *h, int bits) {
+    ((Oracle *)h)->mutate = bits;
+    return 0;
 }
 
 // dynamic pressure (field 11) from given Lamb vector components and v

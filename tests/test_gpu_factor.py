@@ -4,9 +4,15 @@ k_factor) against the oracle's dgetrf restatement of the same matrices.
 The operators are assembled here from the library's own DY matrices with the
 element expressions of src/solver.cpp:577-591 (Av) and 601-603 (Ag), one
 IEEE rounding per operation as numpy float64 does. The pivot rows must agree
-exactly (LAPACK's rule: first largest |a| in the interchanged order); the
-factors agree to rounding (the GPU update fuses multiply and add).
+exactly (LAPACK's rule: first largest |a| in the interchanged order), and so
+must the factors, bit for bit: the GPU rounds every product and difference of
+the elimination separately and applies each element's updates in ascending
+pivot step, as the reference BLAS does. Above 256 rows the factorization runs
+by panels (k_factor_panel); forced at 193 rows it gives the register kernel's
+factors exactly.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -44,7 +50,7 @@ def ipiv_to_perm(ipiv):
     return perm
 
 
-@pytest.mark.parametrize("shape", [(8, 17, 8), (16, 65, 16), (8, 129, 8), (8, 193, 8)])
+@pytest.mark.parametrize("shape", [(8, 17, 8), (16, 65, 16), (8, 129, 8), (8, 193, 8), (8, 257, 8), (8, 288, 8)])
 def test_factor_matches_oracle(owt_mod, oracle_mod, shape):
     NX, NY, NZ = shape
     p = dict(RE180, NX=NX, NY=NY, NZ=NZ, precision=0, method=1)
@@ -67,12 +73,34 @@ def test_factor_matches_oracle(owt_mod, oracle_mod, shape):
             assert info == 0
             lu, perm = ch.factor(sys, op)
             assert np.array_equal(perm, ipiv_to_perm(ipiv)), (sys, op)
-            # column by column, relative to the column's largest entry
-            err = (np.abs(lu - LU).max(axis=0) / np.abs(LU).max(axis=0)).max()
-            assert err < 1e-11, (sys, op, err)
+            assert np.array_equal(lu, LU), (sys, op, np.abs(lu - LU).max())
             # and the factorization itself: L U = A with rows in pivot order
             L = np.tril(lu, -1) + np.eye(NY)
             U = np.triu(lu)
             res = np.abs(L @ U - A[perm]).max() / np.abs(A).max()
             assert res < 1e-13, (sys, op, res)
     ch.close()
+
+
+def test_panel_factorization_equals_register_factorization(owt_mod):
+    """OWT_FACTOR_PANEL=1 forces the panel kernel; at 193 rows both kernels run."""
+    p = dict(RE180, NX=16, NY=193, NZ=16, precision=0, method=1)
+    print("[gpu] factor panel vs register 16x193x16", flush=True)
+    reg = owt_mod.Channel(params=p)
+    os.environ["OWT_FACTOR_PANEL"] = "1"
+    try:
+        pan = owt_mod.Channel(params=p)
+    finally:
+        del os.environ["OWT_FACTOR_PANEL"]
+    for sys in range(8 * 9):
+        for op in (0, 1):
+            a, pa = reg.factor(sys, op)
+            b, pb = pan.factor(sys, op)
+            assert np.array_equal(pa, pb) and np.array_equal(a, b), (sys, op)
+    for ch in (reg, pan):
+        ch.init_fields(2, seed=1, amp=0.15)
+        ch.step(3)
+    for name in ("u", "v", "w"):
+        assert np.array_equal(reg.field(name), pan.field(name)), name
+    reg.close()
+    pan.close()

@@ -8,7 +8,8 @@ Tolerances (relative L2 over a whole field, ||a - b|| / ||b||):
                                              amplify the last-bit differences of fused
                                              multiply-adds (GPU) against separately
                                              rounded products (oracle)
-  batched vs plain method                    1e-13 (same factors, same arithmetic)
+  batched vs plain method                    3x the rounding floor (the batched method
+                                             reorders the arithmetic; see test_gpu_floors.py)
   mixed precision vs FP64 after 20 steps     5e-6 on velocity, 2e-5 on p (from the FP32
                                              Lamb vector), 1e-6 on wall shear
 """
@@ -52,7 +53,8 @@ def orc(oracle_mod, p):
 
 
 # padded lengths 24..1152 cover every transform tile shape (<= 96, 192, 576, 1152)
-@pytest.mark.parametrize("shape", [(16, 17, 16), (64, 65, 64), (12, 33, 24), (128, 9, 96), (384, 9, 384), (768, 7, 32)])
+@pytest.mark.parametrize("shape", [(16, 17, 16), (64, 65, 64), (12, 33, 24), (128, 9, 96), (384, 9, 384), (768, 7, 32),
+                                   (320, 9, 320), (640, 7, 64), (16, 7, 640), (40, 7, 20)])
 def test_transforms_match_oracle(owt_mod, oracle_mod, shape):
     p = params(*shape)
     ch, o = gpu(owt_mod, p), orc(oracle_mod, p)
@@ -66,7 +68,8 @@ def test_transforms_match_oracle(owt_mod, oracle_mod, shape):
         assert rel(back[q], o.p2f(P[q])) < 1e-13
 
 
-@pytest.mark.parametrize("shape", [(16, 17, 16), (64, 65, 64), (128, 9, 128), (384, 9, 384), (768, 7, 32)])
+@pytest.mark.parametrize("shape", [(16, 17, 16), (64, 65, 64), (128, 9, 128), (384, 9, 384), (768, 7, 32), (320, 9, 320),
+                                   (640, 7, 64)])
 def test_lamb_matches_oracle(owt_mod, oracle_mod, shape):
     p = params(*shape)
     ch, o = gpu(owt_mod, p), orc(oracle_mod, p)
@@ -124,8 +127,16 @@ def test_ten_step_parity_64(ten_steps_64, method):
 
 
 def test_batched_equals_plain(ten_steps_64):
-    for name in FIELDS:
-        assert rel(ten_steps_64[1][name], ten_steps_64[0][name]) < 1e-13, name
+    """The plain method keeps the reference's arithmetic; the batched method
+    reorders it (matrix-core mat-vecs, stored DY2 v, fused substitution). Each
+    is within 1.5 times the rounding floor of the oracle (test_gpu_floors.py),
+    so they differ from each other by at most three times that floor."""
+    import json
+    floor = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "floors.json")))["re180_64"]["floor"]
+    for name in ("u", "v", "w", "omx", "omy", "omz", "p"):
+        e = rel(ten_steps_64[1][name], ten_steps_64[0][name])
+        print(f"[parity] batched vs plain {name} {e:.2e} (floor {floor[name]:.2e})", flush=True)
+        assert e <= 3 * floor[name], (name, e, floor[name])
 
 
 def test_reference_initial_wave_parity(owt_mod, oracle_mod):
@@ -139,41 +150,6 @@ def test_reference_initial_wave_parity(owt_mod, oracle_mod):
         ch.init_fields(1)
         ch.step(10)
         check_fields({n: ch.field(n) for n in FIELDS}, {n: o.field(n) for n in FIELDS}, f"wave 16x33x16 method {method}")
-
-
-def test_ten_step_parity_128(owt_mod, oracle_mod):
-    """Re_tau 180, box 4pi x 2 x 4pi/3, 128x129x128: first 10 steps against the oracle.
-
-    At this resolution the v equation's operator (nu DY4 with the tanh-stretched
-    wall spacing) is conditioned so that one unit in the last place of the
-    initial field moves v and w after 10 steps by a few 1e-10 relative: the
-    oracle run twice, from the seeded field and from that field perturbed by
-    one ulp, differs by that much from itself. Two correct implementations can
-    agree no better, so the bar is the larger of 1e-10 and 4x that measured
-    floor, field by field.
-    """
-    p = params(128, 129, 128, dkz=1.5)
-    names = ("u", "v", "w", "omx", "omy", "omz")
-    o = orc(oracle_mod, p)
-    o.init(2, seed=0, amp=0.15)
-    o2 = orc(oracle_mod, p)
-    o2.init(2, seed=0, amp=0.15)
-    rng = np.random.default_rng(0)
-    uvw = [o2.field(n) for n in ("u", "v", "w")]
-    o2.set_velocity(*(f * (1 + 2.0 ** -52 * rng.standard_normal(f.shape)) for f in uvw))
-    o.step(10)
-    o2.step(10)
-    ref = {n: o.field(n) for n in names}
-    floor = {n: rel(o2.field(n), ref[n]) for n in names}
-    ch = gpu(owt_mod, p, method=1)
-    ch.init_fields(2, seed=0, amp=0.15)
-    ch.step(10)
-    err = {n: rel(ch.field(n), ref[n]) for n in names}
-    print("[parity] 128x129x128 gpu:   " + " ".join(f"{n}={err[n]:.2e}" for n in names), flush=True)
-    print("[parity] 128x129x128 floor: " + " ".join(f"{n}={floor[n]:.2e}" for n in names), flush=True)
-    for n in names:
-        assert err[n] < max(1e-10, 4 * floor[n]), (n, err[n], floor[n])
-    np.testing.assert_allclose(ch.diag(), o.diag(), rtol=1e-10)
 
 
 def test_mixed_precision_close_to_fp64(owt_mod):
