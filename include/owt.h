@@ -133,6 +133,15 @@ int owt_restore(owt_handle *h, const char *path);
  * on nf fields (FP64); physical layout [NZ2][NX2][NY] */
 int owt_f2p(owt_handle *h, const double *wave, int nf, double *phys);
 int owt_p2f(owt_handle *h, const double *phys, int nf, double *wave);
+/* owt_f2p without any allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). wave_dev and phys_dev
+ * are device buffers on h's GPU ([nf][NZ][NXH][NY] complex and
+ * [nf][NZ2][NX2][NY] double); the z-padded intermediate uses a scratch array
+ * the library holds on each GPU (64 MiB), so nf * NZ2 * NXH * NY * 16 bytes
+ * must fit in it, else OWT_E_ARG. Runs on h's stream and returns when done.
+ * Calls that overlap on one GPU share the scratch array and race. */
+int owt_f2p_noalloc(owt_handle *h, const double *wave_dev, int nf, double *phys_dev);
 /* Lamb vector of (u, v, w, omx, omy, omz) through the step's transform kernels
  * (src/solver.cpp:262-291) in the handle's precision; three output fields */
 int owt_lamb(owt_handle *h, const double *six, double *three);

@@ -344,6 +344,11 @@ int owt_f2p(owt_handle *h, const double *wave, int nf, double *phys) {
     return guard(h, [&] { h->ch->f2p(wave, nf, phys); });
 }
 
+int owt_f2p_noalloc(owt_handle *h, const double *wave_dev, int nf, double *phys_dev) {
+    if (!wave_dev || !phys_dev) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->f2p_noalloc(wave_dev, nf, phys_dev); });
+}
+
 int owt_p2f(owt_handle *h, const double *phys, int nf, double *wave) {
     if (!wave || !phys) return OWT_E_ARG;
     return guard(h, [&] { h->ch->p2f(phys, nf, wave); });

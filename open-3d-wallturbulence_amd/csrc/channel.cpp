@@ -67,6 +67,7 @@ Channel::Channel(const Params &p, int device, bool verbose, hipStream_t stream) 
     } else {
         OWT_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     }
+    noalloc_ = noalloc_scratch(device);
     op_ = build_operators((int)P_.NY, P_.alpha_mesh);
     try {
         alloc();
@@ -776,6 +777,28 @@ void Channel::f2p(const double *wave, int nf, double *phys) {
     (void)hipFree(dw);
     (void)hipFree(dz);
     (void)hipFree(dp);
+}
+
+// f2p on device buffers with the intermediate in the scratch array: no
+// allocation. Every element of the intermediate that the c2r transform reads is
+// written by the z inverse first, so the scratch needs no clearing; calls that
+// overlap on one GPU share the array and race.
+void Channel::f2p_noalloc(const double *wave_dev, int nf, double *phys_dev) {
+    if (nf < 1 || nf > 6) throw Error(2, "f2p takes 1..6 fields");
+    if (!wave_dev || !phys_dev) throw Error(2, "f2p_noalloc: null device buffer");
+    const size_t W = c_.W, need = (size_t)nf * c_.I * sizeof(double2);
+    if (need > noalloc_scratch_bytes())
+        throw Error(2, "f2p_noalloc: " + std::to_string(nf) + " fields need " + std::to_string(need) +
+                           " bytes of scratch; the limit is " + std::to_string(noalloc_scratch_bytes()));
+    OWT_HIP(hipSetDevice(device_));
+    Chan d = c_;
+    d.precision = 0;
+    const double2 *in[6];
+    for (int f = 0; f < nf; ++f) in[f] = reinterpret_cast<const double2 *>(wave_dev) + f * W;
+    launch_z_inverse(d, in, nf, noalloc_, s_);
+    launch_x_c2r(d, noalloc_, nf, phys_dev, s_);
+    OWT_HIP(hipGetLastError());
+    OWT_HIP(hipStreamSynchronize(s_));
 }
 
 void Channel::p2f(const double *phys, int nf, double *wave) {

@@ -51,6 +51,9 @@ class Channel {
 
     // stand-alone transforms, FP64 (stage-level parity)
     void f2p(const double *wave, int nf, double *phys);
+    // the same on caller-owned device buffers, without allocating: the z-padded
+    // intermediate lives in the device's scratch array (noalloc.hip)
+    void f2p_noalloc(const double *wave_dev, int nf, double *phys_dev);
     void p2f(const double *phys, int nf, double *wave);
     void lamb(const double *six, double *three);
 
@@ -137,6 +140,7 @@ class Channel {
     Operators op_;
     Chan c_{};
     int device_ = 0;
+    double2 *noalloc_ = nullptr;  // this device's scratch array (noalloc.hip)
     bool verbose_ = false;
     hipStream_t s_ = nullptr;
     std::vector<void *> allocs_;

@@ -92,6 +92,12 @@ struct StepPtrs {
     int s[3], sn;
 };
 
+// ---- scratch of the allocation-free entry points (noalloc.hip) ----
+// device address of the file-scope scratch array on `device`, looked up on the
+// first call for that device (which must be current); and its size in bytes
+double2 *noalloc_scratch(int device);
+size_t noalloc_scratch_bytes();
+
 // ---- FFT phase launches (fft_kernels.hip) ----
 // z inverse: nf wave fields -> nf z-padded fields; fields given as pointer table
 void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out, hipStream_t s);

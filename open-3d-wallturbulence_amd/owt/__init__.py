@@ -39,6 +39,7 @@ EXPORTS = [
     "owt_ensemble_init_fields", "owt_ensemble_step", "owt_ensemble_time_steps",
     "owt_tangent_init", "owt_tangent_handle", "owt_tangent_step", "owt_adjoint_run",
     "owt_autotune", "owt_get_tune", "owt_set_tune", "owt_save_fields", "owt_statistics",
+    "owt_f2p_noalloc",
 ]
 
 
@@ -96,6 +97,7 @@ def lib():
         L.owt_restore.argtypes = [P, ctypes.c_char_p]
         L.owt_f2p.argtypes = [P, P, I, P]
         L.owt_p2f.argtypes = [P, P, I, P]
+        L.owt_f2p_noalloc.argtypes = [P, P, I, P]
         L.owt_lamb.argtypes = [P, P, P]
         L.owt_get_matrix.argtypes = [P, I, P]
         L.owt_time_steps.argtypes = [P, I, P]
@@ -254,6 +256,10 @@ class Channel:
         out = np.empty((nf,) + self.phys_shape, dtype=np.float64)
         self._check(lib().owt_f2p(self._h, _p(wave), nf, _p(out)))
         return out if nf > 1 else out[0]
+
+    def f2p_noalloc(self, wave_dev, nf, phys_dev):
+        """owt_f2p on device buffers (addresses as ints), allocating nothing."""
+        self._check(lib().owt_f2p_noalloc(self._h, ctypes.c_void_p(wave_dev), int(nf), ctypes.c_void_p(phys_dev)))
 
     def p2f(self, phys):
         phys = np.ascontiguousarray(phys, dtype=np.float64)
