@@ -80,8 +80,27 @@ int owt_get_sizes(const owt_handle *h, int *sizes7);
 
 /* init_fields (src/solver.cpp:329-426). mode 1: laminar profile plus the
  * reference's fixed (2, +-2) wave; mode 2: seeded divergence-free perturbation
- * of peak amplitude amp (seed, amp unused in mode 1). */
+ * of peak amplitude amp; mode 3: the laminar profile alone (seed, amp used in
+ * mode 2 only). */
 int owt_init_fields(owt_handle *h, int mode, uint64_t seed, double amp);
+
+/* The run keys of the case file the handle was created from (src/IO.cpp:14-25
+ * and the [owt] table): NT and save_interval of the driver loop
+ * (src/main.cpp:69-87), and the initial field: init_mode 1..3 as for
+ * owt_init_fields with seed and amp, or 4 to restart from the full-state
+ * checkpoint `restart` (owt_restore). The reference's initial_condition key
+ * (test-cases/Retau-180/case.toml:12) is read: 1 and 2 give mode 1, which is what the
+ * reference computes for either (its init_fields never reads the key), 3
+ * (renewal) gives mode 4. Handles made by owt_create_params report NT = 0,
+ * save_interval = 1, mode 1. */
+typedef struct {
+    long NT, save_interval;
+    int init_mode;
+    uint64_t seed;
+    double amp;
+    char restart[1024];
+} owt_run_params;
+int owt_get_run_params(const owt_handle *h, owt_run_params *out);
 /* injects u, v, w and runs the rest of init_fields (src/solver.cpp:379-425) */
 int owt_set_velocity(owt_handle *h, const double *u, const double *v, const double *w);
 

@@ -276,6 +276,19 @@ int owt_get_sizes(const owt_handle *h, int *s) {
     return OWT_OK;
 }
 
+int owt_get_run_params(const owt_handle *h, owt_run_params *out) {
+    if (!h || !out) return OWT_E_ARG;
+    const owt::Params &p = h->ch->params();
+    out->NT = p.NT;
+    out->save_interval = p.save_interval;
+    out->init_mode = p.init_mode;
+    out->seed = p.seed;
+    out->amp = p.amp;
+    std::memset(out->restart, 0, sizeof out->restart);
+    std::strncpy(out->restart, p.restart.c_str(), sizeof out->restart - 1);
+    return OWT_OK;
+}
+
 int owt_init_fields(owt_handle *h, int mode, uint64_t seed, double amp) {
     return guard(h, [&] { h->ch->init_fields(mode, seed, amp); });
 }

@@ -518,8 +518,11 @@ void Channel::init_fields(int mode, uint64_t seed, double amp) {
             }
         }
         laminar(u, true);
+    } else if (mode == 3) {
+        // the laminar profile alone (the README's initial_condition 1)
+        laminar(u, false);
     } else {
-        throw Error(2, "init mode must be 1 (reference) or 2 (seeded)");
+        throw Error(2, "init mode must be 1 (reference), 2 (seeded) or 3 (laminar)");
     }
     set_velocity(u.data(), v.data(), w.data());
 }

@@ -149,11 +149,33 @@ Params load_params(const std::string &path) {
         else if (it->second.raw == "batched") p.method = 1;
         else throw Error(1, "Error loading configuration file:owt.method must be \"plain\" or \"batched\"");
     }
+    // initial_condition (test-cases/*/case.toml:12: 1 laminar only, 2 laminar
+    // perturbation, 3 renewal) is in the reference's case files but never read
+    // there: its init_fields always builds the laminar profile plus the (2, +-2)
+    // wave (src/solver.cpp:329-426). To give the same run from the same file,
+    // 1 and 2 select that field; 3 restarts from the full-state checkpoint
+    // (owt.restart, default DATA/restart.ckpt, which owt_solver writes at the end
+    // of a run). [owt] init, when present, takes precedence.
+    it = kv.find("initial_condition");
+    if (it != kv.end()) {
+        const long ic = as_int(it->second, "initial_condition");
+        if (ic < 1 || ic > 3) throw Error(1, "Error loading configuration file:initial_condition must be 1, 2 or 3");
+        p.init_mode = ic == 3 ? 4 : 1;
+    }
     it = kv.find("owt.init");
     if (it != kv.end()) {
         if (it->second.raw == "reference") p.init_mode = 1;
         else if (it->second.raw == "seeded") p.init_mode = 2;
-        else throw Error(1, "Error loading configuration file:owt.init must be \"reference\" or \"seeded\"");
+        else if (it->second.raw == "laminar") p.init_mode = 3;
+        else if (it->second.raw == "restart") p.init_mode = 4;
+        else
+            throw Error(1, "Error loading configuration file:owt.init must be \"reference\", \"seeded\", \"laminar\" or "
+                           "\"restart\"");
+    }
+    it = kv.find("owt.restart");
+    if (it != kv.end()) {
+        if (!it->second.is_string) throw Error(1, "Error loading configuration file:owt.restart must be a string");
+        p.restart = it->second.raw;
     }
     it = kv.find("owt.seed");
     if (it != kv.end()) p.seed = (uint64_t)as_int(it->second, "owt.seed");

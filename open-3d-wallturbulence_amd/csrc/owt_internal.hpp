@@ -33,7 +33,11 @@ struct Params {
     int method = 1;     // 0 = plain (per-step LU, eager), 1 = batched (factor once, graph)
     uint64_t seed = 0;
     double amp = 0.15;
-    int init_mode = 1;  // 1 = reference laminar + wave, 2 = seeded recipe
+    // initial field of a run from a case file: 1 = the reference's init_fields
+    // (laminar + (2, +-2) wave), 2 = seeded recipe, 3 = laminar only, 4 = restart
+    // from the full-state checkpoint `restart`
+    int init_mode = 1;
+    std::string restart = "DATA/restart.ckpt";
 };
 
 Params load_params(const std::string &path);

@@ -1,9 +1,12 @@
 // owt_solver <case.toml>: the reference driver (src/main.cpp:7-93) on one GPU.
-// Same call order: setup, init_fields, then NT+1 iterations of
+// Same call order: setup, the initial field, then NT+1 iterations of
 // (save if id_step % save_interval == 0) + time step, with the reference's
-// per-step stdout lines. Snapshots are the reference's HDF5 files under ./DATA
-// (F-<id_step>.H5, u, v, w); the full state for a restart is written to
-// ./DATA/restart.ckpt at the end.
+// per-step stdout lines. NT, save_interval and the initial field come from the
+// case file as owt_create parsed it (owt_get_run_params): the reference's keys,
+// its initial_condition key (3 = renewal restarts from the full-state
+// checkpoint) and the [owt] table (init, seed, amp, restart). Snapshots are the
+// reference's HDF5 files under ./DATA (F-<id_step>.H5, u, v, w); the full state
+// for a restart is written to ./DATA/restart.ckpt at the end.
 #include <chrono>
 #include <cstdio>
 #include <filesystem>
@@ -25,31 +28,26 @@ int main(int argc, char **argv) {
     }
     int sz[7];
     owt_get_sizes(h, sz);
+    owt_run_params rp;
+    owt_get_run_params(h, &rp);
     std::printf("NX, NY, NZ = %d %d %d\n", sz[0], sz[1], sz[2]);
     std::printf("NXH, NZH, NX2, NZ2 = %d %d %d %d\n", sz[3], sz[4], sz[5], sz[6]);
+    std::printf("NT, save_interval = %ld %ld\n", rp.NT, rp.save_interval);
     if (!std::filesystem::exists("DATA")) std::filesystem::create_directory("DATA");
-    // the case file's [owt] init table selects the initial field; default reference
-    rc = owt_init_fields(h, 1, 0, 0.0);
+    if (rp.init_mode == 4) {
+        rc = owt_restore(h, rp.restart);
+        if (rc == 0) std::printf("restart from %s at step %ld\n", rp.restart, owt_steps_done(h));
+    } else {
+        rc = owt_init_fields(h, rp.init_mode, rp.seed, rp.amp);
+    }
     if (rc) {
         std::fprintf(stderr, "%s\n", owt_last_error(h));
         owt_destroy(h);
         return rc;
     }
-    // NT and save_interval are re-read from the case file
-    long NT = 0, save_interval = 1;
-    {
-        FILE *f = std::fopen(argv[1], "r");
-        char line[512];
-        while (f && std::fgets(line, sizeof line, f)) {
-            long v;
-            if (std::sscanf(line, " NT = %ld", &v) == 1) NT = v;
-            if (std::sscanf(line, " save_interval = %ld", &v) == 1) save_interval = v;
-        }
-        if (f) std::fclose(f);
-    }
-    for (long i = 0; i <= NT; ++i) {
+    for (long i = 0; i <= rp.NT; ++i) {
         const long id_step = owt_steps_done(h);
-        if (save_interval > 0 && id_step % save_interval == 0) {
+        if (rp.save_interval > 0 && id_step % rp.save_interval == 0) {
             // the reference prints and goes on when a snapshot cannot be written
             if (owt_save(h, "./DATA")) std::fprintf(stderr, "%s\n", owt_last_error(h));
         }

@@ -39,7 +39,7 @@ EXPORTS = [
     "owt_ensemble_init_fields", "owt_ensemble_step", "owt_ensemble_time_steps",
     "owt_tangent_init", "owt_tangent_handle", "owt_tangent_step", "owt_adjoint_run",
     "owt_autotune", "owt_get_tune", "owt_set_tune", "owt_save_fields", "owt_statistics",
-    "owt_f2p_noalloc",
+    "owt_f2p_noalloc", "owt_get_run_params",
 ]
 
 
@@ -52,6 +52,11 @@ class OwtError(RuntimeError):
 class _Opts(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("precision", ctypes.c_int),
                 ("method", ctypes.c_int), ("verbose", ctypes.c_int)]
+
+
+class _RunParams(ctypes.Structure):
+    _fields_ = [("NT", ctypes.c_long), ("save_interval", ctypes.c_long), ("init_mode", ctypes.c_int),
+                ("seed", ctypes.c_uint64), ("amp", ctypes.c_double), ("restart", ctypes.c_char * 1024)]
 
 
 class _Params(ctypes.Structure):
@@ -98,6 +103,7 @@ def lib():
         L.owt_f2p.argtypes = [P, P, I, P]
         L.owt_p2f.argtypes = [P, P, I, P]
         L.owt_f2p_noalloc.argtypes = [P, P, I, P]
+        L.owt_get_run_params.argtypes = [P, ctypes.POINTER(_RunParams)]
         L.owt_lamb.argtypes = [P, P, P]
         L.owt_get_matrix.argtypes = [P, I, P]
         L.owt_time_steps.argtypes = [P, I, P]
@@ -183,6 +189,14 @@ class Channel:
     @property
     def gridpoints(self):
         return self.NX * self.NY * self.NZ
+
+    def run_params(self):
+        """NT, save_interval and the initial-field choice of the case file (init_mode
+        1 reference field, 2 seeded, 3 laminar, 4 restart from `restart`)."""
+        r = _RunParams()
+        self._check(lib().owt_get_run_params(self._h, ctypes.byref(r)))
+        return {"NT": r.NT, "save_interval": r.save_interval, "init_mode": r.init_mode, "seed": r.seed,
+                "amp": r.amp, "restart": r.restart.decode()}
 
     def init_fields(self, mode=1, seed=0, amp=0.15):
         self._check(lib().owt_init_fields(self._h, int(mode), int(seed), float(amp)))

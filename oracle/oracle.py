@@ -115,6 +115,7 @@ class Oracle:
     # test instruments (owt_oracle.cpp, Oracle::perturb and Oracle::mutate)
     NOISE_LAMB, NOISE_SOLVE = 1, 2
     MUT_HG_SIGN, MUT_FRICTION_SIGN, MUT_U_OMY_SIGN, MUT_HG_KX_SIGN = 1, 2, 4, 8
+    MUT_LAMB_HISTORY, MUT_FRICTION_HISTORY = 16, 32
 
     def set_noise(self, seed, rel, where):
         """Multiply the fields named by `where` (NOISE_LAMB | NOISE_SOLVE) by (1 + rel g),

@@ -382,7 +382,9 @@ struct Oracle {
     //  mutate: planted mistakes that the pins in tests/ must detect: bit 1 the
     //    sign of Hg, bit 2 the sign of the friction forcing of the mean flow,
     //    bit 4 the sign of the omega_y term of the horizontal recovery of u,
-    //    bit 8 the sign of the i kx Lz term of Hg.
+    //    bit 8 the sign of the i kx Lz term of Hg, bit 16 the mean Lamb history
+    //    not rotated (levels 1 and 2 keep the initial values), bit 32 the friction
+    //    history not rotated.
     //  fft_variant: 1 transforms with the second factorization of fft_rec.
     int fft_variant = 0;
     uint64_t noise_seed = 0;
@@ -714,9 +716,11 @@ struct Oracle {
             p2f(lx.data(), lmbx.data()); p2f(ly.data(), lmby.data()); p2f(lz.data(), lmbz.data());
         }
         if (noise_where & 1) { perturb(lmbx, 1); perturb(lmby, 2); perturb(lmbz, 3); }
-        lmbx_mean[2] = lmbx_mean[1]; lmbx_mean[1] = lmbx_mean[0];
+        if (!(mutate & 16)) {
+            lmbx_mean[2] = lmbx_mean[1]; lmbx_mean[1] = lmbx_mean[0];
+            lmbz_mean[2] = lmbz_mean[1]; lmbz_mean[1] = lmbz_mean[0];
+        }
         lmbx_mean[0].assign(lmbx.begin(), lmbx.begin() + NY);
-        lmbz_mean[2] = lmbz_mean[1]; lmbz_mean[1] = lmbz_mean[0];
         lmbz_mean[0].assign(lmbz.begin(), lmbz.begin() + NY);
     }
 
@@ -731,8 +735,12 @@ struct Oracle {
         last_bottom = 2 * dudy[0].real() * nu / Ly;
         last_upper = 2 * dudy[NY - 1].real() * nu / Ly;
         diag.push_back(fx_fric); diag.push_back(last_bottom); diag.push_back(last_upper);
-        fx_h[2] = fx_h[1]; fx_h[1] = fx_h[0]; fx_h[0] = fx_fric;
-        fz_h[2] = fz_h[1]; fz_h[1] = fz_h[0]; fz_h[0] = fz_fric;
+        if (!(mutate & 32)) {
+            fx_h[2] = fx_h[1]; fx_h[1] = fx_h[0];
+            fz_h[2] = fz_h[1]; fz_h[1] = fz_h[0];
+        }
+        fx_h[0] = fx_fric;
+        fz_h[0] = fz_fric;
     }
 
     // src/solver.cpp:428-514

@@ -1,0 +1,76 @@
+"""The headline workload at full size: Re_tau 550, 384x193x384 (576x576
+dealiased planes), tanh stretch 2.0, nu 1e-4, dt 1e-3, seeded 5 % perturbation.
+
+The oracle cannot run this size in a test, so the checks are the ones that do
+not depend on size (its FP64 parity at this wall-normal grid is
+test_gpu_floors.py's re550_wall_32):
+  - mixed precision (FP32 transforms) against FP64 over 100 steps: the mean wall
+    shear (fx_fric, the friction force) within 1e-6 relative at every step, and
+    the per-field relative L2 errors of u, v, w, p;
+  - the FP64 field after 100 steps: discrete divergence i kx u + DY1 v + i kz w
+    to roundoff, and the kx = 0 plane's Hermitian symmetry. The +kz and -kz
+    pencils of that plane are solved separately, from right-hand sides whose
+    transforms are Hermitian to rounding only, and the v operator at this grid
+    amplifies rounding to about 1e-9 of the field in 10 steps (its floor in
+    tests/golden/floors.json, re550_wall_32): the defect is bounded by 1e-8 of
+    the field's largest value, where a sign or index error would be of order one.
+"""
+import numpy as np
+import pytest
+
+import pins
+
+pytestmark = pytest.mark.gpu
+
+RE550 = dict(nu=1e-4, dt=1e-3, dkx=1.0, dkz=2.0, alpha_mesh=2.0, NX=384, NY=193, NZ=384)
+STEPS = 100
+# per-field relative L2 bars of the mixed step against the FP64 step after 100 steps
+BARS = {"u": 1e-8, "v": 1e-6, "w": 1e-6, "p": 1e-6}
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def runs(owt_mod):
+    out = {}
+    for prec in (0, 1):
+        print(f"[gpu] re550 384x193x384 precision {prec}, {STEPS} steps", flush=True)
+        with owt_mod.Channel(params=dict(RE550, precision=prec, method=1)) as ch:
+            ch.init_fields(2, seed=0, amp=0.075)
+            ch.step(STEPS)
+            out[prec] = ({n: ch.field(n) for n in ("u", "v", "w", "p")}, ch.diag()[:, 0], ch.matrix("DY1"))
+    return out
+
+
+def test_mixed_wall_shear_and_fields_against_fp64(runs):
+    (f64, fx64, _), (fmx, fxmx, _) = runs[0], runs[1]
+    assert len(fx64) == len(fxmx) == STEPS + 1
+    ws = np.abs(fxmx - fx64).max() / np.abs(fx64).max()
+    errs = {n: rel(fmx[n], f64[n]) for n in BARS}
+    print(f"[headline] wall shear max rel {ws:.3e}; " + " ".join(f"{n}={e:.3e}" for n, e in errs.items()), flush=True)
+    assert ws <= 1e-6
+    for n, e in errs.items():
+        assert e <= BARS[n], (n, e)
+
+
+def test_full_size_divergence_and_hermitian_plane(runs):
+    f, _, DY1 = runs[0]
+    kx, kz = pins.wavenumbers(RE550["NX"], RE550["NZ"], RE550["dkx"], RE550["dkz"])
+    u, v, w = f["u"], f["v"], f["w"]
+    NZ, NXH = u.shape[:2]
+    rng = np.random.default_rng(0)
+    worst = 0.0
+    for _ in range(400):
+        k, i = int(rng.integers(NZ)), int(rng.integers(NXH))
+        if kx[i] ** 2 + kz[k] ** 2 < 1e-8:
+            continue
+        d1v = DY1 @ v[k, i]
+        div = 1j * kx[i] * u[k, i] + d1v + 1j * kz[k] * w[k, i]
+        if np.abs(d1v).max() > 0:
+            worst = max(worst, np.abs(div).max() / np.abs(d1v).max())
+    herm = max(np.abs(f[n][1:NZ // 2, 0] - np.conj(f[n][NZ - 1:NZ // 2:-1, 0])).max() / np.abs(f[n]).max()
+               for n in ("u", "v", "w"))
+    print(f"[headline] divergence {worst:.2e}, Hermitian defect {herm:.2e}", flush=True)
+    assert worst < 1e-11 and herm < 1e-8

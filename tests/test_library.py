@@ -87,3 +87,17 @@ def test_reference_case_files_parse(owt_mod, tmp_path):
     assert rc in (0, 3), msg
     if rc == 3:
         assert "HIP" in msg or "device" in msg
+
+
+@pytest.mark.parametrize("extra,code", [("initial_condition = 4", 1), ("initial_condition = 0", 1),
+                                        ("[owt]\ninit = \"bogus\"", 1), ("[owt]\nrestart = 5", 1),
+                                        ("initial_condition = 3", None), ("[owt]\ninit = \"laminar\"", None)])
+def test_initial_condition_keys(owt_mod, tmp_path, extra, code):
+    """initial_condition (test-cases/*/case.toml:12) must be 1, 2 or 3; [owt] init one
+    of the four names; valid files parse (only device creation can fail here)."""
+    text = "\n".join(l for l in REF_KEYS.splitlines() if not l.startswith("initial_condition"))
+    rc, msg = create_rc(owt_mod, text + "\n" + extra + "\n", tmp_path)
+    if code is None:
+        assert rc in (0, 3), msg
+    else:
+        assert rc == code, msg

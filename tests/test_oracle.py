@@ -509,3 +509,122 @@ def test_orr_sommerfeld_eigenmode_grows_at_the_discrete_rate(oracle_mod):
     lm = os_.growth(lambda: o.step(1), lambda: o.field("v")[0, 1], phi, dt, 100, 600)
     assert abs(lm.real - lam.real) < 2e-5 * abs(lam.real), (lm, lam)
     assert abs(lm.imag - lam.imag) < 1e-6 * abs(lam.imag), (lm, lam)
+
+
+# ---------------------------------------------------------------------------
+# Pins of the paths the pins above leave open (tests/pins.py): the omega_y /
+# Squire path, Hg, the horizontal recovery's omega_y terms, and the mean-flow
+# forcing (friction lag, mean-Lamb history). Each is also run with a planted
+# mistake in the oracle (Oracle.set_mutation), which it must detect.
+# ---------------------------------------------------------------------------
+import pins  # noqa: E402
+
+
+def test_recovery_identities_after_stepping(oracle_mod):
+    """After three nonlinear steps every k^2 > 0 pencil satisfies the four exact
+    identities of the horizontal recovery to roundoff (and the mean pencil's
+    vorticity is DY1 W, -DY1 U); a flipped omega_y term in u breaks them."""
+    for mut, expect_ok in ((0, True), (oracle_mod.Oracle.MUT_U_OMY_SIGN, False)):
+        o = make(oracle_mod, 16, 33, 16)
+        o.set_mutation(mut)
+        o.init(2, seed=3, amp=0.3)
+        o.step(3)
+        kx, kz = o.vector("kx"), o.vector("kz")
+        res = pins.recovery_identities(o, kx, kz, o.matrix("DY1"), o.matrix("DY2"))
+        print(f"[pin] recovery identities, mutation {mut}: {res}")
+        assert (res.max() < 1e-12) == expect_ok, res
+
+
+def test_hg_is_the_linearized_squire_forcing(oracle_mod):
+    """Hg of laminar flow plus an oblique perturbation (v, omega_y at kx = 0.5,
+    kz = 2), linearized by the exact +-eps difference of the bilinear step terms:
+    Hg' = -i kz (DY1 U) v - i kx U omega_y (Lx' = -U' v, Lz' = U omega_y through the
+    dealiased transforms of a mean profile times one mode: exact to roundoff).
+    A flipped Hg (src/solver.cpp:257), or a flipped kx term in it, fails."""
+    for mut, expect_ok in ((0, True), (oracle_mod.Oracle.MUT_HG_SIGN, False), (oracle_mod.Oracle.MUT_HG_KX_SIGN, False)):
+        o = make(oracle_mod, 8, 33, 8)
+        o.set_mutation(mut)
+        y, DY1 = o.vector("y"), o.matrix("DY1")
+        kx, kz = o.vector("kx"), o.vector("kz")
+        U = 1.5 * (1 - y ** 2)
+        i, k = 1, 2
+        a, b = kx[i], kz[k]
+        k2 = a * a + b * b
+        vp = (1 - y ** 2) ** 2 * (1 + 0.3j * y + 0.2 * y ** 2)
+        gp = (1 - y ** 2) * (0.5 - 0.7j * y + 0.1 * y ** 3)
+        d1v = DY1 @ vp
+        up = (1j * a / k2) * d1v - (1j * b / k2) * gp
+        wp = (1j * a / k2) * gp + (1j * b / k2) * d1v
+        eps = 1e-3
+        res = []
+        for sgn in (1, -1):
+            u = np.zeros(o.wave_shape, dtype=complex)
+            v = np.zeros_like(u)
+            w = np.zeros_like(u)
+            u[0, 0] = U
+            u[k, i], v[k, i], w[k, i] = sgn * eps * up, sgn * eps * vp, sgn * eps * wp
+            o.set_velocity(u, v, w)
+            res.append((o.field("Hg")[k, i], o.field("omy")[k, i]))
+        hg = (res[0][0] - res[1][0]) / (2 * eps)
+        omy = (res[0][1] - res[1][1]) / (2 * eps)
+        ref = -1j * b * (DY1 @ U) * vp - 1j * a * U * omy
+        err = np.abs(hg - ref).max() / np.abs(ref).max()
+        print(f"[pin] Hg linearization, mutation {mut}: {err:.2e}")
+        assert (err < 1e-12) == expect_ok, err
+
+
+def test_squire_mode_follows_the_bdf3_recursion(oracle_mod):
+    """A kx = 0 omega_y mode without mean flow (Hg = 0): the implicit omega_y solve
+    and the scheme's coefficients give the exact BDF3 recursion at the Dirichlet
+    eigenvalue of DY2 (src/solver.cpp:548-556, 578, 603)."""
+    N, dt, nu = 65, 0.01, 1 / 2800
+    o = make(oracle_mod, 4, N, 8, dt=dt)
+    dev = pins.squire_sequence(o, o.wave_shape, o.matrix("DY2"), o.vector("kz")[1], nu, dt, 30)
+    print(f"[pin] Squire BDF3 recursion: {dev:.2e}")
+    assert dev < 1e-12
+
+
+@pytest.mark.parametrize("mut", [0, 2, 16, 32])
+def test_mean_flow_follows_its_recursion(oracle_mod, mut):
+    """The mean flow and the spanwise mean driven by a streamwise vortex's mean Lamb
+    vector, step by step against the implicit recursion with the friction and
+    mean-Lamb histories (src/solver.cpp:428-514, 306-327, 293-303). The planted
+    mistakes (friction sign, mean-Lamb history or friction history not rotated)
+    each fail."""
+    o = make(oracle_mod, 4, 33, 8, nu=0.02, dt=0.01, dkz=2.0)
+    o.set_mutation(mut)
+    y = o.vector("y")
+    dev, fdev = pins.mean_flow_recursion(o, o.wave_shape, y, o.matrix("DY1"), o.matrix("DY2"), 2.0, 0.02, 0.01, 12)
+    print(f"[pin] mean-flow recursion, mutation {mut}: {dev:.2e}, friction {fdev:.2e}")
+    assert fdev < 1e-12
+    assert (dev < 1e-12) == (mut == 0), dev
+
+
+@pytest.mark.parametrize("mut", [0, 2])
+def test_friction_force_holds_the_flow_rate(oracle_mod, mut):
+    """Bulk flow rate over 40 nonlinear steps: drifts by ~1e-8 with the friction
+    feedback (the quadrature and DY2's wall rows), by ~1e-4 with its sign flipped."""
+    o = make(oracle_mod, 16, 33, 16)
+    o.set_mutation(mut)
+    o.init(2, seed=3, amp=0.3)
+    drift = pins.flow_rate_drift(o, o.vector("y"), 40)
+    print(f"[pin] flow-rate drift, mutation {mut}: {drift:.2e}")
+    assert (drift < 1e-7) == (mut == 0), drift
+
+
+@pytest.mark.parametrize("mut", [0, 1, 8])
+def test_oblique_orr_sommerfeld_squire_mode(oracle_mod, mut):
+    """An oblique mode (kx = 1, kz = 0.5, Re 1e4) of the coupled Orr-Sommerfeld /
+    Squire operator built on the same DY matrices: the stepped mode grows and turns
+    at the discrete eigenvalue to the scheme's O(dt^3) and keeps the eigenvector's
+    omega_y / v ratio; with Hg's sign or its kx term flipped the ratio is off by
+    O(1)."""
+    nu, dt = 1.5e-4, 0.01
+    o = make(oracle_mod, 4, 65, 4, nu=nu, dt=dt, dkx=1.0, dkz=0.5)
+    o.set_mutation(mut)
+    lm, lam, ratio = pins.oblique_mode(o, o.wave_shape, o.vector("y"), o.matrix("DY1"), o.matrix("DY2"),
+                                       o.matrix("DY4"), 1.0, 0.5, nu, dt, 100, 600)
+    print(f"[pin] oblique mode, mutation {mut}: measured {lm:.10e} discrete {lam:.10e} ratio {ratio:.8f}")
+    ok = abs(ratio - 1) < 1e-5 and abs(lm.real - lam.real) < 2e-5 * abs(lam.real) and \
+        abs(lm.imag - lam.imag) < 2e-6 * abs(lam.imag)
+    assert ok == (mut == 0), (lm, lam, ratio)
