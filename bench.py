@@ -49,8 +49,10 @@ def stage_bytes(W, I, es):
     field; es: bytes per complex element in the transform phase (16 FP64, 8 FP32)."""
     c = 16
     return {
-        # v: 3 levels of v, Hv and DY2 v; omega_y: 3 levels of omega_y and Hg
-        "solve": (15 * W + 2 * W) * c,
+        # the v solves: 3 levels of v, Hv and DY2 v in, v out
+        "solve": (9 * W + W) * c,
+        # the omega_y solves: 3 levels of omega_y and Hg in, omega_y out
+        "helmholtz": (6 * W + W) * c,
         # reads v, omega_y; writes u, w, omega_x, omega_z and DY2 v
         "horizontal": (2 * W + 5 * W) * c,
         "mean": 0,
@@ -63,8 +65,9 @@ def stage_bytes(W, I, es):
 
 def operator_bytes(N, nfac):
     """Bytes of stored LU factors one solve launch streams: nfac factor indices
-    x 2 operators x N^2 doubles (design overhead, reported beside the roofline)."""
-    return 2 * nfac * N * N * 8
+    x one operator (Av) x N^2 doubles (design overhead, reported beside the
+    roofline; the omega_y solves read two shared (N-2)^2 matrices)."""
+    return nfac * N * N * 8
 
 
 def cpu_baseline(threads, budget_s=20.0):

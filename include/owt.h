@@ -243,7 +243,8 @@ int owt_time_steps(owt_handle *h, int nsteps, double *ms);
 /* timing of one stage kernel launched reps times on the handle's stream:
  * 0 right-hand sides + implicit solves, 1 horizontal, 2 mean flow, 3 z inverse,
  * 4 x phase, 5 z forward, 6 nonlinear, 7 LU factorization (the plain method
- * refactors every step). Returns the mean milliseconds per launch. */
+ * refactors every step), 8 the batched method's omega_y solves (nothing in the
+ * plain method). Returns the mean milliseconds per launch. */
 int owt_time_kernel(owt_handle *h, int kernel, int reps, double *ms);
 
 #ifdef __cplusplus

@@ -411,7 +411,7 @@ int owt_time_steps(owt_handle *h, int nsteps, double *ms) {
 
 int owt_time_kernel(owt_handle *h, int kernel, int reps, double *ms) {
     if (!ms || reps < 1) return OWT_E_ARG;
-    if (kernel < 0 || kernel > 7) return OWT_E_ARG;
+    if (kernel < 0 || kernel > 8) return OWT_E_ARG;
     return guard(h, [&] {
         owt::Channel &c = *h->ch;
         hipStream_t s = c.stream();

@@ -124,6 +124,10 @@ void Channel::alloc() {
     c.DY2 = (const double *)dmalloc(N * N * sizeof(double));
     c.DY4 = (const double *)dmalloc(N * N * sizeof(double));
     c.qw = (const double *)dmalloc(N * sizeof(double));
+    c.hQ = (const double *)dmalloc((N - 2) * (N - 2) * sizeof(double));
+    c.hQi = (const double *)dmalloc((N - 2) * (N - 2) * sizeof(double));
+    c.hlre = (const double *)dmalloc((N - 2) * sizeof(double));
+    c.hlim = (const double *)dmalloc((N - 2) * sizeof(double));
     c.kx = (const double *)dmalloc(c.NXH * sizeof(double));
     c.kz = (const double *)dmalloc(c.NZ * sizeof(double));
     c.means = (double2 *)dmalloc(12 * N * sizeof(double2));
@@ -169,6 +173,13 @@ void Channel::upload_operators() {
     }
     OWT_HIP(hipMemcpyAsync((void *)c_.kx, kx.data(), kx.size() * sizeof(double), hipMemcpyHostToDevice, s_));
     OWT_HIP(hipMemcpyAsync((void *)c_.kz, kz.data(), kz.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+    // eigendecomposition of DY2's interior block for the omega_y solves
+    const HelmholtzBasis hb = helmholtz_basis(op_.DY2, (int)N);
+    const size_t M = hb.M;
+    OWT_HIP(hipMemcpyAsync((void *)c_.hQ, hb.Q.data(), M * M * sizeof(double), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync((void *)c_.hQi, hb.Qinv.data(), M * M * sizeof(double), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync((void *)c_.hlre, hb.lre.data(), M * sizeof(double), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync((void *)c_.hlim, hb.lim.data(), M * sizeof(double), hipMemcpyHostToDevice, s_));
     auto tx = twiddles(c_.NX2), tz = twiddles(c_.NZ2);
     std::vector<float2> txf(tx.size()), tzf(tz.size());
     for (size_t m = 0; m < tx.size(); ++m) txf[m] = make_float2((float)tx[m].x, (float)tx[m].y);
@@ -183,7 +194,10 @@ void Channel::upload_operators() {
 // The operators Av(k^2), Ag(k^2) do not change in time and depend on k^2
 // only: factor each distinct value once (the reference refactors every pencil
 // every step). Systems are grouped by the bit pattern of k^2 as the device
-// computes it, kx*kx + kz*kz with each product rounded.
+// computes it, kx*kx + kz*kz with each product rounded. The step solves the
+// omega_y systems through the eigendecomposition of DY2 (k_helmholtz_g), so only
+// Av is stored per k^2 (bops_ = 1), plus Ag(0) for the mean flow; an adjoint
+// channel, whose transposed solves use LU factors of both, keeps both (bops_ = 2).
 void Channel::factor_operators() {
     if (factored_) return;
     const size_t N = c_.NY;
@@ -211,14 +225,30 @@ void Channel::factor_operators() {
         }
         nfac_ = (int)frep.size();
         njobs_ = (int)jobs.size();
-        Fb_ = (double *)dmalloc(2 * (size_t)nfac_ * N * N * sizeof(double));
-        Pb_ = (int *)dmalloc(2 * (size_t)nfac_ * N * sizeof(int));
+        Fb_ = (double *)dmalloc((size_t)bops_ * nfac_ * N * N * sizeof(double));
+        Pb_ = (int *)dmalloc((size_t)bops_ * nfac_ * N * sizeof(int));
         jobs_ = (int4 *)dmalloc(njobs_ * sizeof(int4));
         frep_ = (int *)dmalloc(nfac_ * sizeof(int));
         OWT_HIP(hipMemcpy(jobs_, jobs.data(), njobs_ * sizeof(int4), hipMemcpyHostToDevice));
         OWT_HIP(hipMemcpy(frep_, frep.data(), nfac_ * sizeof(int), hipMemcpyHostToDevice));
+        if (bops_ == 1) {
+            // the mean flow's Ag(0) (system 0: kx = kz = 0), the one omega_y operator
+            // kept as an LU factor
+            Fg0_ = (double *)dmalloc(N * N * sizeof(double));
+            Pg0_ = (int *)dmalloc(N * sizeof(int));
+            sys0_ = (int *)dmalloc(sizeof(int));
+            OWT_HIP(hipMemset(sys0_, 0, sizeof(int)));
+        }
     }
-    factor_set(Fb_, Pb_, nfac_, frep_, 0, 2);
+    factor_set(Fb_, Pb_, nfac_, frep_, 0, bops_);
+    if (bops_ == 1) {
+        factor_set(Fg0_, Pg0_, 1, sys0_, 1, 1);
+        Fmean_ = Fg0_;
+        Pmean_ = Pg0_;
+    } else {
+        Fmean_ = Fb_ + ((size_t)fmap_host_[0] * 2 + 1) * N * N;
+        Pmean_ = Pb_ + ((size_t)fmap_host_[0] * 2 + 1) * N;
+    }
     OWT_HIP(hipGetLastError());
     OWT_HIP(hipStreamSynchronize(s_));
     check_errors();
@@ -252,13 +282,30 @@ void Channel::pressure(double *out) {
 void Channel::get_factor(int sys, int op, double *lu, int *perm) {
     if (sys < 0 || sys >= c_.nsys || op < 0 || op > 1) throw Error(2, "factor index out of range");
     factor_operators();
-    const size_t N = c_.NY, f = (size_t)fmap_host_[sys] * 2 + op;
+    const size_t N = c_.NY;
+    const double *F = Fb_;
+    const int *P = Pb_;
+    size_t f = (size_t)fmap_host_[sys] * bops_ + op;
+    if (op == 1 && bops_ == 1) {
+        // the step keeps no Ag factors: factor them on request
+        if (!Fag_) {
+            Fag_ = (double *)dmalloc((size_t)nfac_ * N * N * sizeof(double));
+            Pag_ = (int *)dmalloc((size_t)nfac_ * N * sizeof(int));
+            factor_set(Fag_, Pag_, nfac_, frep_, 1, 1);
+            OWT_HIP(hipGetLastError());
+            OWT_HIP(hipStreamSynchronize(s_));
+            check_errors();
+        }
+        F = Fag_;
+        P = Pag_;
+        f = (size_t)fmap_host_[sys];
+    }
     // the device keeps the packed layout of lu_at; hand out dgetrf's column-major one
     std::vector<double> packed(N * N);
-    OWT_HIP(hipMemcpy(packed.data(), Fb_ + f * N * N, N * N * sizeof(double), hipMemcpyDeviceToHost));
+    OWT_HIP(hipMemcpy(packed.data(), F + f * N * N, N * N * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t col = 0; col < N; ++col)
         for (size_t row = 0; row < N; ++row) lu[col * N + row] = packed[lu_at((int)N, (int)row, (int)col)];
-    OWT_HIP(hipMemcpy(perm, Pb_ + f * N, N * sizeof(int), hipMemcpyDeviceToHost));
+    OWT_HIP(hipMemcpy(perm, P + f * N, N * sizeof(int), hipMemcpyDeviceToHost));
 }
 
 // LU factors of nfac operators (launch_factor). Above 256 rows the factorization
@@ -592,9 +639,11 @@ void Channel::enqueue_step_body(int ring) {
     const double *F = plain ? Fp_ : Fb_;
     const int *P = plain ? Pp_ : Pb_;
     if (plain) factor_set(Fp_, Pp_, c_.npencil, nullptr, 0, 2);
-    launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, s_);
+    launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, plain ? 2 : bops_, s_);
+    if (!plain && bops_ == 1) launch_helmholtz_g(c_, sp, s_);
     launch_horizontal(c_, sp, plain, s_);
-    launch_mean(c_, sp, F, P, plain ? 0 : fmap_host_[0], s_);
+    // plain: pencil 0's own factors hold Ag(0) as its second operator
+    launch_mean(c_, sp, plain ? Fp_ + (size_t)c_.NY * c_.NY : Fmean_, plain ? Pp_ + c_.NY : Pmean_, s_);
     // the six transformed fields: u, v, w, omega_x, omega_y, omega_z of the new level
     const size_t W = c_.W;
     const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
@@ -643,18 +692,23 @@ void Channel::enqueue_kernel(int id) {
     const size_t W = c_.W;
     const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
     switch (id) {
-        case 0: launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, s_); break;
+        case 0: launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, plain ? 2 : bops_, s_); break;
         case 1: launch_horizontal(c_, sp, plain, s_); break;
-        case 2: launch_mean(c_, sp, F, P, plain ? 0 : fmap_host_[0], s_); break;
+        case 2:
+            launch_mean(c_, sp, plain ? Fp_ + (size_t)c_.NY * c_.NY : Fmean_, plain ? Pp_ + c_.NY : Pmean_, s_);
+            break;
         case 3: launch_z_inverse(c_, six, 6, c_.zbuf, s_); break;
         case 4: launch_x_lamb(c_, s_); break;
         case 5: launch_z_forward(c_, s_); break;
         case 6: launch_nonlinear(c_, sp, plain, s_); break;
         case 7:
             if (plain) factor_set(Fp_, Pp_, c_.npencil, nullptr, 0, 2);
-            else factor_set(Fb_, Pb_, nfac_, frep_, 0, 2);
+            else factor_set(Fb_, Pb_, nfac_, frep_, 0, bops_);
             break;
-        default: throw Error(2, "kernel id must be 0..7");
+        case 8:
+            if (!plain && bops_ == 1) launch_helmholtz_g(c_, sp, s_);
+            break;
+        default: throw Error(2, "kernel id must be 0..8");
     }
     OWT_HIP(hipGetLastError());
 }
@@ -1141,6 +1195,7 @@ double Channel::adjoint_run(int K, int ckpt, int first, int last, double *gu, do
         q.precision = 0;
         q.method = 1;
         adjoint_ = std::make_unique<Channel>(q, device_, false, s_);
+        adjoint_->bops_ = 2;  // the transposed solves use LU factors of Av and Ag
     }
     Channel &A = *adjoint_;
     A.adjoint_prepare();

@@ -61,7 +61,7 @@ class Channel {
     void enqueue_step();
     // one launch of a single stage of the step at the current ring phase:
     // 0 solve, 1 horizontal, 2 mean, 3 z inverse, 4 x phase, 5 z forward,
-    // 6 nonlinear, 7 factorization
+    // 6 nonlinear, 7 factorization, 8 omega_y solves by the eigendecomposition
     void enqueue_kernel(int id);
     // Launch-shape autotuner: times every candidate of every knob (TuneKnob) on
     // this channel's data, keeps the fastest, and recaptures the step graphs.
@@ -152,6 +152,17 @@ class Channel {
     double *Fq_ = nullptr;
     int *Pq_ = nullptr;
     bool factored_ = false;
+    // operators per factor index of the batched set: 1 (Av; omega_y by the
+    // eigendecomposition of DY2) or 2 (Av, Ag: adjoint channels)
+    int bops_ = 1;
+    // Ag(0) for the mean flow when bops_ == 1, and the factor the mean-flow kernel uses
+    double *Fg0_ = nullptr;
+    int *Pg0_ = nullptr, *sys0_ = nullptr;
+    const double *Fmean_ = nullptr;
+    const int *Pmean_ = nullptr;
+    // all Ag factors, made only when requested (get_factor) with bops_ == 1
+    double *Fag_ = nullptr;
+    int *Pag_ = nullptr;
     // scratch of the panel factorization (NY > 256): fchunk_ matrices x 2 operators
     double *fscratch_ = nullptr;
     int fchunk_ = 0;

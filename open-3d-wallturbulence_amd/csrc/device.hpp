@@ -82,6 +82,11 @@ struct Chan {
     const float2 *twxf, *twzf; // same, fp32
     FFTPlan px, pz;
     int tune[T_COUNT];
+    // omega_y solves of the batched method: D = DY2[1..N-2, 1..N-2] = Q B Q^-1
+    // (M = N - 2, column-major M x M), per row the eigenvalue's real part hlre and
+    // imaginary part hlim (0 for a real one; rows 2m, 2m+1 of a pair hold a, b of
+    // a +- i b and form the block [[a, b], [-b, a]])
+    const double *hQ, *hQi, *hlre, *hlim;
 };
 
 // History slots of one step: level 0 is the current field, levels 1 and 2 the
@@ -132,14 +137,17 @@ void launch_pressure_rhs(const Chan &c, const double2 *dv, double2 *R, hipStream
 void launch_pressure_solve(const Chan &c, const double *F, const int *P, const int4 *jobs, int njobs, const double2 *R,
                            double2 *out, hipStream_t s);
 // right-hand sides + solves; plain: njobs = pencils, each with its own factor;
-// batched: job j = {factor, system, system or -1}
+// batched: job j = {factor, system, system or -1}. nops: operators per factor
+// index in F (2: Av and Ag, both solved; 1: Av only, omega_y by launch_helmholtz_g)
 void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int4 *jobs, int njobs, bool plain,
-                  hipStream_t s);
+                  int nops, hipStream_t s);
+// omega_y of the new level by the eigendecomposition of DY2 (batched method)
+void launch_helmholtz_g(const Chan &c, const StepPtrs &sp, hipStream_t s);
 // exact: the plain method's kernels in the reference's arithmetic (column-order
 // mat-vecs, separate roundings); otherwise the matrix-core kernels
 void launch_horizontal(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s);
-// mean flow, friction force and the k = 0 pencil (factor mf holds Ag(0))
-void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *P, int mf, hipStream_t s);
+// mean flow, friction force and the k = 0 pencil (Fg, Pg: the LU factor of Ag(0))
+void launch_mean(const Chan &c, const StepPtrs &sp, const double *Fg, const int *Pg, hipStream_t s);
 void launch_nonlinear(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s);
 // initialisation helpers (src/solver.cpp:379-425)
 void launch_vor_from_vel(const Chan &c, hipStream_t s);

@@ -450,7 +450,8 @@ __device__ __forceinline__ void wave_sync() {
 
 // ---------------------------------------------------------------------------
 // Right-hand sides (src/solver.cpp:568-574, 593-599) and dgetrs.
-// Four waves per block: wave w takes job 2*block + w/2 and operator w%2.
+// NOPS waves per job (2: Av and Ag; 1: Av only, the batched method solving
+// omega_y by k_helmholtz_g), JPB jobs per block.
 // A batched job applies one factor to up to two +-kz systems that share it
 // (jobs[j] = {factor, system, system or -1}): up to four pencils, eight real
 // right-hand sides, one pass over the factor. A plain job is one pencil with
@@ -465,8 +466,9 @@ __device__ __forceinline__ void wave_sync() {
 // that term from the stored DY2 v of each level (computed by the horizontal
 // kernel when the level was new) and substitutes with fused multiply-adds.
 // ---------------------------------------------------------------------------
-template <int NQ, bool PLAIN, int JPB>
-__global__ __launch_bounds__(128 * JPB) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P, const int4 *jobs, int njobs) {
+template <int NQ, bool PLAIN, int JPB, int NOPS>
+__global__ __launch_bounds__(64 * NOPS * JPB) void k_solve(Chan c, StepPtrs sp, const double *F, const int *P, const int4 *jobs,
+                                                          int njobs) {
     constexpr int NP = PLAIN ? 1 : 4;  // pencils per job
     constexpr int NR = 2 * NP;         // real right-hand sides
     // plain: sum alpha v of the v-wave's pencil in natural row order
@@ -482,7 +484,7 @@ __global__ __launch_bounds__(128 * JPB) void k_solve(Chan c, StepPtrs sp, const 
         blk = (blockIdx.x % 8) * per + blockIdx.x / 8;
         if (blk >= nb) return;
     }
-    const int job = blk * JPB + (w >> 1), op = w & 1;
+    const int job = blk * JPB + w / NOPS, op = w % NOPS;
     const int N = c.NY;
     if (job >= njobs) return;
     int fac, pc[NP], kz, ix;
@@ -508,14 +510,14 @@ __global__ __launch_bounds__(128 * JPB) void k_solve(Chan c, StepPtrs sp, const 
     const double dt = c.dt;
     const double k2 = k2_of(c, kz, ix);
     const double k2dt = __ddiv_rn(k2, dt);
-    const double *Fo = F + ((size_t)fac * 2 + op) * N * N;
-    const int *Po = P + ((size_t)fac * 2 + op) * N;
+    const double *Fo = F + ((size_t)fac * NOPS + op) * N * N;
+    const int *Po = P + ((size_t)fac * NOPS + op) * N;
 
     // plain, v equation: DY2 (sum alpha v) at this lane's pivot rows, the column
     // loop in the reference's order
     double2 d2s[PLAIN ? NQ : 1];
     if (PLAIN && op == 0) {
-        double2 *sa = ssa[(w >> 1) % (PLAIN ? JPB : 1)];
+        double2 *sa = ssa[(w / NOPS) % (PLAIN ? JPB : 1)];
         const size_t o0 = (size_t)pc[0] * N;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
@@ -743,22 +745,25 @@ void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, 
 }
 
 void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int4 *jobs, int njobs, bool plain,
-                  hipStream_t s) {
-    // tuned jobs per block (two waves each): candidates 2, 1, 4
+                  int nops, hipStream_t s) {
+    // tuned jobs per block: candidates 2, 1, 4
     const int sel = c.tune[T_SOLVE_JOBS];
     if (plain) {
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, true, 2>), dim3((njobs + 1) / 2), dim3(256), 0, s, c, sp, F, P, jobs, njobs));
-    } else if (sel == 1) {
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 1>), dim3(8 * ((njobs + 7) / 8)), dim3(128), 0, s, c, sp, F, P, jobs,
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, true, 2, 2>), dim3((njobs + 1) / 2), dim3(256), 0, s, c, sp, F, P, jobs,
                                                  njobs));
-    } else if (sel == 2) {
-        const int nb = (njobs + 3) / 4;
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 4>), dim3(8 * ((nb + 7) / 8)), dim3(512), 0, s, c, sp, F, P, jobs,
-                                                 njobs));
+        return;
+    }
+    const int jpb = sel == 1 ? 1 : sel == 2 ? 4 : 2;
+    const int nb = (njobs + jpb - 1) / jpb;
+    const dim3 g(8 * ((nb + 7) / 8)), b(64 * nops * jpb);
+    if (nops == 2) {
+        if (jpb == 1) OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 1, 2>), g, b, 0, s, c, sp, F, P, jobs, njobs));
+        else if (jpb == 4) OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 4, 2>), g, b, 0, s, c, sp, F, P, jobs, njobs));
+        else OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 2, 2>), g, b, 0, s, c, sp, F, P, jobs, njobs));
     } else {
-        const int nb = (njobs + 1) / 2;
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 2>), dim3(8 * ((nb + 7) / 8)), dim3(256), 0, s, c, sp, F, P, jobs,
-                                                 njobs));
+        if (jpb == 1) OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 1, 1>), g, b, 0, s, c, sp, F, P, jobs, njobs));
+        else if (jpb == 4) OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 4, 1>), g, b, 0, s, c, sp, F, P, jobs, njobs));
+        else OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve<NQ, false, 2, 1>), g, b, 0, s, c, sp, F, P, jobs, njobs));
     }
 }
 

@@ -54,6 +54,18 @@ struct Operators {
 };
 Operators build_operators(int NY, double alpha_mesh);
 
+// D = DY2's interior block (rows and columns 1..N-2) as Q B Q^-1, B block
+// diagonal: rows (2m, 2m+1), m < npair, a 2 x 2 block [[a, b], [-b, a]] of the
+// pair a +- i b (lre = a, lim = b on both rows); the other rows real eigenvalues
+// (lim = 0). Q, Qinv: M x M column-major, M = N - 2. residual = max|D Q - Q B| /
+// (|D| |Q|), identity = max|Q Qinv - I| (eigen.cpp)
+struct HelmholtzBasis {
+    int M = 0, npair = 0;
+    std::vector<double> Q, Qinv, lre, lim;
+    double residual = 0, identity = 0;
+};
+HelmholtzBasis helmholtz_basis(const std::vector<double> &DY2, int N);
+
 // host dense LU with partial pivoting (dgetrf semantics), column-major
 int host_lu_factor(int n, double *A, int *ipiv);
 void host_lu_solve(int n, const double *LU, const int *ipiv, double *b);

@@ -303,6 +303,185 @@ __global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
 }
 
 // ---------------------------------------------------------------------------
+// omega_y of the new level, batched method (src/solver.cpp:574-603 for the
+// omega_y equation): rows 0 and N-1 are omega_y = 0, and on the interior
+//   (c I - nu D) x = r,  c = nu k^2 + gamma0/dt,  r = sum beta Hg + sum alpha omega_y / dt,
+// with D = Q B Q^-1 (eigen.cpp): x = Q (c I - nu B)^-1 Q^-1 r. Per block of 16
+// pencils (32 real columns) two GEMMs on the matrix cores:
+//   Z = Q^-1 R    R staged chunk by chunk in LDS as it is formed from the six
+//                 history fields; then each Z row scaled by its block of
+//                 (c I - nu B)^-1 (a 2 x 2 pair's partner row is the lane 16
+//                 apart) and the whole Z kept in LDS
+//   X = Q Z       and stored with the zero wall rows.
+// Q and Q^-1 are shared by every pencil and read from L2, like the DY matrices
+// of the derivative kernels; the operands of the next four k steps are in flight.
+// ---------------------------------------------------------------------------
+template <int RT>
+__global__ __launch_bounds__(512) void k_helmholtz_g(Chan c, StepPtrs sp) {
+    constexpr int NW = 8, PF = 4;
+    static_assert((kKC / 4) % PF == 0, "chunks hold whole rounds of the operand ring");
+    __shared__ double2 st[kMP][kKC + 1];
+    extern __shared__ double szs[];  // Z: 32 columns of ZS = 16 ntile + 1 rows
+    const int N = c.NY, M = N - 2, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int p0 = blockIdx.x * kMP, ntile = (M + 15) / 16, ZS = 16 * ntile + 1;
+    const double dt = c.dt, nu = c.nu;
+    // staging: this thread's element of a chunk (kMP * kKC == blockDim)
+    const int sq = threadIdx.x / kKC, skk = threadIdx.x % kKC, spn = p0 + sq;
+    const size_t so = (size_t)(spn < c.npencil ? spn : c.npencil - 1) * N;
+    double2 ng[3], nh[3];
+    auto fetch = [&](int k0) {
+        const int j = (k0 + skk < M ? k0 + skk : M - 1) + 1;  // interior row j of y
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            ng[l] = sp.g[l][so + j];
+            nh[l] = sp.hg[l][so + j];
+        }
+    };
+    auto stage = [&](int k0) {
+        double2 t = c2(0, 0);
+        if (spn < c.npencil && k0 + skk < M) {
+            const double2 a = caddc(caddc(cscale(ng[0], AL0), cscale(ng[1], AL1)), cscale(ng[2], AL2));
+            const double2 h = caddc(caddc(cscale(nh[0], BE0), cscale(nh[1], BE1)), cscale(nh[2], BE2));
+            t = caddc(h, cdiv(a, dt));
+        }
+        st[sq][skk] = t;
+    };
+    owt_d4 acc[RT][2];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = owt_d4{0.0, 0.0, 0.0, 0.0};
+    const int am = lane & 15, ak = lane >> 4, bq = (lane & 15) >> 1, bc = lane & 1;
+    // clamped, unconditional operand loads of an M x M column-major matrix
+    auto loadA = [&](const double *A, int k, double (&a1)[RT]) {
+        const size_t kr = (size_t)(k < M ? k : M - 1) * M;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            const int row = (w + NW * rt) * 16 + am;
+            a1[rt] = A[kr + (row < M ? row : M - 1)];
+        }
+    };
+    double q1[PF][RT];
+    // ---- Z = Q^-1 R ----
+    fetch(0);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int d = 0; d < PF; ++d) loadA(c.hQi, 4 * d + ak, q1[d]);
+#pragma unroll
+    for (int ci = 0; ci < kMaxNY / kKC; ++ci) {
+        const int k0 = ci * kKC;
+        if (k0 >= M) break;
+        stage(k0);
+        __syncthreads();
+        fetch(k0 + kKC);
+        const int nst = ((M - k0 < kKC ? M - k0 : kKC) + 3) / 4;
+#pragma unroll
+        for (int stp = 0; stp < kKC / 4; ++stp) {
+            const int d = stp % PF, kk = 4 * stp;
+            if (stp < nst) {
+                double bt[2];
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    const double2 x = st[ct * 8 + bq][kk + ak];
+                    bt[ct] = bc ? x.y : x.x;
+                }
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    if (w + NW * rt >= ntile) break;
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = mfma4(q1[d][rt], bt[ct], acc[rt][ct]);
+                }
+            }
+            loadA(c.hQi, k0 + kk + 4 * PF + ak, q1[d]);
+        }
+        __syncthreads();
+    }
+    // operands of the second product start loading under the scaling
+#pragma unroll
+    for (int d = 0; d < PF; ++d) loadA(c.hQ, 4 * d + ak, q1[d]);
+    // ---- Z <- (c I - nu B)^-1 Z, into LDS ----
+    double cp[2];  // c of this lane's two pencils
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+        const int p = p0 + ct * 8 + bq, pq = p < c.npencil ? p : 0;
+        cp[ct] = add(mul(nu, k2_of(c, pq / c.NXH, pq % c.NXH)), __ddiv_rn(G0, dt));
+    }
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int tile = w + NW * rt;
+        if (tile >= ntile) break;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = tile * 16 + 4 * r + (lane >> 4);
+            const int ic = i < M ? i : M - 1;
+            const double a = c.hlre[ic], b = c.hlim[ic];
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+                const double y = acc[rt][ct][r];
+                const double yo = __shfl_xor(y, 16);  // row i ^ 1 of the same column
+                const double cc = sub(cp[ct], mul(nu, a));
+                double z;
+                if (b == 0.0) {
+                    z = __ddiv_rn(y, cc);
+                } else {
+                    const double nb = mul(nu, b), det = add(mul(cc, cc), mul(nb, nb));
+                    z = (i & 1) ? __ddiv_rn(sub(mul(cc, y), mul(nb, yo)), det) : __ddiv_rn(add(mul(cc, y), mul(nb, yo)), det);
+                }
+                szs[(ct * 16 + am) * ZS + i] = i < M ? z : 0.0;
+            }
+        }
+    }
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = owt_d4{0.0, 0.0, 0.0, 0.0};
+    __syncthreads();
+    // ---- X = Q Z ----
+    const int nk = (M + 3) / 4;  // k steps; Z rows M..4 nk - 1 are zero
+#pragma unroll
+    for (int ci = 0; ci < kMaxNY / kKC; ++ci) {
+        const int k0 = ci * kKC;
+        if (k0 >= M) break;
+#pragma unroll
+        for (int stp = 0; stp < kKC / 4; ++stp) {
+            const int d = stp % PF, kk = k0 + 4 * stp;
+            if (kk < 4 * nk) {
+                double bt[2];
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) bt[ct] = szs[(ct * 16 + am) * ZS + kk + ak];
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    if (w + NW * rt >= ntile) break;
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) acc[rt][ct] = mfma4(q1[d][rt], bt[ct], acc[rt][ct]);
+                }
+            }
+            loadA(c.hQ, kk + 4 * PF + ak, q1[d]);
+        }
+    }
+    // store omega_y: interior rows from X, wall rows 0
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        const int tile = w + NW * rt;
+        if (tile >= ntile) break;
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double m = acc[rt][ct][r], o1 = __shfl_xor(m, 1);
+                const int i = tile * 16 + 4 * r + (lane >> 4), p = p0 + ct * 8 + bq;
+                if (bc != 0 || i >= M || p >= c.npencil) continue;
+                sp.gn[(size_t)p * N + i + 1] = c2(m, o1);
+            }
+        }
+    }
+    if (threadIdx.x < 2 * kMP) {
+        const int p = p0 + (threadIdx.x >> 1);
+        if (p < c.npencil) sp.gn[(size_t)p * N + ((threadIdx.x & 1) ? N - 1 : 0)] = c2(0, 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The plain method's derivative stages in the reference's arithmetic: each
 // mat-vec in column order (src/linalg.hpp:434-453) with every product and sum
 // rounded separately, TP pencils per block staged in LDS, rows looped over the
@@ -462,10 +641,10 @@ __device__ void mean_friction(const Chan &c, const double2 *U, const double2 *Wm
 
 // One block: right-hand sides of the mean streamwise and spanwise momentum
 // equations with the friction forcing (src/solver.cpp:452-470), the solve with
-// Ag(0) (factor mf, the k = 0 pencil's) in wave 0, then the friction and mean
+// Ag(0) (factor Fg, pivot rows Pg) in wave 0, then the friction and mean
 // vorticity.
 template <int NQ>
-__global__ __launch_bounds__(256) void k_mean(Chan c, StepPtrs sp, const double *F, const int *P, int mf) {
+__global__ __launch_bounds__(256) void k_mean(Chan c, StepPtrs sp, const double *Fg, const int *Pg) {
     __shared__ double S[4][64 * NQ];
     const int N = c.NY, lane = threadIdx.x & 63;
     const double dt = c.dt;
@@ -500,8 +679,8 @@ __global__ __launch_bounds__(256) void k_mean(Chan c, StepPtrs sp, const double 
     __syncthreads();
     if (threadIdx.x < 64) {
         double t[4][NQ];
-        lu_gather<NQ>(S, P + ((size_t)mf * 2 + 1) * N, N, lane, t);
-        lu_substitute<NQ, 4, 4, true>(F + ((size_t)mf * 2 + 1) * N * N, N, lane, t);
+        lu_gather<NQ>(S, Pg, N, lane, t);
+        lu_substitute<NQ, 4, 4, true>(Fg, N, lane, t);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int row = lane + 64 * q;
@@ -1096,8 +1275,16 @@ void launch_horizontal(const Chan &c, const StepPtrs &sp, bool exact, hipStream_
         hipLaunchKernelGGL(k_horizontal_mfma<3>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
 }
 
-void launch_mean(const Chan &c, const StepPtrs &sp, const double *F, const int *P, int mf, hipStream_t s) {
-    OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_mean<NQ>, dim3(1), dim3(256), 0, s, c, sp, F, P, mf));
+void launch_helmholtz_g(const Chan &c, const StepPtrs &sp, hipStream_t s) {
+    const int M = c.NY - 2, ntile = (M + 15) / 16;
+    const size_t lds = sizeof(double) * 2 * kMP * (16 * ntile + 1);
+    const dim3 g((c.npencil + kMP - 1) / kMP);
+    if (ntile <= 16) hipLaunchKernelGGL(k_helmholtz_g<2>, g, dim3(512), lds, s, c, sp);
+    else hipLaunchKernelGGL(k_helmholtz_g<3>, g, dim3(512), lds, s, c, sp);
+}
+
+void launch_mean(const Chan &c, const StepPtrs &sp, const double *Fg, const int *Pg, hipStream_t s) {
+    OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_mean<NQ>, dim3(1), dim3(256), 0, s, c, sp, Fg, Pg));
 }
 
 void launch_nonlinear(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s) {

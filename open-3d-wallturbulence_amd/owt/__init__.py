@@ -26,7 +26,7 @@ FIELDS = {"u": 0, "v": 1, "w": 2, "omx": 3, "omy": 4, "omz": 5,
 PLAIN, BATCHED = 0, 1
 FP64, MIXED = 0, 1
 KERNELS = {"solve": 0, "horizontal": 1, "mean": 2, "z_inverse": 3, "x_lamb": 4, "z_forward": 5, "nonlinear": 6,
-           "factor": 7}
+           "factor": 7, "helmholtz": 8}
 
 # every symbol include/owt.h declares
 EXPORTS = [

@@ -376,7 +376,8 @@ struct Oracle {
     //  noise: every value of the fields named by noise_where is multiplied by
     //    (1 + noise_rel g), g ~ N(0, 1) from a stream keyed by (noise_seed, step,
     //    field): bit 1 the Lamb vector after each evaluation (the transforms'
-    //    rounding), bit 2 v and omega_y after each solve (the solves' rounding).
+    //    rounding), bit 2 v and omega_y after each solve (the solves' rounding),
+    //    bit 4 omega_y alone after each solve.
     //    Runs with noise give the spread that rounding alone causes, the floor
     //    below which two correct implementations cannot agree.
     //  mutate: planted mistakes that the pins in tests/ must detect: bit 1 the
@@ -917,6 +918,7 @@ struct Oracle {
         }
         if (errflag) throw std::runtime_error("LU factorization failed. Matrix A may be singular.");
         if (noise_where & 2) { perturb(vely, 4); perturb(vory, 5); }
+        if (noise_where & 4) perturb(vory, 6);
         solve_horizontal();
         lamb_vec();
         nonlinear_source();
