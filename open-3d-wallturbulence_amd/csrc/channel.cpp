@@ -195,9 +195,11 @@ void Channel::upload_operators() {
 // only: factor each distinct value once (the reference refactors every pencil
 // every step). Systems are grouped by the bit pattern of k^2 as the device
 // computes it, kx*kx + kz*kz with each product rounded. The step solves the
-// omega_y systems through the eigendecomposition of DY2 (k_helmholtz_g), so only
-// Av is stored per k^2 (bops_ = 1), plus Ag(0) for the mean flow; an adjoint
-// channel, whose transposed solves use LU factors of both, keeps both (bops_ = 2).
+// omega_y systems through the eigendecomposition of DY2 (k_helmholtz_g) and the
+// v systems with the explicit inverse of each Av (k_solve_inv), so Av is
+// factored per k^2 (bops_ = 1) and inverted, plus Ag(0) is factored for the mean
+// flow; an adjoint channel, whose transposed solves use LU factors of both,
+// keeps both (bops_ = 2).
 void Channel::factor_operators() {
     if (factored_) return;
     const size_t N = c_.NY;
@@ -245,6 +247,7 @@ void Channel::factor_operators() {
         factor_set(Fg0_, Pg0_, 1, sys0_, 1, 1);
         Fmean_ = Fg0_;
         Pmean_ = Pg0_;
+        invert_operators();
     } else {
         Fmean_ = Fb_ + ((size_t)fmap_host_[0] * 2 + 1) * N * N;
         Pmean_ = Pb_ + ((size_t)fmap_host_[0] * 2 + 1) * N;
@@ -321,6 +324,18 @@ void Channel::factor_set(double *F, int *P, int nfac, const int *frep, int op0, 
         fscratch_ = (double *)dmalloc((size_t)fchunk_ * per);
     }
     launch_factor(c_, F, P, nfac, frep, op0, nops, fscratch_, fchunk_ > 0 ? fchunk_ : 1, force_panel_, s_);
+}
+
+// Explicit inverses of the Av factors for the batched v solves (launch_invert),
+// the unrefined inverses of up to about 1 GiB of matrices at a time in scratch.
+void Channel::invert_operators() {
+    const size_t per = (size_t)c_.NY * c_.NY * sizeof(double);
+    if (!Xb_) {
+        Xb_ = (double *)dmalloc((size_t)nfac_ * per);
+        xchunk_ = (int)std::min<size_t>((size_t)nfac_, std::max<size_t>(1, ((size_t)1 << 30) / per));
+        xscratch_ = (double *)dmalloc((size_t)xchunk_ * per);
+    }
+    launch_invert(c_, Fb_, Pb_, nfac_, Xb_, xscratch_, xchunk_, s_);
 }
 
 // the plain method's per-pencil factor set (refilled every step)
@@ -639,8 +654,12 @@ void Channel::enqueue_step_body(int ring) {
     const double *F = plain ? Fp_ : Fb_;
     const int *P = plain ? Pp_ : Pb_;
     if (plain) factor_set(Fp_, Pp_, c_.npencil, nullptr, 0, 2);
-    launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, plain ? 2 : bops_, s_);
-    if (!plain && bops_ == 1) launch_helmholtz_g(c_, sp, s_);
+    if (!plain && bops_ == 1) {
+        launch_solve_inv(c_, sp, Xb_, jobs_, njobs_, s_);
+        launch_helmholtz_g(c_, sp, s_);
+    } else {
+        launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, plain ? 2 : bops_, s_);
+    }
     launch_horizontal(c_, sp, plain, s_);
     // plain: pencil 0's own factors hold Ag(0) as its second operator
     launch_mean(c_, sp, plain ? Fp_ + (size_t)c_.NY * c_.NY : Fmean_, plain ? Pp_ + c_.NY : Pmean_, s_);
@@ -692,7 +711,10 @@ void Channel::enqueue_kernel(int id) {
     const size_t W = c_.W;
     const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
     switch (id) {
-        case 0: launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, plain ? 2 : bops_, s_); break;
+        case 0:
+            if (!plain && bops_ == 1) launch_solve_inv(c_, sp, Xb_, jobs_, njobs_, s_);
+            else launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, plain ? 2 : bops_, s_);
+            break;
         case 1: launch_horizontal(c_, sp, plain, s_); break;
         case 2:
             launch_mean(c_, sp, plain ? Fp_ + (size_t)c_.NY * c_.NY : Fmean_, plain ? Pp_ + c_.NY : Pmean_, s_);
@@ -702,8 +724,12 @@ void Channel::enqueue_kernel(int id) {
         case 5: launch_z_forward(c_, s_); break;
         case 6: launch_nonlinear(c_, sp, plain, s_); break;
         case 7:
-            if (plain) factor_set(Fp_, Pp_, c_.npencil, nullptr, 0, 2);
-            else factor_set(Fb_, Pb_, nfac_, frep_, 0, bops_);
+            if (plain) {
+                factor_set(Fp_, Pp_, c_.npencil, nullptr, 0, 2);
+            } else {
+                factor_set(Fb_, Pb_, nfac_, frep_, 0, bops_);
+                if (bops_ == 1) invert_operators();
+            }
             break;
         case 8:
             if (!plain && bops_ == 1) launch_helmholtz_g(c_, sp, s_);

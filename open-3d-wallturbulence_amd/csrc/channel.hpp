@@ -126,6 +126,7 @@ class Channel {
     void factor_operators();
     void ensure_plain_factors();
     void factor_set(double *F, int *P, int nfac, const int *frep, int op0, int nops);
+    void invert_operators();
     void pressure(double *out);
     void init_post();
     StepPtrs step_ptrs(int ring) const;
@@ -160,6 +161,10 @@ class Channel {
     int *Pg0_ = nullptr, *sys0_ = nullptr;
     const double *Fmean_ = nullptr;
     const int *Pmean_ = nullptr;
+    // explicit inverses of the Av factors (bops_ == 1), the step's v solves,
+    // and the scratch of their computation (xchunk_ matrices)
+    double *Xb_ = nullptr, *xscratch_ = nullptr;
+    int xchunk_ = 0;
     // all Ag factors, made only when requested (get_factor) with bops_ == 1
     double *Fag_ = nullptr;
     int *Pag_ = nullptr;

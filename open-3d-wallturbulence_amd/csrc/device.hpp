@@ -141,6 +141,12 @@ void launch_pressure_solve(const Chan &c, const double *F, const int *P, const i
 // index in F (2: Av and Ag, both solved; 1: Av only, omega_y by launch_helmholtz_g)
 void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int *P, const int4 *jobs, int njobs, bool plain,
                   int nops, hipStream_t s);
+// explicit inverses X_f = Av_f^-1 (N x N column-major each) of nfac Av factors
+// (bops 1), refined once; scratch holds chunk matrices
+void launch_invert(const Chan &c, const double *F, const int *P, int nfac, double *X, double *scratch,
+                   int chunk, hipStream_t s);
+// the batched method's v solves with the inverses X of Av (one per factor index)
+void launch_solve_inv(const Chan &c, const StepPtrs &sp, const double *X, const int4 *jobs, int njobs, hipStream_t s);
 // omega_y of the new level by the eigendecomposition of DY2 (batched method)
 void launch_helmholtz_g(const Chan &c, const StepPtrs &sp, hipStream_t s);
 // exact: the plain method's kernels in the reference's arithmetic (column-order

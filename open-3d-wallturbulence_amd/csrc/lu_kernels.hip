@@ -585,6 +585,267 @@ __global__ __launch_bounds__(64 * NOPS * JPB) void k_solve(Chan c, StepPtrs sp, 
 }
 
 // ---------------------------------------------------------------------------
+// The batched method's v solves by the explicit inverse X = Av^-1 of each
+// distinct k^2, so every step's solve is x = X b: a product streamed column by
+// column with no dependency from one column to the next, where the
+// substitution waits at every step for the one before. X is computed once:
+//  1. X0 from the LU factors (k_invert: the substitution applied to the
+//     identity's columns, eight per wave);
+//  2. one step of iterative refinement (k_refine_inverse),
+//     X1 = X0 + X0 (I - P^-1 L U X0), with the residual accumulated in twice
+//     the working precision (Dot2 of Ogita, Rump and Oishi, "Accurate sum and
+//     dot product", SIAM J. Sci. Comput. 26 (2005), Algorithm 5.3).
+// For these operators (condition numbers up to about 3e10) X0 carries errors
+// near cond * eps that are the same at every step, a fixed change of the
+// operator. After one refinement step X is the inverse of the factored
+// operator P^-1 L U to within a few roundings: x = X b differs from the
+// substitution with the same factors by the substitution's own rounding, which
+// changes from step to step with the right-hand side.
+// ---------------------------------------------------------------------------
+template <int NQ>
+__global__ __launch_bounds__(64) void k_invert(const double *F, const int *P, int N, double *X) {
+    const int lane = threadIdx.x, f = blockIdx.x, c0 = blockIdx.y * 8;
+    const double *Fo = F + (size_t)f * N * N;
+    const int *Po = P + (size_t)f * N;
+    double t[8][NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int pos = lane + 64 * q;
+        const int r = pos < N ? Po[pos] : -1;  // unit vector e_col, gathered in pivot order
+#pragma unroll
+        for (int k = 0; k < 8; ++k) t[k][q] = (r == c0 + k) ? 1.0 : 0.0;
+    }
+    lu_substitute<NQ, 8>(Fo, N, lane, t);
+    double *Xo = X + (size_t)f * N * N;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int row = lane + 64 * q;
+        if (row >= N) continue;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (c0 + k < N) Xo[(size_t)(c0 + k) * N + row] = t[k][q];
+    }
+}
+
+// error-free transformations: a + b = x + y and a * b = x + y exactly
+__device__ __forceinline__ void two_sum(double a, double b, double &x, double &y) {
+#pragma clang fp contract(off)
+    x = a + b;
+    const double z = x - a;
+    y = (a - (x - z)) + (b - z);
+}
+__device__ __forceinline__ void two_prod(double a, double b, double &x, double &y) {
+    x = mul(a, b);
+    y = __builtin_fma(a, b, -x);
+}
+
+// accumulator of a compensated dot product: the sum is p + q
+struct Dot2 {
+    double p, q;
+    __device__ __forceinline__ void add_prod(double a, double b) {
+        double h, r, t;
+        two_prod(a, b, h, r);
+        two_sum(p, h, p, t);
+        q = add(q, add(t, r));
+    }
+    __device__ __forceinline__ void add_value(double a) {
+        double t;
+        two_sum(p, a, p, t);
+        q = add(q, t);
+    }
+};
+
+// One block per (matrix, tile of RCT columns j), the tiles of a matrix dealt
+// to one XCD in a row so that they share its L2 copy of the factor and of X0.
+//  1. Y = U X0(:, j) as a double-double pair (yh, yl), one thread per row;
+//  2. R(:, j) = e_j - P^-1 L Y: row pos of L Y is row perm[pos] of the factored
+//     operator's product, accumulated in Dot2 form and rounded once;
+//  3. X1(:, j) = X0(:, j) + X0 R(:, j).
+// The residual is taken against P^-1 L U, the operator the factors represent,
+// not against Av itself: x = X1 b then reproduces the substitution with these
+// factors up to the rounding of the substitution alone. X0 (nf matrices,
+// column-major) and X1 are distinct buffers: other blocks still read X0 while
+// X1 is written.
+template <int RCT>
+__global__ __launch_bounds__(256) void k_refine_inverse(Chan c, const double *F, const int *P, int nf, const double *X0,
+                                                        double *X1) {
+    __shared__ double xs[RCT][kMaxNY], yh[RCT][kMaxNY], yl[RCT][kMaxNY];
+    const int N = c.NY, nrt = (N + RCT - 1) / RCT;
+    const int nb = nf * nrt, per = (nb + 7) / 8;
+    const int idx = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (idx >= nb) return;
+    const int f = idx / nrt, j0 = (idx % nrt) * RCT;
+    const double *Fo = F + (size_t)f * N * N, *Xo = X0 + (size_t)f * N * N;
+    const int *Po = P + (size_t)f * N;
+    for (int e = threadIdx.x; e < RCT * N; e += blockDim.x) {
+        const int cc = e / N, k = e % N;
+        xs[cc][k] = j0 + cc < N ? Xo[(size_t)(j0 + cc) * N + k] : 0.0;
+    }
+    __syncthreads();
+    // 1. Y = U X0: row m sums over k >= m; a wave starts at its first row
+    for (int m = threadIdx.x; m < N; m += blockDim.x) {
+        Dot2 d[RCT];
+#pragma unroll
+        for (int cc = 0; cc < RCT; ++cc) d[cc] = {0.0, 0.0};
+        for (int k = m & ~63; k < N; ++k) {
+            const double u = k >= m ? Fo[lu_upper_at(N, k) + (m < k ? m : k)] : 0.0;
+#pragma unroll
+            for (int cc = 0; cc < RCT; ++cc) d[cc].add_prod(u, xs[cc][k]);
+        }
+#pragma unroll
+        for (int cc = 0; cc < RCT; ++cc) two_sum(d[cc].p, d[cc].q, yh[cc][m], yl[cc][m]);
+    }
+    __syncthreads();
+    // 2. R = e_j - P^-1 L Y (L unit lower): row pos sums over m < pos; a wave
+    // stops after its last row. R overwrites the staged X0 columns.
+    double rv[2][RCT];
+    int rrow[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int pos = threadIdx.x + 256 * h;
+        rrow[h] = -1;
+        if (pos >= N) continue;
+        const int row = Po[pos];
+        rrow[h] = row;
+        Dot2 d[RCT];
+#pragma unroll
+        for (int cc = 0; cc < RCT; ++cc) {
+            d[cc] = {row == j0 + cc ? 1.0 : 0.0, 0.0};
+            d[cc].add_value(-yh[cc][pos]);
+            d[cc].add_value(-yl[cc][pos]);
+        }
+        const int mend = (pos | 63) + 1 < N ? (pos | 63) + 1 : N;
+        for (int m = 0; m < mend; ++m) {
+            const double l = m < pos ? Fo[lu_lower_at(N, m) + (m < pos ? pos - m - 1 : 0)] : 0.0;
+#pragma unroll
+            for (int cc = 0; cc < RCT; ++cc) {
+                d[cc].add_prod(-l, yh[cc][m]);
+                d[cc].q = fma(-l, yl[cc][m], d[cc].q);
+            }
+        }
+#pragma unroll
+        for (int cc = 0; cc < RCT; ++cc) rv[h][cc] = add(d[cc].p, d[cc].q);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+        if (rrow[h] >= 0)
+#pragma unroll
+            for (int cc = 0; cc < RCT; ++cc) xs[cc][rrow[h]] = rv[h][cc];
+    __syncthreads();
+    // 3. X1 = X0 + X0 R
+    double *Xn = X1 + (size_t)f * N * N;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double acc[RCT];
+#pragma unroll
+        for (int cc = 0; cc < RCT; ++cc) acc[cc] = 0.0;
+        for (int k = 0; k < N; ++k) {
+            const double x = Xo[(size_t)k * N + i];
+#pragma unroll
+            for (int cc = 0; cc < RCT; ++cc) acc[cc] = fma(x, xs[cc][k], acc[cc]);
+        }
+#pragma unroll
+        for (int cc = 0; cc < RCT; ++cc)
+            if (j0 + cc < N) Xn[(size_t)(j0 + cc) * N + i] = Xo[(size_t)(j0 + cc) * N + i] + acc[cc];
+    }
+}
+
+// One wave per batched job {factor, system, system or -1} (up to four pencils,
+// eight real right-hand sides), JPB jobs per block. The right-hand sides are
+// formed in natural row order (src/solver.cpp:568-574, 593-599, the v equation,
+// DY2 sum alpha v from the stored DY2 v of each level) into the wave's LDS, then
+// x_row = sum_j X(row, j) b_j: lane l accumulates rows l + 64q, the columns of X
+// read coalesced and eight at a time ahead of their use.
+template <int NQ, int JPB>
+__global__ __launch_bounds__(64 * JPB) void k_solve_inv(Chan c, StepPtrs sp, const double *X, const int4 *jobs, int njobs) {
+    __shared__ double sb[JPB][64 * NQ][8];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nb = (njobs + JPB - 1) / JPB, per = (nb + 7) / 8;
+    const int blk = (blockIdx.x % 8) * per + blockIdx.x / 8;  // XCD-contiguous jobs
+    if (blk >= nb) return;
+    const int job = blk * JPB + w;
+    const int N = c.NY;
+    if (job >= njobs) return;
+    const int4 jb = jobs[job];
+    int pc[4];
+    job_pencils(c, jb, pc);
+    const int kz = jb.y / c.NXH, ix = jb.y % c.NXH;
+    const double dt = c.dt, k2dt = __ddiv_rn(k2_of(c, kz, ix), dt);
+    double(*b)[8] = sb[w];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int r = lane + 64 * q;
+        const int rr = r < N ? r : 0;
+        const bool bc = r <= 1 || r >= N - 2;
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) {
+            const size_t o = (size_t)(pc[pp] >= 0 ? pc[pp] : pc[0]) * N + rr;
+            const double2 a = caddc(caddc(cscale(sp.v[0][o], AL0), cscale(sp.v[1][o], AL1)), cscale(sp.v[2][o], AL2));
+            const double2 h = caddc(caddc(cscale(sp.hv[0][o], BE0), cscale(sp.hv[1][o], BE1)), cscale(sp.hv[2][o], BE2));
+            const double2 d = caddc(caddc(cscale(sp.dv[0][o], AL0), cscale(sp.dv[1][o], AL1)), cscale(sp.dv[2][o], AL2));
+            double2 rhs = caddc(csubc(h, cscale(a, k2dt)), cdiv(d, dt));
+            if (bc || r >= N || pc[pp] < 0) rhs = c2(0, 0);
+            b[r][2 * pp] = rhs.x;
+            b[r][2 * pp + 1] = rhs.y;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const double *Xo = X + (size_t)jb.x * N * N;
+    double acc[8][NQ];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) acc[k][q] = 0.0;
+    int rc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) rc[q] = lane + 64 * q < N ? lane + 64 * q : N - 1;
+    // columns in chunks of CJ, the next chunk's loads issued before this one's use
+    constexpr int CJ = 8;
+    double xa[CJ][NQ], xb[CJ][NQ];
+    auto load = [&](int j0, double (&xc)[CJ][NQ]) {
+#pragma unroll
+        for (int cc = 0; cc < CJ; ++cc) {
+            const int j = j0 + cc < N ? j0 + cc : N - 1;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) xc[cc][q] = Xo[(size_t)j * N + rc[q]];
+        }
+    };
+    auto apply = [&](int j0, double (&xc)[CJ][NQ]) {
+#pragma unroll
+        for (int cc = 0; cc < CJ; ++cc) {
+            const int j = j0 + cc;
+            if (j >= N) break;
+            double bj[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) bj[k] = b[j][k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) acc[k][q] = fma(xc[cc][q], bj[k], acc[k][q]);
+        }
+    };
+    load(0, xa);
+    for (int j0 = 0; j0 < N; j0 += 2 * CJ) {
+        if (j0 + CJ < N) load(j0 + CJ, xb);
+        apply(j0, xa);
+        if (j0 + CJ >= N) break;
+        if (j0 + 2 * CJ < N) load(j0 + 2 * CJ, xa);
+        apply(j0 + CJ, xb);
+    }
+    double2 *XN = sp.vn;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int row = lane + 64 * q;
+        if (row >= N) continue;
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp)
+            if (pc[pp] >= 0) XN[(size_t)pc[pp] * N + row] = c2(acc[2 * pp][q], acc[2 * pp + 1][q]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Dynamic pressure (no reference counterpart): the right-hand sides R of the
 // current level (y_kernels.hip) solved with the pressure operator
 //   (DY2 - k^2) P = R on rows 1..N-2,  DY1 P = R on rows 0 and N-1,
@@ -767,6 +1028,34 @@ void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int 
     }
 }
 
+
+void launch_invert(const Chan &c, const double *F, const int *P, int nfac, double *X, double *scratch,
+                   int chunk, hipStream_t s) {
+    const size_t NN = (size_t)c.NY * c.NY;
+    constexpr int RCT = 8;
+    const int nch = (c.NY + 7) / 8, nrt = (c.NY + RCT - 1) / RCT;
+    static_assert(kMaxNY <= 512, "k_refine_inverse holds at most two rows per thread");
+    if (chunk > 65535) chunk = 65535;
+    for (int f0 = 0; f0 < nfac; f0 += chunk) {
+        const int nf = nfac - f0 < chunk ? nfac - f0 : chunk;
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_invert<NQ>, dim3(nf, nch), dim3(64), 0, s, F + f0 * NN, P + (size_t)f0 * c.NY,
+                                                 c.NY, scratch));
+        const int nb = nf * nrt;
+        hipLaunchKernelGGL(k_refine_inverse<RCT>, dim3(8 * ((nb + 7) / 8)), dim3(256), 0, s, c, F + f0 * NN, P + (size_t)f0 * c.NY,
+                           nf, scratch, X + f0 * NN);
+    }
+}
+
+void launch_solve_inv(const Chan &c, const StepPtrs &sp, const double *X, const int4 *jobs, int njobs, hipStream_t s) {
+    // tuned jobs (waves) per block: candidates 2, 1, 4
+    const int sel = c.tune[T_SOLVE_JOBS];
+    const int jpb = sel == 1 ? 1 : sel == 2 ? 4 : 2;
+    const int nb = (njobs + jpb - 1) / jpb;
+    const dim3 g(8 * ((nb + 7) / 8)), b(64 * jpb);
+    if (jpb == 1) OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_inv<NQ, 1>), g, b, 0, s, c, sp, X, jobs, njobs));
+    else if (jpb == 4) OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_inv<NQ, 4>), g, b, 0, s, c, sp, X, jobs, njobs));
+    else OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_inv<NQ, 2>), g, b, 0, s, c, sp, X, jobs, njobs));
+}
 
 void launch_factor_transpose(double *F, int nmat, int N, hipStream_t s) {
     for (int m0 = 0; m0 < nmat; m0 += 65535) {
