@@ -63,10 +63,18 @@ def stage_bytes(W, I, es):
     }
 
 
+# the kernel of each stage in the mixed-precision batched step
+STAGE_KERNEL = {"solve": "k_solve_inv", "helmholtz": "k_helmholtz_g", "horizontal": "k_horizontal_mfma",
+                "mean": "k_mean", "z_inverse": "k_z_inverse", "x_lamb": "k_x_lamb", "z_forward": "k_z_forward",
+                "nonlinear": "k_nonlinear_mfma"}
+
+
 def operator_bytes(N, nfac):
-    """Bytes of stored LU factors one solve launch streams: nfac factor indices
-    x one operator (Av) x N^2 doubles (design overhead, reported beside the
-    roofline; the omega_y solves read two shared (N-2)^2 matrices)."""
+    """Bytes of the stored explicit inverses of Av one solve launch streams at
+    least once: nfac distinct k^2 x N^2 doubles (operator data, reported beside
+    the roofline; a job streams its inverse once for up to two +-kz systems, so
+    an inverse shared by more systems is read more than once; the omega_y solves
+    read two shared (N-2)^2 matrices)."""
     return nfac * N * N * 8
 
 
@@ -402,7 +410,8 @@ def main():
         # step; FP64 transform variants run once at initialisation)
         # the launch shape in use: the profile run autotunes first, so the candidates
         # it rejected ran only while being timed and the chosen one has the most launches
-        hits = [(n, v) for k, (v, n) in per.items() if (f"owt::k_{dom}<" in k or f"owt::k_{dom}(" in k)
+        sym = STAGE_KERNEL[dom]
+        hits = [(n, v) for k, (v, n) in per.items() if (f"owt::{sym}<" in k or f"owt::{sym}(" in k)
                 and "true," not in k and "true>" not in k and "<double" not in k and "type<double" not in k]
         if hits:
             roof["traffic"] = round(max(hits)[1])

@@ -215,6 +215,15 @@ int owt_tangent_step(owt_handle *h, int nsteps);
 enum { OWT_OBJ_MEAN_FX = 0 };
 int owt_adjoint_run(owt_handle *h, int K, int ckpt_every, int objective, int first, int last, double *J, double *grad_u,
                     double *grad_v, double *grad_w);
+/* The same reverse sweep for the linear map L from the initial velocity to the
+ * velocity after K steps (about h's trajectory): writes L^T w, w = (wu, wv, ww)
+ * given in the wave layout, as grad_u, grad_v, grad_w (same convention as
+ * owt_adjoint_run). With the tangent d of owt_tangent_init and its velocity
+ * after K steps, L d, the two satisfy <L d, w> = <d, L^T w> with
+ * <a, b> = sum Re(a) Re(b) + Im(a) Im(b) over the three fields. Leaves h after
+ * step K. */
+int owt_adjoint_apply(owt_handle *h, int K, int ckpt_every, const double *wu, const double *wv, const double *ww, double *grad_u,
+                      double *grad_v, double *grad_w);
 
 /* Ensembles: n independent channels on one GPU (sizes and parameters may
  * differ), stepped together. No reference counterpart (the reference runs one

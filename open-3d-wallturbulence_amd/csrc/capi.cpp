@@ -190,6 +190,12 @@ int owt_adjoint_run(owt_handle *h, int K, int ckpt_every, int objective, int fir
     return guard(h, [&] { *J = h->ch->adjoint_run(K, ckpt_every, first, last, grad_u, grad_v, grad_w); });
 }
 
+int owt_adjoint_apply(owt_handle *h, int K, int ckpt_every, const double *wu, const double *wv, const double *ww, double *grad_u,
+                      double *grad_v, double *grad_w) {
+    if (!wu || !wv || !ww || !grad_u || !grad_v || !grad_w) return OWT_E_ARG;
+    return guard(h, [&] { h->ch->adjoint_run(K, ckpt_every, K, K, grad_u, grad_v, grad_w, wu, wv, ww); });
+}
+
 int owt_ensemble_create(int n, const owt_params *params, const owt_opts *opts, owt_ensemble **out) {
     if (!out) return OWT_E_ARG;
     *out = nullptr;

@@ -1169,13 +1169,13 @@ void Channel::adjoint_lamb(const double2 *base_zbuf, int slot) {
 }
 
 // transpose of one step at ring phase `ring` (reverse stage order)
-void Channel::adjoint_step(int ring, const double2 *base_zbuf, double seed) {
+void Channel::adjoint_step(int ring, const double2 *base_zbuf, double seed, const double *Xv) {
     const StepPtrs sp = step_ptrs(ring);
     launch_nonlinear_adj(c_, sp, s_);
     adjoint_lamb(base_zbuf, sp.sn);
     launch_mean_adj(c_, sp, Fb_, Pb_, fmap_host_[0], seed, s_);
     launch_horizontal_adj(c_, sp, s_);
-    launch_solve_adj(c_, sp, Fb_, Pb_, jobs_, njobs_, s_);
+    launch_solve_adj(c_, sp, Fb_, Pb_, Xv, jobs_, njobs_, s_);
     OWT_HIP(hipGetLastError());
 }
 
@@ -1194,7 +1194,10 @@ void Channel::adjoint_init(const double2 *base_zbuf, double *gu, double *gv, dou
     OWT_HIP(hipStreamSynchronize(s_));
 }
 
-double Channel::adjoint_run(int K, int ckpt, int first, int last, double *gu, double *gv, double *gw) {
+double Channel::adjoint_run(int K, int ckpt, int first, int last, double *gu, double *gv, double *gw, const double *tu,
+                            const double *tv, const double *tw) {
+    const bool terminal = tu && tv && tw;
+    if (terminal) first = last = K;
     if (c_.precision != 0) throw Error(2, "the adjoint needs an FP64 channel");
     if (K < 1 || ckpt < 1 || first < 1 || first > last || last > K)
         throw Error(2, "adjoint: need K >= 1, ckpt >= 1 and 1 <= first <= last <= K");
@@ -1214,8 +1217,10 @@ double Channel::adjoint_run(int K, int ckpt, int first, int last, double *gu, do
     OWT_HIP(hipMemcpy(d.data(), c_.diag, d.size() * sizeof(double), hipMemcpyDeviceToHost));
     const double wgt = 1.0 / (double)(last - first + 1);
     double J = 0.0;
-    for (int m = first; m <= last; ++m) J += d[(size_t)m * 3];
-    J *= wgt;
+    if (!terminal) {
+        for (int m = first; m <= last; ++m) J += d[(size_t)m * 3];
+        J *= wgt;
+    }
     if (!adjoint_) {
         Params q = P_;
         q.precision = 0;
@@ -1226,13 +1231,23 @@ double Channel::adjoint_run(int K, int ckpt, int first, int last, double *gu, do
     Channel &A = *adjoint_;
     A.adjoint_prepare();
     A.zero_state();
+    if (terminal) {
+        // the velocity after step K: u and w in cur, v in the history slot of level K
+        const size_t W = c_.W;
+        const StepPtrs spK = A.step_ptrs((K - 1) % 3);
+        OWT_HIP(hipMemcpyAsync(A.c_.cur, tu, W * sizeof(double2), hipMemcpyHostToDevice, s_));
+        OWT_HIP(hipMemcpyAsync(A.c_.cur + W, tw, W * sizeof(double2), hipMemcpyHostToDevice, s_));
+        OWT_HIP(hipMemcpyAsync(spK.vn, tv, W * sizeof(double2), hipMemcpyHostToDevice, s_));
+    }
+    // the transposed v solves use the forward step's explicit inverses (batched method)
+    const double *Xv = P_.method == 1 ? Xb_ : nullptr;
     // reverse sweep: step n -> n+1 needs the base's six fields of level n+1
     for (int n = K - 1; n >= 0; --n) {
         const int c0 = (n / ckpt) * ckpt;
         load_state(ckpts_[n / ckpt]);
         for (int m = c0; m <= n; ++m) enqueue_step();
-        const double seed = (n + 1 >= first && n + 1 <= last) ? wgt : 0.0;
-        A.adjoint_step(n % 3, (const double2 *)c_.zbuf, seed);
+        const double seed = (!terminal && n + 1 >= first && n + 1 <= last) ? wgt : 0.0;
+        A.adjoint_step(n % 3, (const double2 *)c_.zbuf, seed, Xv);
     }
     load_state(ckpts_[0]);
     six_to_zbuf(0);

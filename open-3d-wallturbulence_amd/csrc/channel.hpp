@@ -101,7 +101,11 @@ class Channel {
     // base state recomputed from its checkpoint. Returns J; grad_* get dJ/d(u,
     // v, w) of the initial field ([NZ][NXH][NY] complex, d/dRe + i d/dIm). The
     // channel is left after step K.
-    double adjoint_run(int K, int ckpt, int first, int last, double *gu, double *gv, double *gw);
+    // objective J = mean fx_fric over steps first..last; or, with a terminal seed
+    // (tu, tv, tw non-null), the adjoint of the map initial velocity -> velocity
+    // after K steps applied to the seed (J = 0)
+    double adjoint_run(int K, int ckpt, int first, int last, double *gu, double *gv, double *gw, const double *tu = nullptr,
+                       const double *tv = nullptr, const double *tw = nullptr);
 
   private:
     // device copy of the whole evolving state (in-memory checkpoint)
@@ -118,7 +122,7 @@ class Channel {
     void six_to_zbuf(int slot);
     void adjoint_prepare();
     void adjoint_lamb(const double2 *base_zbuf, int slot);
-    void adjoint_step(int ring, const double2 *base_zbuf, double seed);
+    void adjoint_step(int ring, const double2 *base_zbuf, double seed, const double *Xv);
     void adjoint_init(const double2 *base_zbuf, double *gu, double *gv, double *gw);
     void alloc();
     void free_all();

@@ -175,7 +175,9 @@ void launch_z_inverse_T(const Chan &c, const double2 *in, int nf, double2 *out, 
 // factor set -> the form solving with A^T (nmat matrices of N x N, in place)
 void launch_factor_transpose(double *F, int nmat, int N, hipStream_t s);
 // transposes of the step's stages on an adjoint state c at the ring phase of sp
-void launch_solve_adj(const Chan &c, const StepPtrs &sp, const double *G, const int *P, const int4 *jobs, int njobs, hipStream_t s);
+// Xv: the forward step's explicit inverses of Av (null: the v solve transposes the LU factors too)
+void launch_solve_adj(const Chan &c, const StepPtrs &sp, const double *G, const int *P, const double *Xv, const int4 *jobs,
+                      int njobs, hipStream_t s);
 void launch_horizontal_adj(const Chan &c, const StepPtrs &sp, hipStream_t s);
 void launch_mean_adj(const Chan &c, const StepPtrs &sp, const double *G, const int *P, int mf, double seed, hipStream_t s);
 void launch_nonlinear_adj(const Chan &c, const StepPtrs &sp, hipStream_t s);

@@ -602,9 +602,14 @@ __global__ __launch_bounds__(64 * NOPS * JPB) void k_solve(Chan c, StepPtrs sp, 
 // substitution with the same factors by the substitution's own rounding, which
 // changes from step to step with the right-hand side.
 // ---------------------------------------------------------------------------
+// One wave per (matrix, 8 columns); the column groups of a matrix are dealt to
+// one XCD in a row, so that they share its L2 copy of the factor.
 template <int NQ>
-__global__ __launch_bounds__(64) void k_invert(const double *F, const int *P, int N, double *X) {
-    const int lane = threadIdx.x, f = blockIdx.x, c0 = blockIdx.y * 8;
+__global__ __launch_bounds__(64) void k_invert(const double *F, const int *P, int N, int nf, double *X) {
+    const int nch = (N + 7) / 8, nb = nf * nch, per = (nb + 7) / 8;
+    const int idx = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (idx >= nb) return;
+    const int lane = threadIdx.x, f = idx / nch, c0 = (idx % nch) * 8;
     const double *Fo = F + (size_t)f * N * N;
     const int *Po = P + (size_t)f * N;
     double t[8][NQ];
@@ -913,8 +918,13 @@ __global__ __launch_bounds__(256) void k_factor_transpose(double *F, int N) {
 // through A^-T, boundary rows zeroed, into the adjoints of the three history
 // levels of v or omega_y, Hv or Hg and (v only) DY2 v. The adjoint of slot sn
 // itself becomes that of the level it held before the step (l = 2).
+// When the forward step solved v with the explicit inverses X (k_solve_inv),
+// Xv is non-null and the v solve here is the exact transpose of that product,
+// mu = X^T y, in natural row order; otherwise the transposed LU substitution.
 template <int NQ>
-__global__ __launch_bounds__(256) void k_solve_adj(Chan c, StepPtrs sp, const double *G, const int *P, const int4 *jobs, int njobs) {
+__global__ __launch_bounds__(256) void k_solve_adj(Chan c, StepPtrs sp, const double *G, const int *P, const double *Xv,
+                                                   const int4 *jobs, int njobs) {
+    __shared__ double sy[2][64 * NQ][8];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int job = blockIdx.x * 2 + (w >> 1), op = w & 1;
     const int N = c.NY;
@@ -947,13 +957,37 @@ __global__ __launch_bounds__(256) void k_solve_adj(Chan c, StepPtrs sp, const do
             t[2 * pp + 1][q] = v.y;
         }
     }
-    lu_substitute<NQ, 8>(Go, N, lane, t);
+    const bool inv = op == 0 && Xv;
+    if (inv) {
+        double(*y)[8] = sy[w >> 1];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) y[lane + 64 * q][k] = t[k][q];
+        wave_sync();
+        const double *Xo = Xv + (size_t)jb.x * N * N;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int r = lane + 64 * q < N ? lane + 64 * q : N - 1;
+            const double *col = Xo + (size_t)r * N;  // column r of X: row r of X^T
+            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int j = 0; j < N; ++j) {
+                const double x = col[j];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) acc[k] = fma(x, y[j][k], acc[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) t[k][q] = acc[k];
+        }
+    } else {
+        lu_substitute<NQ, 8>(Go, N, lane, t);
+    }
     const double AL[3] = {AL0, AL1, AL2}, BE[3] = {BE0, BE1, BE2};
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
         const int k = lane + 64 * q;
         if (k >= N) continue;
-        int r = Po[k];
+        int r = inv ? k : Po[k];
         if ((unsigned)r >= (unsigned)N) r = 0;
         const bool bc = op == 0 ? (r <= 1 || r >= N - 2) : (r == 0 || r == N - 1);
 #pragma unroll
@@ -1038,8 +1072,9 @@ void launch_invert(const Chan &c, const double *F, const int *P, int nfac, doubl
     if (chunk > 65535) chunk = 65535;
     for (int f0 = 0; f0 < nfac; f0 += chunk) {
         const int nf = nfac - f0 < chunk ? nfac - f0 : chunk;
-        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_invert<NQ>, dim3(nf, nch), dim3(64), 0, s, F + f0 * NN, P + (size_t)f0 * c.NY,
-                                                 c.NY, scratch));
+        const int nbi = nf * nch;
+        OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_invert<NQ>, dim3(8 * ((nbi + 7) / 8)), dim3(64), 0, s, F + f0 * NN,
+                                                 P + (size_t)f0 * c.NY, c.NY, nf, scratch));
         const int nb = nf * nrt;
         hipLaunchKernelGGL(k_refine_inverse<RCT>, dim3(8 * ((nb + 7) / 8)), dim3(256), 0, s, c, F + f0 * NN, P + (size_t)f0 * c.NY,
                            nf, scratch, X + f0 * NN);
@@ -1064,8 +1099,10 @@ void launch_factor_transpose(double *F, int nmat, int N, hipStream_t s) {
     }
 }
 
-void launch_solve_adj(const Chan &c, const StepPtrs &sp, const double *G, const int *P, const int4 *jobs, int njobs, hipStream_t s) {
-    OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_solve_adj<NQ>, dim3((njobs + 1) / 2), dim3(256), 0, s, c, sp, G, P, jobs, njobs));
+void launch_solve_adj(const Chan &c, const StepPtrs &sp, const double *G, const int *P, const double *Xv, const int4 *jobs,
+                      int njobs, hipStream_t s) {
+    OWT_NQ_DISPATCH(c.NY,
+                    hipLaunchKernelGGL(k_solve_adj<NQ>, dim3((njobs + 1) / 2), dim3(256), 0, s, c, sp, G, P, Xv, jobs, njobs));
 }
 
 void launch_pressure_solve(const Chan &c, const double *F, const int *P, const int4 *jobs, int njobs, const double2 *R,

@@ -37,7 +37,7 @@ EXPORTS = [
     "owt_time_steps", "owt_time_kernel", "owt_get_factor", "owt_save",
     "owt_ensemble_create", "owt_ensemble_destroy", "owt_ensemble_last_error", "owt_ensemble_member",
     "owt_ensemble_init_fields", "owt_ensemble_step", "owt_ensemble_time_steps",
-    "owt_tangent_init", "owt_tangent_handle", "owt_tangent_step", "owt_adjoint_run",
+    "owt_tangent_init", "owt_tangent_handle", "owt_tangent_step", "owt_adjoint_run", "owt_adjoint_apply",
     "owt_autotune", "owt_get_tune", "owt_set_tune", "owt_save_fields", "owt_statistics",
     "owt_f2p_noalloc", "owt_get_run_params",
 ]
@@ -117,6 +117,7 @@ def lib():
         L.owt_tangent_handle.restype = P
         L.owt_tangent_step.argtypes = [P, I]
         L.owt_adjoint_run.argtypes = [P, I, I, I, I, I, P, P, P, P]
+        L.owt_adjoint_apply.argtypes = [P, I, I, P, P, P, P, P, P]
         L.owt_ensemble_create.argtypes = [I, P, ctypes.POINTER(_Opts), PP]
         L.owt_ensemble_destroy.argtypes = [P]
         L.owt_ensemble_destroy.restype = None
@@ -342,6 +343,15 @@ class Channel:
         self._check(lib().owt_adjoint_run(self._h, int(K), int(ckpt_every), 0, int(first), int(last), ctypes.byref(J),
                                           *[_p(a) for a in g]))
         return J.value, g
+
+    def adjoint_apply(self, K, wu, wv, ww, ckpt_every=5):
+        """L^T w for the linear map L from the initial velocity to the velocity
+        after K steps (about this channel's trajectory from its initial field),
+        w = (wu, wv, ww): the gradients of <L d, w> with respect to d."""
+        ws = [np.ascontiguousarray(a, dtype=np.complex128).reshape(self.wave_shape) for a in (wu, wv, ww)]
+        g = [np.empty(self.wave_shape, dtype=np.complex128) for _ in range(3)]
+        self._check(lib().owt_adjoint_apply(self._h, int(K), int(ckpt_every), *[_p(a) for a in ws], *[_p(a) for a in g]))
+        return g
 
     def time_kernel(self, name, reps):
         ms = ctypes.c_double()

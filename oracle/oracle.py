@@ -55,6 +55,7 @@ def lib():
         L.orc_seeded_gauss.argtypes = [U64, I, I, I, I, P]
         L.orc_set_noise.argtypes = [P, U64, D, I]
         L.orc_set_mutation.argtypes = [P, I]
+        L.orc_set_factor_cache.argtypes = [P, I]
         L.orc_set_fft_variant.argtypes = [P, I]
         _lib = L
     return _lib
@@ -129,6 +130,12 @@ class Oracle:
     def set_mutation(self, bits):
         """Planted mistakes (MUT_*) that the oracle's pins must detect."""
         self._check(lib().orc_set_mutation(self._h, int(bits)))
+
+    def set_factor_cache(self, on=True):
+        """Factor Av and Ag once per distinct k^2 and reuse the factors: the same
+        fields bit for bit as factoring every pencil at every step, in a fraction
+        of the time at large sizes."""
+        self._check(lib().orc_set_factor_cache(self._h, int(bool(on))))
 
     def field(self, name):
         out = np.empty(self.wave_shape, dtype=np.complex128)

@@ -32,6 +32,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <random>
 #include <stdexcept>
@@ -119,17 +120,21 @@ void lu_solve(int n, const double *LU, const int *ipiv, double *b) {
 // Mat::solve(Vec<Complex>) (src/linalg.hpp:615-689): dgetrf on a copy, zgetrs
 // with the real factors promoted to complex; with a zero imaginary part in the
 // factors the complex solve is two independent real solves.
+void lu_solve_complex(int n, const double *LU, const int *ipiv, cd *x) {
+    std::vector<double> re(n), im(n);
+    for (int i = 0; i < n; ++i) { re[i] = x[i].real(); im[i] = x[i].imag(); }
+    lu_solve(n, LU, ipiv, re.data());
+    lu_solve(n, LU, ipiv, im.data());
+    for (int i = 0; i < n; ++i) x[i] = cd(re[i], im[i]);
+}
+
 void dense_solve_complex(const Mat &A, cd *x) {
     int n = A.n;
     std::vector<double> LU(A.a);
     std::vector<int> ipiv(n);
     if (lu_factor(n, LU.data(), ipiv.data()) != 0)
         throw std::runtime_error("LU factorization failed. Matrix A may be singular.");
-    std::vector<double> re(n), im(n);
-    for (int i = 0; i < n; ++i) { re[i] = x[i].real(); im[i] = x[i].imag(); }
-    lu_solve(n, LU.data(), ipiv.data(), re.data());
-    lu_solve(n, LU.data(), ipiv.data(), im.data());
-    for (int i = 0; i < n; ++i) x[i] = cd(re[i], im[i]);
+    lu_solve_complex(n, LU.data(), ipiv.data(), x);
 }
 
 // Mat::solve(Mat) (src/linalg.hpp:493-540): A^{-1} B column by column.
@@ -387,7 +392,20 @@ struct Oracle {
     //    not rotated (levels 1 and 2 keep the initial values), bit 32 the friction
     //    history not rotated.
     //  fft_variant: 1 transforms with the second factorization of fft_rec.
+    //  cache_factors: the Av and Ag factors of each distinct k^2 are computed
+    //    once and reused. The matrices depend on k^2 alone and the factorization
+    //    is deterministic, so the fields are bit for bit those of factoring
+    //    every pencil at every step (as the reference does); only the time
+    //    differs. For runs at sizes where the per-pencil factorizations would
+    //    take hours.
     int fft_variant = 0;
+    bool cache_factors = false;
+    struct PencilLU {
+        std::vector<double> v, g;
+        std::vector<int> pv, pg;
+    };
+    std::vector<double> lu_keys;       // distinct k^2, ascending
+    std::vector<PencilLU> lu_cache;    // factors of Av and Ag, by lu_keys index
     uint64_t noise_seed = 0;
     double noise_rel = 0.0;
     int noise_where = 0;
@@ -864,10 +882,63 @@ struct Oracle {
         return out;
     }
 
+    // Av and Ag of one pencil (src/solver.cpp:577-591, 601-603)
+    void assemble_operators(double k2, Mat &Av, Mat &Ag) const {
+        const double g0 = 11.0 / 6.0;
+        double k4 = std::pow(k2, 2);
+        double sa1 = g0 / dt + 2 * nu * k2, sa2 = nu * k4 + g0 * k2 / dt;
+        double sg1 = nu * k2 + g0 / dt;
+        for (size_t e = 0; e < Av.a.size(); ++e) {
+            Av.a[e] = (-nu) * DY4.a[e] + sa1 * DY2.a[e];
+            Ag.a[e] = (-nu) * DY2.a[e];
+        }
+        for (int j = 0; j < NY; ++j) { Av(j, j) = Av(j, j) - sa2; Ag(j, j) = Ag(j, j) + sg1; }
+        for (int j = 0; j < NY; ++j) {
+            Av(0, j) = 0.0; Av(NY - 1, j) = 0.0;
+            Av(1, j) = DY1(0, j); Av(NY - 2, j) = DY1(NY - 1, j);
+            Ag(0, j) = 0.0; Ag(NY - 1, j) = 0.0;
+        }
+        Av(0, 0) = 1.0; Av(NY - 1, NY - 1) = 1.0;
+        Ag(0, 0) = 1.0; Ag(NY - 1, NY - 1) = 1.0;
+    }
+
+    double pencil_k2(int k, int i) const { return std::pow(kx[i], 2) + std::pow(kz[k], 2); }
+
+    // cache_factors: factor Av and Ag of every distinct k^2 once
+    void build_factor_cache() {
+        std::vector<double> keys;
+        for (int k = 0; k < NZ; ++k)
+            for (int i = 0; i < NXH; ++i) keys.push_back(pencil_k2(k, i));
+        std::sort(keys.begin(), keys.end());
+        keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+        lu_cache.assign(keys.size(), PencilLU());
+        int errflag = 0;
+#pragma omp parallel
+        {
+            Mat Av(NY), Ag(NY);
+#pragma omp for schedule(dynamic, 4)
+            for (long e = 0; e < (long)keys.size(); ++e) {
+                assemble_operators(keys[e], Av, Ag);
+                PencilLU &L = lu_cache[e];
+                L.v = Av.a;
+                L.g = Ag.a;
+                L.pv.assign(NY, 0);
+                L.pg.assign(NY, 0);
+                if (lu_factor(NY, L.v.data(), L.pv.data()) != 0 || lu_factor(NY, L.g.data(), L.pg.data()) != 0) {
+#pragma omp atomic write
+                    errflag = 1;
+                }
+            }
+        }
+        if (errflag) throw std::runtime_error("LU factorization failed. Matrix A may be singular.");
+        lu_keys = std::move(keys);
+    }
+
     // src/solver.cpp:541-627
     void time_integration() {
-        const double g0 = 11.0 / 6.0, a0 = 3.0, a1 = -1.5, a2 = 1.0 / 3.0, b0 = 3.0, b1 = -3.0, b2 = 1.0;
+        const double a0 = 3.0, a1 = -1.5, a2 = 1.0 / 3.0, b0 = 3.0, b1 = -3.0, b2 = 1.0;
         int errflag = 0;
+        if (cache_factors && lu_keys.empty()) build_factor_cache();
 #pragma omp parallel
         {
             std::vector<cd> sv(NY), sg(NY), sa(NY), d2(NY);
@@ -875,8 +946,7 @@ struct Oracle {
 #pragma omp for collapse(2) schedule(dynamic, 4)
             for (int k = 0; k < NZ; ++k)
                 for (int i = 0; i < NXH; ++i) {
-                    double k2 = std::pow(kx[i], 2) + std::pow(kz[k], 2);
-                    double k4 = std::pow(k2, 2);
+                    double k2 = pencil_k2(k, i);
                     size_t o = at(k, i);
                     for (int j = 0; j < NY; ++j) {
                         sv[j] = (Hv_h[0][o + j] * b0 + Hv_h[1][o + j] * b1) + Hv_h[2][o + j] * b2;
@@ -890,25 +960,19 @@ struct Oracle {
                         cd va = (vory_h[0][o + j] * a0 + vory_h[1][o + j] * a1) + vory_h[2][o + j] * a2;
                         sg[j] = hg + cd(va.real() / dt, va.imag() / dt);
                     }
-                    double sa1 = g0 / dt + 2 * nu * k2, sa2 = nu * k4 + g0 * k2 / dt;
-                    double sg1 = nu * k2 + g0 / dt;
-                    for (size_t e = 0; e < Av.a.size(); ++e) {
-                        Av.a[e] = (-nu) * DY4.a[e] + sa1 * DY2.a[e];
-                        Ag.a[e] = (-nu) * DY2.a[e];
-                    }
-                    for (int j = 0; j < NY; ++j) { Av(j, j) = Av(j, j) - sa2; Ag(j, j) = Ag(j, j) + sg1; }
-                    for (int j = 0; j < NY; ++j) {
-                        Av(0, j) = 0.0; Av(NY - 1, j) = 0.0;
-                        Av(1, j) = DY1(0, j); Av(NY - 2, j) = DY1(NY - 1, j);
-                        Ag(0, j) = 0.0; Ag(NY - 1, j) = 0.0;
-                    }
-                    Av(0, 0) = 1.0; Av(NY - 1, NY - 1) = 1.0;
-                    Ag(0, 0) = 1.0; Ag(NY - 1, NY - 1) = 1.0;
                     sv[0] = sv[1] = sv[NY - 2] = sv[NY - 1] = 0.0;
                     sg[0] = sg[NY - 1] = 0.0;
                     try {
-                        dense_solve_complex(Av, sv.data());
-                        dense_solve_complex(Ag, sg.data());
+                        if (cache_factors) {
+                            const size_t e = std::lower_bound(lu_keys.begin(), lu_keys.end(), k2) - lu_keys.begin();
+                            const PencilLU &L = lu_cache[e];
+                            lu_solve_complex(NY, L.v.data(), L.pv.data(), sv.data());
+                            lu_solve_complex(NY, L.g.data(), L.pg.data(), sg.data());
+                        } else {
+                            assemble_operators(k2, Av, Ag);
+                            dense_solve_complex(Av, sv.data());
+                            dense_solve_complex(Ag, sg.data());
+                        }
                     } catch (...) {
 #pragma omp atomic write
                         errflag = 1;
@@ -1135,6 +1199,14 @@ int orc_set_fft_variant(void *h, int variant) {
 }
 int orc_set_mutation(void *h, int bits) {
     ((Oracle *)h)->mutate = bits;
+    return 0;
+}
+
+int orc_set_factor_cache(void *h, int on) {
+    Oracle *o = (Oracle *)h;
+    o->cache_factors = on != 0;
+    o->lu_keys.clear();
+    o->lu_cache.clear();
     return 0;
 }
 

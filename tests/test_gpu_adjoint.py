@@ -146,3 +146,78 @@ def test_torch_autograd_function_harness(base):
     fd = float(Jp - Jm) / (2 * eps)
     print(f"[torch] dJ={dJ:.15e} fd={fd:.15e}", flush=True)
     assert abs(dJ - fd) <= 1e-6 * abs(fd)
+
+
+def _dot(a, b):
+    return sum(float(np.vdot(x, y).real) for x, y in zip(a, b))
+
+
+@pytest.mark.parametrize("shape,K,ckpt", [((16, 33, 16), 6, 2), ((128, 129, 128), 20, 5)])
+def test_adjoint_apply_dot_product_identity(owt_mod, shape, K, ckpt):
+    """<L d, w> = <d, L^T w> for the map L from the initial velocity to the
+    velocity after K steps, with white-noise d (through owt_tangent_init) and
+    white-noise w (through owt_adjoint_apply): every mode of every field."""
+    print(f"[gpu] adjoint apply {shape} K={K}", flush=True)
+    ch = owt_mod.Channel(params=params(*shape))
+    ch.init_fields(2, seed=0, amp=0.15)
+    x0 = [ch.field(n) for n in ("u", "v", "w")]
+    d, w = directions(x0[0].shape, [41, 42])
+    t = ch.tangent_init(*d)
+    ch.step(K)
+    Ld = [t.field(n) for n in ("u", "v", "w")]
+    ch.set_velocity(*x0)
+    LTw = ch.adjoint_apply(K, *w, ckpt_every=ckpt)
+    assert ch.steps_done() == K
+    a, b = _dot(Ld, w), _dot(d, LTw)
+    scale = sum(float(np.sum(np.abs(x.real * y.real) + np.abs(x.imag * y.imag))) for x, y in zip(Ld, w))
+    print(f"[adjoint] <Ld,w>={a:.15e} <d,LTw>={b:.15e} rel {abs(a - b) / abs(a):.2e} "
+          f"(terms {abs(a - b) / scale:.2e})", flush=True)
+    assert abs(a - b) <= 1e-12 * abs(a)
+    ch.close()
+
+
+def test_adjoint_config_re180_128_K20(owt_mod):
+    """The adjoint workload at full size: Re_tau 180, 128x129x128, K = 20 steps,
+    checkpoints every 5, J = mean friction force over steps 11..20. The adjoint
+    identity against the tangent on two white-noise directions, and central
+    finite differences (step 1e-6) on two smooth directions of the state's own
+    scale: an O(1) mean-flow profile plus the seeded divergence-free
+    perturbation (along white noise J is not linear enough at that step, and
+    along divergence-free perturbations alone dJ is of second order)."""
+    K, ckpt, first, last, eps = 20, 5, 11, 20, 1e-6
+    p = dict(nu=1 / 2800, dt=1e-3, dkx=0.5, dkz=1.5, alpha_mesh=1.5, NX=128, NY=129, NZ=128, precision=0, method=1)
+    print("[gpu] adjoint 128x129x128 K=20", flush=True)
+    ch = owt_mod.Channel(params=p)
+    NY = ch.NY
+    s = 2.0 * np.arange(NY) / (NY - 1) - 1.0
+    y = np.tanh(p["alpha_mesh"] * s) / np.tanh(p["alpha_mesh"])
+    ch.init_fields(2, seed=0, amp=0.15)
+    x0 = [ch.field(n) for n in ("u", "v", "w")]
+    ch.set_velocity(*x0)
+    J, g = ch.adjoint_run(K, ckpt, first, last)
+
+    def J_of(x):
+        ch.set_velocity(*x)
+        ch.step(K)
+        return ch.diag()[first:last + 1, 0].mean()
+
+    assert abs(J - J_of(x0)) <= 1e-14 * abs(J)
+    for d in directions(x0[0].shape, [1000, 1001]):
+        ch.set_velocity(*x0)
+        t = ch.tangent_init(*d)
+        ch.step(K)
+        dJ_t = float(t.diag()[first:last + 1, 0].mean())
+        dJ_a = _dot(g, d)
+        print(f"[adjoint 128] identity <g,d>={dJ_a:.15e} tangent={dJ_t:.15e} rel {abs(dJ_a - dJ_t) / abs(dJ_t):.2e}",
+              flush=True)
+        assert abs(dJ_a - dJ_t) <= 1e-12 * abs(dJ_t)
+    for sd in (1000, 1001):
+        ch.init_fields(2, seed=sd, amp=0.15)
+        d = [ch.field(n) for n in ("u", "v", "w")]
+        c = np.random.default_rng(sd).standard_normal(4)
+        d[0][0, 0] = (1 - y ** 2) * (c[0] + c[1] * y + c[2] * y ** 2 + c[3] * y ** 3)
+        dJ_a = _dot(g, d)
+        fd = (J_of([a + eps * b for a, b in zip(x0, d)]) - J_of([a - eps * b for a, b in zip(x0, d)])) / (2 * eps)
+        print(f"[adjoint 128] fd <g,d>={dJ_a:.15e} fd={fd:.15e} rel {abs(dJ_a - fd) / abs(fd):.2e}", flush=True)
+        assert abs(dJ_a - fd) <= 1e-6 * abs(fd)
+    ch.close()

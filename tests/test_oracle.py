@@ -628,3 +628,19 @@ def test_oblique_orr_sommerfeld_squire_mode(oracle_mod, mut):
     ok = abs(ratio - 1) < 1e-5 and abs(lm.real - lam.real) < 2e-5 * abs(lam.real) and \
         abs(lm.imag - lam.imag) < 2e-6 * abs(lam.imag)
     assert ok == (mut == 0), (lm, lam, ratio)
+
+
+@pytest.mark.parametrize("NX,NY,NZ", [(16, 33, 16), (8, 160, 8)])
+def test_factor_cache_is_bit_identical(oracle_mod, NX, NY, NZ):
+    """The oracle's factor cache (one factorization per distinct k^2, used for
+    the full-size reference run) gives the fields and friction series of
+    factoring every pencil at every step, bit for bit."""
+    outs = []
+    for cache in (False, True):
+        o = make(oracle_mod, NX, NY, NZ)
+        o.set_factor_cache(cache)
+        o.init(2, seed=3, amp=0.15)
+        o.step(4)
+        outs.append({n: o.field(n) for n in ("u", "v", "w", "omy", "Hv", "Hg")} | {"diag": o.diag()})
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
