@@ -167,13 +167,16 @@ def accuracy_mixed_vs_fp64(owt, cfg, amp, steps):
 
 
 def ensemble_throughput(owt, name, steps):
-    """An ensemble workload stepped together (one graph per step, members as
-    independent branches) against the same members stepped one after another
-    (each channel's own graph step, summed over the members)."""
+    """An ensemble workload stepped together (one graph per step: one launch per
+    stage over the members of each distinct shape, the shapes as parallel
+    branches) against the same members stepped one after another (each channel's
+    own graph step, summed over the members). Both are autotuned first."""
     ps = owt.ENSEMBLES[name]
     with owt.Ensemble(ps, method=owt.BATCHED, precision=owt.FP64) as ens:
         ens.init_fields(2, seeds=range(len(ps)), amp=0.15)
+        ens.autotune(5)
         G = ens.gridpoints
+        launches = ens.launches_per_step()
         t = ens.time_steps(steps) / steps
     seq = 0.0
     shapes = {}
@@ -184,10 +187,12 @@ def ensemble_throughput(owt, name, steps):
         p = next(q for q in ps if (q["NX"], q["NY"], q["NZ"]) == (NX, NY, NZ))
         with owt.Channel(params=p, method=owt.BATCHED, precision=owt.FP64) as ch:
             ch.init_fields(2, seed=0, amp=0.15)
+            ch.autotune(5)
             seq += count * ch.time_steps(steps) / steps
     return {"workload": name, "members": len(ps),
             "shapes": {f"{a}x{b}x{c}": n for (a, b, c), n in shapes.items()},
-            "gridpoints": G, "precision": "fp64", "steps": steps, "ms_per_step": round(t, 4),
+            "gridpoints": G, "precision": "fp64", "steps": steps, "launches_per_step": launches,
+            "ms_per_step": round(t, 4),
             "ns_per_gridpoint_step": round(t * 1e6 / G, 5),
             "members_one_by_one_ms_per_step": round(seq, 4), "speedup_vs_one_by_one": round(seq / t, 2)}
 

@@ -227,8 +227,13 @@ int owt_adjoint_apply(owt_handle *h, int K, int ckpt_every, const double *wu, co
 
 /* Ensembles: n independent channels on one GPU (sizes and parameters may
  * differ), stepped together. No reference counterpart (the reference runs one
- * channel per job). With the batched method every member's step is captured
- * into one graph per step, the members as independent branches. Member m is a
+ * channel per job). With the batched method each stage of the step is one
+ * kernel launch over all members that run the same kernel instance (same
+ * rows-per-lane class, transform length class, precision and tuned shape),
+ * each block finding its member in a device table; members with equal
+ * parameters share one copy of the operators. One graph per step holds these
+ * launches, the members of each distinct shape as a branch of their own, so
+ * their number grows with the distinct shapes, not with the member count. Member m is a
  * borrowed owt_handle for every single-channel call (fields, diagnostics,
  * checkpoint, save); owt_destroy on it does nothing, it lives until
  * owt_ensemble_destroy. A member stepped on its own leaves the ensemble's step
@@ -245,6 +250,13 @@ int owt_ensemble_init_fields(owt_ensemble *e, int mode, const uint64_t *seeds, d
 int owt_ensemble_step(owt_ensemble *e, int nsteps);
 /* nsteps ensemble steps between two HIP events; elapsed milliseconds */
 int owt_ensemble_time_steps(owt_ensemble *e, int nsteps, double *ms);
+/* kernel launches in one grouped ensemble step (*n = 0 when the members are
+ * stepped one by one: another method, or ring phases that differ) */
+int owt_ensemble_launches(owt_ensemble *e, int *n);
+/* launch-shape autotuner over the ensemble: the first member of each distinct
+ * parameter set is tuned as by owt_autotune (reps launches per candidate) and
+ * the members with the same parameters take its choices */
+int owt_ensemble_autotune(owt_ensemble *e, int reps);
 
 /* timing: enqueue nsteps steps on the handle's stream between two HIP events
  * and return the elapsed milliseconds (inputs stay resident on the device) */

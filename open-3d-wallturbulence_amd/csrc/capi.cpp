@@ -263,6 +263,15 @@ int owt_ensemble_time_steps(owt_ensemble *e, int nsteps, double *ms) {
     return eguard(e, [&] { *ms = e->e->time_steps(nsteps); });
 }
 
+int owt_ensemble_autotune(owt_ensemble *e, int reps) {
+    return eguard(e, [&] { e->e->autotune(reps); });
+}
+
+int owt_ensemble_launches(owt_ensemble *e, int *n) {
+    if (!n) return OWT_E_ARG;
+    return eguard(e, [&] { *n = e->e->launches_per_step(); });
+}
+
 const char *owt_last_error(const owt_handle *h) {
     if (h) return h->err.c_str();
     std::lock_guard<std::mutex> l(g_mu);

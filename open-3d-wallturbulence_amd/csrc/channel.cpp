@@ -67,6 +67,9 @@ Channel::Channel(const Params &p, int device, bool verbose, hipStream_t stream) 
     } else {
         OWT_HIP(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking));
     }
+    OWT_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    OWT_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    OWT_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
     noalloc_ = noalloc_scratch(device);
     op_ = build_operators((int)P_.NY, P_.alpha_mesh);
     try {
@@ -88,8 +91,14 @@ Channel::~Channel() {
 
 void Channel::free_all() {
     if (s_) (void)hipStreamSynchronize(s_);
+    if (side_) (void)hipStreamSynchronize(side_);
     for (void *p : allocs_) (void)hipFree(p);
     allocs_.clear();
+    if (ev_fork_) (void)hipEventDestroy(ev_fork_);
+    if (ev_join_) (void)hipEventDestroy(ev_join_);
+    if (side_) (void)hipStreamDestroy(side_);
+    ev_fork_ = ev_join_ = nullptr;
+    side_ = nullptr;
     if (s_ && own_stream_) (void)hipStreamDestroy(s_);
     s_ = nullptr;
 }
@@ -654,15 +663,29 @@ void Channel::enqueue_step_body(int ring) {
     const double *F = plain ? Fp_ : Fb_;
     const int *P = plain ? Pp_ : Pb_;
     if (plain) factor_set(Fp_, Pp_, c_.npencil, nullptr, 0, 2);
+    // The mean flow (one block: pencil 0's u, w, omega_x, omega_z, the mean
+    // histories and the friction) reads and writes nothing the v and omega_y
+    // solves or the horizontal stage touch: it runs on a side stream forked from
+    // the step's stream (a parallel branch of the captured graph), so that its
+    // serial chain hides behind the v solves. The omega_y solves stay on the
+    // step's stream: beside the v solves, whose one-wave blocks take every CU
+    // slot that frees, their 512-thread blocks were starved (0.57 ms alone,
+    // 2.4 ms beside them).
     if (!plain && bops_ == 1) {
+        OWT_HIP(hipEventRecord(ev_fork_, s_));
+        OWT_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
+        launch_mean(c_, sp, Fmean_, Pmean_, side_);
+        OWT_HIP(hipEventRecord(ev_join_, side_));
         launch_solve_inv(c_, sp, Xb_, jobs_, njobs_, s_);
         launch_helmholtz_g(c_, sp, s_);
+        OWT_HIP(hipStreamWaitEvent(s_, ev_join_, 0));
+        launch_horizontal(c_, sp, plain, s_);
     } else {
         launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, plain ? 2 : bops_, s_);
+        launch_horizontal(c_, sp, plain, s_);
+        // plain: pencil 0's own factors hold Ag(0) as its second operator
+        launch_mean(c_, sp, plain ? Fp_ + (size_t)c_.NY * c_.NY : Fmean_, plain ? Pp_ + c_.NY : Pmean_, s_);
     }
-    launch_horizontal(c_, sp, plain, s_);
-    // plain: pencil 0's own factors hold Ag(0) as its second operator
-    launch_mean(c_, sp, plain ? Fp_ + (size_t)c_.NY * c_.NY : Fmean_, plain ? Pp_ + c_.NY : Pmean_, s_);
     // the six transformed fields: u, v, w, omega_x, omega_y, omega_z of the new level
     const size_t W = c_.W;
     const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
@@ -671,6 +694,21 @@ void Channel::enqueue_step_body(int ring) {
     else launch_x_lamb(c_, s_);
     launch_z_forward(c_, s_);
     launch_nonlinear(c_, sp, plain, s_);
+}
+
+EnsEntry Channel::ens_entry(int ring) const {
+    EnsEntry e{};
+    e.c = c_;
+    e.sp = step_ptrs(ring);
+    const size_t W = c_.W;
+    const double2 *six[6] = {c_.cur, e.sp.vn, c_.cur + W, c_.cur + 2 * W, e.sp.gn, c_.cur + 3 * W};
+    for (int f = 0; f < 6; ++f) e.six[f] = six[f];
+    e.X = Xb_;
+    e.jobs = jobs_;
+    e.njobs = njobs_;
+    e.Fg = Fmean_;
+    e.Pg = Pmean_;
+    return e;
 }
 
 void Channel::build_graph(int ring) {

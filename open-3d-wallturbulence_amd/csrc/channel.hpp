@@ -88,6 +88,11 @@ class Channel {
     }
     unsigned generation() const { return generation_; }
     void finish();
+    // grouped launches (Ensemble): whether this channel's step is the batched
+    // one with the omega_y eigendecomposition and nothing attached (tangent),
+    // and its member entry at ring phase `ring` (after prepare_step)
+    bool ens_ready() const { return P_.method == 1 && bops_ == 1 && factored_ && !tangent_ && !tangent_base_; }
+    EnsEntry ens_entry(int ring) const;
 
     // Tangent (forward-mode linearization, FP64). tangent_init attaches a tangent
     // state started from the direction (du, dv, dw) through the linearized
@@ -148,6 +153,10 @@ class Channel {
     double2 *noalloc_ = nullptr;  // this device's scratch array (noalloc.hip)
     bool verbose_ = false;
     hipStream_t s_ = nullptr;
+    // side stream and events of the step's parallel branch (omega_y solves and
+    // mean flow beside the v solves)
+    hipStream_t side_ = nullptr;
+    hipEvent_t ev_fork_ = nullptr, ev_join_ = nullptr;
     std::vector<void *> allocs_;
     // factor sets (lu_kernels.hip): batched, one per +-kz system, computed once;
     // plain, one per pencil, recomputed every step

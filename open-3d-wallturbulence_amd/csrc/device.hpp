@@ -97,6 +97,51 @@ struct StepPtrs {
     int s[3], sn;
 };
 
+// ---- ensembles: one launch of a step stage over several channels ----
+// A member of a grouped launch: its channel (the operator pointers may be those
+// of an earlier member with the same parameters, whose operators are the same
+// bit for bit), its history slots at the ensemble's ring phase, and the
+// operands the single-channel launches take as arguments.
+struct EnsEntry {
+    Chan c;
+    StepPtrs sp;
+    const double2 *six[6];  // z-inverse inputs: u, v, w, omega_x, omega_y, omega_z of the new level
+    const double *X;        // explicit inverses of Av (v solves)
+    const int4 *jobs;
+    int njobs;
+    const double *Fg;  // LU factor of Ag(0) and its pivot rows (mean flow)
+    const int *Pg;
+};
+// member of block b of a grouped launch: the largest m < nm with first[m] <= b
+__device__ __forceinline__ int ens_member(const int *__restrict__ first, int nm, int b) {
+    int lo = 0, hi = nm - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+// the step's stages in order
+enum EnsStage { ES_SOLVE = 0, ES_HELMHOLTZ, ES_MEAN, ES_HORIZONTAL, ES_Z_INVERSE, ES_X_LAMB, ES_Z_FORWARD, ES_NONLINEAR, ES_COUNT };
+// The kernel instance a member runs in a stage (members with equal keys share a
+// launch) and the blocks it takes there: a multiple of 8, so that each member's
+// blocks start on the same XCD as the grid and the per-XCD tile numbering of the
+// transforms and solves holds inside every member.
+int ens_key(const EnsEntry &e, int stage);
+int ens_blocks(const EnsEntry &e, int stage);
+// one launch of `stage` over nm members of one key: ents and first are device
+// arrays (first[m] the first block of member m), grid the sum of their blocks,
+// rep a host copy of one of the entries, ny_max the largest NY among them
+void launch_ens(int stage, const EnsEntry *ents, const int *first, int nm, int grid, const EnsEntry &rep, int ny_max,
+                hipStream_t s);
+// the parts of the above in fft_kernels.hip (z inverse, x phase, z forward) and
+// lu_kernels.hip (v solves)
+int ens_key_fft(const EnsEntry &e, int stage, int *blocks);
+void launch_ens_fft(int stage, const EnsEntry *ents, const int *first, int nm, int grid, const EnsEntry &rep, hipStream_t s);
+int ens_key_solve(const EnsEntry &e, int *blocks);
+void launch_ens_solve(const EnsEntry *ents, const int *first, int nm, int grid, const EnsEntry &rep, hipStream_t s);
+
 // ---- scratch of the allocation-free entry points (noalloc.hip) ----
 // device address of the file-scope scratch array on `device`, looked up on the
 // first call for that device (which must be current); and its size in bytes

@@ -173,11 +173,12 @@ __device__ __forceinline__ const float2 *tw_of<float2>(const Chan &c, bool xdir)
 // XCD b % 8, each XCD with its own L2), so tiles are numbered to give every XCD
 // a contiguous range: the y-tiles of one line set, which share cache lines,
 // run side by side in one L2. Padding blocks get -1.
-__device__ __forceinline__ int xcd_tile(int ntiles) {
-    const int b = blockIdx.x, per = (ntiles + 7) / 8;
+__device__ __forceinline__ int xcd_tile_of(int ntiles, int b) {
+    const int per = (ntiles + 7) / 8;
     const int t = (b % 8) * per + b / 8;
     return t < ntiles ? t : -1;
 }
+__device__ __forceinline__ int xcd_tile(int ntiles) { return xcd_tile_of(ntiles, blockIdx.x); }
 
 static inline int xcd_grid(int ntiles) { return 8 * ((ntiles + 7) / 8); }
 
@@ -193,14 +194,15 @@ struct PtrTable {
 // NZH into the Nyquist slot NZ (0 in the step, which drops it): with div = NZ2
 // and nyq = 1 the kernel is the transpose of k_z_forward.
 template <typename T, int TJ, int TL, int LMAX>
-__global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typename CT<T>::t *out, int nf, T div, int nyq) {
+__device__ __forceinline__ void z_inverse_body(const Chan &c, const double2 *const *in, typename CT<T>::t *out, int nf, T div, int nyq,
+                                               const int bid) {
     using C = typename CT<T>::t;
     extern __shared__ unsigned char smem[];
     C *tw = reinterpret_cast<C *>(smem);
     C *buf = tw + LMAX;
     const int L = c.NZ2, NZ = c.NZ, NZH = c.NZH, NXH = c.NXH, NY = c.NY;
     const int nyt = (NY + TJ - 1) / TJ;
-    const int tile = xcd_tile(nyt * NXH * nf);
+    const int tile = xcd_tile_of(nyt * NXH * nf, bid);
     if (tile < 0) return;
     const int i = (tile / nyt) % NXH, f = tile / (nyt * NXH);
     const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typen
     const bool valid = j < NY;
     const C *twg = tw_of<C>(c, false);
     for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = twg[n];
-    const double2 *src = in.p[f];
+    const double2 *src = in[f];
     const bool scale = div != (T)1;  // x / 1 == x: the step's call skips the divisions
     for (int n = tl; n < L; n += TL) {
         // slot n <- mode n (n < NZH); slots NZH..NZ are padding and the dropped z-Nyquist
@@ -223,6 +225,17 @@ __global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typen
     if (valid)
         for (int n = tl; n < L; n += TL) dst[((size_t)n * NXH + i) * NY + j] = buf[n * TJ + jj];
 }
+template <typename T, int TJ, int TL, int LMAX>
+__global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typename CT<T>::t *out, int nf, T div, int nyq) {
+    z_inverse_body<T, TJ, TL, LMAX>(c, in.p, out, nf, div, nyq, blockIdx.x);
+}
+template <typename T, int TJ, int TL, int LMAX>
+__global__ __launch_bounds__(TJ *TL) void k_z_inverse_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    z_inverse_body<T, TJ, TL, LMAX>(e.c, e.six, (typename CT<T>::t *)e.c.zbuf, 6, (T)1, 0, bid);
+}
 
 // ---------------------------------------------------------------------------
 // z forward (src/FFT.cpp:171-190): per (i, y tile, field)
@@ -231,14 +244,15 @@ __global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typen
 // to zero (1 in the step): with div = 1 and nyq = 0 the kernel is the transpose
 // of k_z_inverse.
 template <typename T, int TJ, int TL, int LMAX, typename OutC>
-__global__ __launch_bounds__(TJ *TL) void k_z_forward(Chan c, const typename CT<T>::t *in, OutC *out, int nf, T div, int nyq) {
+__device__ __forceinline__ void z_forward_body(const Chan &c, const typename CT<T>::t *in, OutC *out, int nf, T div, int nyq,
+                                               const int bid) {
     using C = typename CT<T>::t;
     extern __shared__ unsigned char smem[];
     C *tw = reinterpret_cast<C *>(smem);
     C *buf = tw + LMAX;
     const int L = c.NZ2, NZ = c.NZ, NZH = c.NZH, NXH = c.NXH, NY = c.NY;
     const int nyt = (NY + TJ - 1) / TJ;
-    const int tile = xcd_tile(nyt * NXH * nf);
+    const int tile = xcd_tile_of(nyt * NXH * nf, bid);
     if (tile < 0) return;
     const int i = (tile / nyt) % NXH, f = tile / (nyt * NXH);
     const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
@@ -262,6 +276,17 @@ __global__ __launch_bounds__(TJ *TL) void k_z_forward(Chan c, const typename CT<
             o.y = v.y / div;
             dst[((size_t)k * NXH + i) * NY + j] = o;
         }
+}
+template <typename T, int TJ, int TL, int LMAX, typename OutC>
+__global__ __launch_bounds__(TJ *TL) void k_z_forward(Chan c, const typename CT<T>::t *in, OutC *out, int nf, T div, int nyq) {
+    z_forward_body<T, TJ, TL, LMAX, OutC>(c, in, out, nf, div, nyq, blockIdx.x);
+}
+template <typename T, int TJ, int TL, int LMAX, typename OutC>
+__global__ __launch_bounds__(TJ *TL) void k_z_forward_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    z_forward_body<T, TJ, TL, LMAX, OutC>(e.c, (const typename CT<T>::t *)e.c.xbuf, (OutC *)e.c.lmb, 3, (T)e.c.NZ2, 1, bid);
 }
 
 // ---------------------------------------------------------------------------
@@ -309,7 +334,7 @@ __device__ __forceinline__ void place_pair_spectrum(C *buf, const PairRegs<C, MP
 }
 
 template <typename T, int TJ, int TL, int LMAX>
-__global__ __launch_bounds__(TJ *TL) void k_x_lamb(Chan c, const typename CT<T>::t *in, typename CT<T>::t *out) {
+__device__ __forceinline__ void x_lamb_body(const Chan &c, const typename CT<T>::t *in, typename CT<T>::t *out, const int bid) {
     using C = typename CT<T>::t;
     constexpr int EPT = (LMAX + TL - 1) / TL;
     extern __shared__ unsigned char smem[];
@@ -317,7 +342,7 @@ __global__ __launch_bounds__(TJ *TL) void k_x_lamb(Chan c, const typename CT<T>:
     C *buf = tw + LMAX;
     const int L = c.NX2, NXH = c.NXH, NY = c.NY;
     const int nyt = (NY + TJ - 1) / TJ;
-    const int tile = xcd_tile(nyt * c.NZ2);
+    const int tile = xcd_tile_of(nyt * c.NZ2, bid);
     if (tile < 0) return;
     const int k2 = tile / nyt;
     const int jj = threadIdx.x % TJ, tl = threadIdx.x / TJ;
@@ -384,6 +409,17 @@ __global__ __launch_bounds__(TJ *TL) void k_x_lamb(Chan c, const typename CT<T>:
         }
         __syncthreads();
     }
+}
+template <typename T, int TJ, int TL, int LMAX>
+__global__ __launch_bounds__(TJ *TL) void k_x_lamb(Chan c, const typename CT<T>::t *in, typename CT<T>::t *out) {
+    x_lamb_body<T, TJ, TL, LMAX>(c, in, out, blockIdx.x);
+}
+template <typename T, int TJ, int TL, int LMAX>
+__global__ __launch_bounds__(TJ *TL) void k_x_lamb_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    x_lamb_body<T, TJ, TL, LMAX>(e.c, (const typename CT<T>::t *)e.c.zbuf, (typename CT<T>::t *)e.c.xbuf, bid);
 }
 
 // ---------------------------------------------------------------------------
@@ -871,6 +907,48 @@ void launch_x_r2c(const Chan &c, const double *phys, int nf, void *out, hipStrea
 
 void launch_peak3(const double *phys, size_t n, unsigned long long *out, hipStream_t s) {
     hipLaunchKernelGGL(k_peak3, dim3(1024), dim3(256), 0, s, phys, n, out);
+}
+
+// ---- grouped launches of the transform stages over ensemble members ----
+// The instance is fixed by the precision, the length class (LMAX) and the tuned
+// y-run TJ (every candidate has 256 threads); the blocks are the single launch's.
+int ens_key_fft(const EnsEntry &e, int stage, int *blocks) {
+    const Chan &c = e.c;
+    const int L = stage == ES_X_LAMB ? c.NX2 : c.NZ2;
+    const int sel = c.tune[stage == ES_Z_INVERSE ? T_Z_INVERSE_TILE : stage == ES_X_LAMB ? T_X_TILE : T_Z_FORWARD_TILE];
+    int key = 0, tj = 1;
+    if (c.precision == 0) OWT_TUNED_DISPATCH(L, sel, { key = LMAX * 64 + TJ; tj = TJ; });
+    else OWT_TUNED_DISPATCH_F32(L, sel, { key = (1 << 20) + LMAX * 64 + TJ; tj = TJ; });
+    const int nyt = (c.NY + tj - 1) / tj;
+    *blocks = xcd_grid(stage == ES_X_LAMB ? c.NZ2 * nyt : c.NXH * nyt * (stage == ES_Z_INVERSE ? 6 : 3));
+    return key;
+}
+
+void launch_ens_fft(int stage, const EnsEntry *ents, const int *first, int nm, int grid, const EnsEntry &rep, hipStream_t s) {
+    const Chan &c = rep.c;
+    const int L = stage == ES_X_LAMB ? c.NX2 : c.NZ2;
+    const int sel = c.tune[stage == ES_Z_INVERSE ? T_Z_INVERSE_TILE : stage == ES_X_LAMB ? T_X_TILE : T_Z_FORWARD_TILE];
+    const dim3 g(grid);
+    if (c.precision == 0)
+        OWT_TUNED_DISPATCH(L, sel, {
+            const size_t lds = (smem_bytes<double, TJ, TL, LMAX>());
+            if (stage == ES_Z_INVERSE)
+                hipLaunchKernelGGL((k_z_inverse_ens<double, TJ, TL, LMAX>), g, dim3(TJ * TL), lds, s, ents, first, nm);
+            else if (stage == ES_X_LAMB)
+                hipLaunchKernelGGL((k_x_lamb_ens<double, TJ, TL, LMAX>), g, dim3(TJ * TL), lds, s, ents, first, nm);
+            else
+                hipLaunchKernelGGL((k_z_forward_ens<double, TJ, TL, LMAX, double2>), g, dim3(TJ * TL), lds, s, ents, first, nm);
+        });
+    else
+        OWT_TUNED_DISPATCH_F32(L, sel, {
+            const size_t lds = (smem_bytes<float, TJ, TL, LMAX>());
+            if (stage == ES_Z_INVERSE)
+                hipLaunchKernelGGL((k_z_inverse_ens<float, TJ, TL, LMAX>), g, dim3(TJ * TL), lds, s, ents, first, nm);
+            else if (stage == ES_X_LAMB)
+                hipLaunchKernelGGL((k_x_lamb_ens<float, TJ, TL, LMAX>), g, dim3(TJ * TL), lds, s, ents, first, nm);
+            else
+                hipLaunchKernelGGL((k_z_forward_ens<float, TJ, TL, LMAX, float2>), g, dim3(TJ * TL), lds, s, ents, first, nm);
+        });
 }
 
 }  // namespace owt

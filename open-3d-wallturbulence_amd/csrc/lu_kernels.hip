@@ -762,11 +762,12 @@ __global__ __launch_bounds__(256) void k_refine_inverse(Chan c, const double *F,
 // x_row = sum_j X(row, j) b_j: lane l accumulates rows l + 64q, the columns of X
 // read coalesced and eight at a time ahead of their use.
 template <int NQ, int JPB>
-__global__ __launch_bounds__(64 * JPB) void k_solve_inv(Chan c, StepPtrs sp, const double *X, const int4 *jobs, int njobs) {
+__device__ __forceinline__ void solve_inv_body(const Chan &c, const StepPtrs &sp, const double *X, const int4 *jobs, int njobs,
+                                               const int bid) {
     __shared__ double sb[JPB][64 * NQ][8];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nb = (njobs + JPB - 1) / JPB, per = (nb + 7) / 8;
-    const int blk = (blockIdx.x % 8) * per + blockIdx.x / 8;  // XCD-contiguous jobs
+    const int blk = (bid % 8) * per + bid / 8;  // XCD-contiguous jobs
     if (blk >= nb) return;
     const int job = blk * JPB + w;
     const int N = c.NY;
@@ -848,6 +849,17 @@ __global__ __launch_bounds__(64 * JPB) void k_solve_inv(Chan c, StepPtrs sp, con
         for (int pp = 0; pp < 4; ++pp)
             if (pc[pp] >= 0) XN[(size_t)pc[pp] * N + row] = c2(acc[2 * pp][q], acc[2 * pp + 1][q]);
     }
+}
+template <int NQ, int JPB>
+__global__ __launch_bounds__(64 * JPB) void k_solve_inv(Chan c, StepPtrs sp, const double *X, const int4 *jobs, int njobs) {
+    solve_inv_body<NQ, JPB>(c, sp, X, jobs, njobs, blockIdx.x);
+}
+template <int NQ, int JPB>
+__global__ __launch_bounds__(64 * JPB) void k_solve_inv_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    solve_inv_body<NQ, JPB>(e.c, e.sp, e.X, e.jobs, e.njobs, bid);
 }
 
 // ---------------------------------------------------------------------------
@@ -1081,15 +1093,35 @@ void launch_invert(const Chan &c, const double *F, const int *P, int nfac, doubl
     }
 }
 
-void launch_solve_inv(const Chan &c, const StepPtrs &sp, const double *X, const int4 *jobs, int njobs, hipStream_t s) {
-    // tuned jobs (waves) per block: candidates 2, 1, 4
+// tuned jobs (waves) per block of the v solves: candidates 2, 1, 4
+static int solve_jobs_per_block(const Chan &c) {
     const int sel = c.tune[T_SOLVE_JOBS];
-    const int jpb = sel == 1 ? 1 : sel == 2 ? 4 : 2;
+    return sel == 1 ? 1 : sel == 2 ? 4 : 2;
+}
+
+void launch_solve_inv(const Chan &c, const StepPtrs &sp, const double *X, const int4 *jobs, int njobs, hipStream_t s) {
+    const int jpb = solve_jobs_per_block(c);
     const int nb = (njobs + jpb - 1) / jpb;
     const dim3 g(8 * ((nb + 7) / 8)), b(64 * jpb);
     if (jpb == 1) OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_inv<NQ, 1>), g, b, 0, s, c, sp, X, jobs, njobs));
     else if (jpb == 4) OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_inv<NQ, 4>), g, b, 0, s, c, sp, X, jobs, njobs));
     else OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_inv<NQ, 2>), g, b, 0, s, c, sp, X, jobs, njobs));
+}
+
+// grouped v solves over ensemble members: the instance is fixed by the rows per
+// lane (NQ) and the jobs per block
+int ens_key_solve(const EnsEntry &e, int *blocks) {
+    const int jpb = solve_jobs_per_block(e.c), nb = (e.njobs + jpb - 1) / jpb;
+    *blocks = 8 * ((nb + 7) / 8);
+    return ((e.c.NY + 63) / 64) * 16 + jpb;
+}
+
+void launch_ens_solve(const EnsEntry *ents, const int *first, int nm, int grid, const EnsEntry &rep, hipStream_t s) {
+    const int jpb = solve_jobs_per_block(rep.c);
+    const dim3 g(grid), b(64 * jpb);
+    if (jpb == 1) OWT_NQ_DISPATCH(rep.c.NY, hipLaunchKernelGGL((k_solve_inv_ens<NQ, 1>), g, b, 0, s, ents, first, nm));
+    else if (jpb == 4) OWT_NQ_DISPATCH(rep.c.NY, hipLaunchKernelGGL((k_solve_inv_ens<NQ, 4>), g, b, 0, s, ents, first, nm));
+    else OWT_NQ_DISPATCH(rep.c.NY, hipLaunchKernelGGL((k_solve_inv_ens<NQ, 2>), g, b, 0, s, ents, first, nm));
 }
 
 void launch_factor_transpose(double *F, int nmat, int N, hipStream_t s) {

@@ -48,13 +48,14 @@ __device__ __forceinline__ owt_d4 mfma4(double a, double b, owt_d4 c) { return _
 // owns row tiles w, w + 8, ... (RT of them: 2 up to NY = 256, 3 above) and
 // keeps the DY operands of the next four k steps in flight.
 template <int RT>
-__global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
+__device__ __forceinline__ void horizontal_mfma_body(const Chan &c, const StepPtrs &sp, const int bid) {
     constexpr int NW = 8;
     // the block's whole columns of v and omega_y (2 x 16 x 289 x 16 B = 148 KB
     // of the 160 KB per CU), staged chunk by chunk and read again by the epilogue
     __shared__ double2 sv[kMP][kMaxNY + 1], sg[kMP][kMaxNY + 1];
     const int N = c.NY, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int p0 = blockIdx.x * kMP, ntile = (N + 15) / 16;
+    const int p0 = bid * kMP, ntile = (N + 15) / 16;
+    if (p0 >= c.npencil) return;
     owt_d4 acc[RT][2][3];
     // staging: this thread's element of a chunk (kMP * kKC == blockDim)
     const int sq = threadIdx.x / kKC, skk = threadIdx.x % kKC, sp_ = p0 + sq;
@@ -182,6 +183,17 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
         }
     }
 }
+template <int RT>
+__global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
+    horizontal_mfma_body<RT>(c, sp, blockIdx.x);
+}
+template <int RT>
+__global__ __launch_bounds__(512) void k_horizontal_mfma_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    horizontal_mfma_body<RT>(e.c, e.sp, bid);
+}
 
 // nonlinear source terms (src/solver.cpp:240-260) with DY1 (i kx Lx + i kz Lz)
 // on the matrix cores; Hg and the mean Lamb history (the k = 0 pencil of the
@@ -189,14 +201,15 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma(Chan c, StepPtrs sp) {
 // Eight waves, wave w owning row tiles w and w + 8, DY1 prefetched as in
 // k_horizontal_mfma.
 template <int RT, typename LC>
-__global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
+__device__ __forceinline__ void nonlinear_mfma_body(const Chan &c, const StepPtrs &sp, const int bid) {
     constexpr int NW = 8, PF = 4;
     static_assert((kKC / 4) % PF == 0, "chunks hold whole rounds of the operand ring");
     __shared__ double2 st[kMP][kKC + 1];
     const int N = c.NY, ns = sp.sn, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const LC *lx = reinterpret_cast<const LC *>(c.lmb);
     const LC *ly = lx + c.W, *lz = lx + 2 * c.W;
-    const int p0 = blockIdx.x * kMP, ntile = (N + 15) / 16;
+    const int p0 = bid * kMP, ntile = (N + 15) / 16;
+    if (p0 >= c.npencil) return;
     // staging: this thread's element of a chunk (kMP * kKC == blockDim)
     const int sq = threadIdx.x / kKC, skk = threadIdx.x % kKC, spn = p0 + sq;
     double skx = 0.0, skz = 0.0;
@@ -301,6 +314,17 @@ __global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
         }
     }
 }
+template <int RT, typename LC>
+__global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
+    nonlinear_mfma_body<RT, LC>(c, sp, blockIdx.x);
+}
+template <int RT, typename LC>
+__global__ __launch_bounds__(512) void k_nonlinear_mfma_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    nonlinear_mfma_body<RT, LC>(e.c, e.sp, bid);
+}
 
 // ---------------------------------------------------------------------------
 // omega_y of the new level, batched method (src/solver.cpp:574-603 for the
@@ -317,13 +341,14 @@ __global__ __launch_bounds__(512) void k_nonlinear_mfma(Chan c, StepPtrs sp) {
 // of the derivative kernels; the operands of the next four k steps are in flight.
 // ---------------------------------------------------------------------------
 template <int RT>
-__global__ __launch_bounds__(512) void k_helmholtz_g(Chan c, StepPtrs sp) {
+__device__ __forceinline__ void helmholtz_g_body(const Chan &c, const StepPtrs &sp, const int bid) {
     constexpr int NW = 8, PF = 4;
     static_assert((kKC / 4) % PF == 0, "chunks hold whole rounds of the operand ring");
     __shared__ double2 st[kMP][kKC + 1];
     extern __shared__ double szs[];  // Z: 32 columns of ZS = 16 ntile + 1 rows
     const int N = c.NY, M = N - 2, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int p0 = blockIdx.x * kMP, ntile = (M + 15) / 16, ZS = 16 * ntile + 1;
+    const int p0 = bid * kMP, ntile = (M + 15) / 16, ZS = 16 * ntile + 1;
+    if (p0 >= c.npencil) return;
     const double dt = c.dt, nu = c.nu;
     // staging: this thread's element of a chunk (kMP * kKC == blockDim)
     const int sq = threadIdx.x / kKC, skk = threadIdx.x % kKC, spn = p0 + sq;
@@ -479,6 +504,15 @@ __global__ __launch_bounds__(512) void k_helmholtz_g(Chan c, StepPtrs sp) {
         const int p = p0 + (threadIdx.x >> 1);
         if (p < c.npencil) sp.gn[(size_t)p * N + ((threadIdx.x & 1) ? N - 1 : 0)] = c2(0, 0);
     }
+}
+template <int RT>
+__global__ __launch_bounds__(512) void k_helmholtz_g(Chan c, StepPtrs sp) { helmholtz_g_body<RT>(c, sp, blockIdx.x); }
+template <int RT>
+__global__ __launch_bounds__(512) void k_helmholtz_g_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    helmholtz_g_body<RT>(e.c, e.sp, bid);
 }
 
 // ---------------------------------------------------------------------------
@@ -644,8 +678,9 @@ __device__ void mean_friction(const Chan &c, const double2 *U, const double2 *Wm
 // Ag(0) (factor Fg, pivot rows Pg) in wave 0, then the friction and mean
 // vorticity.
 template <int NQ>
-__global__ __launch_bounds__(256) void k_mean(Chan c, StepPtrs sp, const double *Fg, const int *Pg) {
+__device__ __forceinline__ void mean_body(const Chan &c, const StepPtrs &sp, const double *Fg, const int *Pg, const int bid) {
     __shared__ double S[4][64 * NQ];
+    if (bid != 0) return;
     const int N = c.NY, lane = threadIdx.x & 63;
     const double dt = c.dt;
     const int step = c.ctrl[C_STEP];
@@ -694,6 +729,17 @@ __global__ __launch_bounds__(256) void k_mean(Chan c, StepPtrs sp, const double 
     mean_friction(c, c.msol, c.msol + N, false, sp.sn, step + 1);
     __syncthreads();
     if (threadIdx.x == 0) c.ctrl[C_STEP] = step + 1;
+}
+template <int NQ>
+__global__ __launch_bounds__(256) void k_mean(Chan c, StepPtrs sp, const double *Fg, const int *Pg) {
+    mean_body<NQ>(c, sp, Fg, Pg, blockIdx.x);
+}
+template <int NQ>
+__global__ __launch_bounds__(256) void k_mean_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    mean_body<NQ>(e.c, e.sp, e.Fg, e.Pg, bid);
 }
 
 // ---------------------------------------------------------------------------
@@ -1285,6 +1331,65 @@ void launch_helmholtz_g(const Chan &c, const StepPtrs &sp, hipStream_t s) {
 
 void launch_mean(const Chan &c, const StepPtrs &sp, const double *Fg, const int *Pg, hipStream_t s) {
     OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_mean<NQ>, dim3(1), dim3(256), 0, s, c, sp, Fg, Pg));
+}
+
+// ---- grouped launches of the step's stages over ensemble members ----
+static int round8(int n) { return 8 * ((n + 7) / 8); }
+
+int ens_key(const EnsEntry &e, int stage) {
+    int blocks = 0;
+    const Chan &c = e.c;
+    switch (stage) {
+        case ES_SOLVE: return ens_key_solve(e, &blocks);
+        case ES_HELMHOLTZ: return (c.NY - 2 + 15) / 16 <= 16 ? 2 : 3;
+        case ES_MEAN: return (c.NY + 63) / 64;
+        case ES_HORIZONTAL: return c.NY <= 256 ? 2 : 3;
+        case ES_NONLINEAR: return (c.NY <= 256 ? 2 : 3) * 2 + c.precision;
+        default: return ens_key_fft(e, stage, &blocks);
+    }
+}
+
+int ens_blocks(const EnsEntry &e, int stage) {
+    int blocks = 0;
+    switch (stage) {
+        case ES_SOLVE: ens_key_solve(e, &blocks); return blocks;
+        case ES_MEAN: return 8;  // one block does the work
+        case ES_HELMHOLTZ:
+        case ES_HORIZONTAL:
+        case ES_NONLINEAR: return round8((e.c.npencil + kMP - 1) / kMP);
+        default: ens_key_fft(e, stage, &blocks); return blocks;
+    }
+}
+
+void launch_ens(int stage, const EnsEntry *ents, const int *first, int nm, int grid, const EnsEntry &rep, int ny_max,
+                hipStream_t s) {
+    const Chan &c = rep.c;
+    const dim3 g(grid);
+    switch (stage) {
+        case ES_SOLVE: launch_ens_solve(ents, first, nm, grid, rep, s); break;
+        case ES_HELMHOLTZ: {
+            const int ntile = (ny_max - 2 + 15) / 16;
+            const size_t lds = sizeof(double) * 2 * kMP * (16 * ntile + 1);
+            if ((c.NY - 2 + 15) / 16 <= 16) hipLaunchKernelGGL(k_helmholtz_g_ens<2>, g, dim3(512), lds, s, ents, first, nm);
+            else hipLaunchKernelGGL(k_helmholtz_g_ens<3>, g, dim3(512), lds, s, ents, first, nm);
+            break;
+        }
+        case ES_MEAN: OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_mean_ens<NQ>, g, dim3(256), 0, s, ents, first, nm)); break;
+        case ES_HORIZONTAL:
+            if (c.NY <= 256) hipLaunchKernelGGL(k_horizontal_mfma_ens<2>, g, dim3(512), 0, s, ents, first, nm);
+            else hipLaunchKernelGGL(k_horizontal_mfma_ens<3>, g, dim3(512), 0, s, ents, first, nm);
+            break;
+        case ES_NONLINEAR:
+            if (c.NY <= 256) {
+                if (c.precision == 0) hipLaunchKernelGGL((k_nonlinear_mfma_ens<2, double2>), g, dim3(512), 0, s, ents, first, nm);
+                else hipLaunchKernelGGL((k_nonlinear_mfma_ens<2, float2>), g, dim3(512), 0, s, ents, first, nm);
+            } else {
+                if (c.precision == 0) hipLaunchKernelGGL((k_nonlinear_mfma_ens<3, double2>), g, dim3(512), 0, s, ents, first, nm);
+                else hipLaunchKernelGGL((k_nonlinear_mfma_ens<3, float2>), g, dim3(512), 0, s, ents, first, nm);
+            }
+            break;
+        default: launch_ens_fft(stage, ents, first, nm, grid, rep, s); break;
+    }
 }
 
 void launch_nonlinear(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s) {

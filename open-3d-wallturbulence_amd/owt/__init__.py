@@ -36,7 +36,8 @@ EXPORTS = [
     "owt_checkpoint", "owt_restore", "owt_f2p", "owt_p2f", "owt_lamb", "owt_get_matrix",
     "owt_time_steps", "owt_time_kernel", "owt_get_factor", "owt_save",
     "owt_ensemble_create", "owt_ensemble_destroy", "owt_ensemble_last_error", "owt_ensemble_member",
-    "owt_ensemble_init_fields", "owt_ensemble_step", "owt_ensemble_time_steps",
+    "owt_ensemble_init_fields", "owt_ensemble_step", "owt_ensemble_time_steps", "owt_ensemble_launches",
+    "owt_ensemble_autotune",
     "owt_tangent_init", "owt_tangent_handle", "owt_tangent_step", "owt_adjoint_run", "owt_adjoint_apply",
     "owt_autotune", "owt_get_tune", "owt_set_tune", "owt_save_fields", "owt_statistics",
     "owt_f2p_noalloc", "owt_get_run_params",
@@ -128,6 +129,8 @@ def lib():
         L.owt_ensemble_init_fields.argtypes = [P, I, P, D]
         L.owt_ensemble_step.argtypes = [P, I]
         L.owt_ensemble_time_steps.argtypes = [P, I, P]
+        L.owt_ensemble_launches.argtypes = [P, P]
+        L.owt_ensemble_autotune.argtypes = [P, I]
         _lib = L
     return _lib
 
@@ -404,6 +407,17 @@ class Ensemble:
         ms = ctypes.c_double()
         self._check(lib().owt_ensemble_time_steps(self._e, int(n), ctypes.byref(ms)))
         return ms.value
+
+    def autotune(self, reps=5):
+        """Launch-shape autotuner: one member per distinct parameter set is tuned,
+        the members with the same parameters take its choices."""
+        self._check(lib().owt_ensemble_autotune(self._e, int(reps)))
+
+    def launches_per_step(self):
+        """Kernel launches of one grouped ensemble step (0: members stepped one by one)."""
+        n = ctypes.c_int()
+        self._check(lib().owt_ensemble_launches(self._e, ctypes.byref(n)))
+        return n.value
 
     @property
     def gridpoints(self):

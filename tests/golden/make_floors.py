@@ -6,7 +6,8 @@ fields: the oracle is run from the seeded initial field, and again from the
 same field with a one-ulp Gaussian perturbation of u, v, w and with every
 step's Lamb vector and solved v, omega_y multiplied by (1 + 2^-52 g), g ~ N(0, 1)
 (Oracle.set_noise). The floor of a field is the largest relative L2 difference
-over four perturbation seeds. The GPU parity tests require each field to stay
+over four perturbation seeds. A case's "seed" (0 if absent) is the seed of its
+initial field. The GPU parity tests require each field to stay
 within 1.5 times its floor (and within 1e-10 where the floor allows).
 
     python tests/golden/make_floors.py [case ...]     (writes tests/golden/floors.json)
@@ -38,6 +39,10 @@ CASES = {
     "case180_wall_16": dict(nu=3.571428571428571e-4, dt=2e-3, dkx=0.25, dkz=0.5, alpha_mesh=2.0, NX=16, NY=160, NZ=16,
                             amp=0.15),
     "case550_wall_16": dict(nu=1e-4, dt=1e-3, dkx=0.5, dkz=1.0, alpha_mesh=2.4, NX=16, NY=288, NZ=16, amp=0.075),
+    # the other two member shapes of the 32-channel Re_tau 180 ensemble (4pi x 2 x 2pi box),
+    # each from the seed of its first member (16 and 24)
+    "ens180_64x97": dict(nu=1 / 2800, dt=1e-3, dkx=0.5, dkz=1.0, alpha_mesh=1.5, NX=64, NY=97, NZ=64, amp=0.15, seed=16),
+    "ens180_96x49": dict(nu=1 / 2800, dt=1e-3, dkx=0.5, dkz=1.0, alpha_mesh=1.5, NX=96, NY=49, NZ=96, amp=0.15, seed=24),
 }
 
 
@@ -47,7 +52,7 @@ def make(c):
 
 def run(c, seed=None):
     o = make(c)
-    o.init(2, seed=0, amp=c["amp"])
+    o.init(2, seed=c.get("seed", 0), amp=c["amp"])
     if seed is not None:
         rng = np.random.default_rng(seed)
         uvw = [o.field(n) for n in ("u", "v", "w")]
