@@ -45,20 +45,33 @@ template <typename C>
 __device__ __forceinline__ C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
 template <typename C>
 __device__ __forceinline__ C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+// The transform arithmetic states every fused multiply-add itself and lets the
+// compiler contract nothing, so that every kernel instance (tile shape, length
+// plan, single or ensemble launch) rounds the same way and gives the same bits.
+__device__ __forceinline__ double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <typename C>
-__device__ __forceinline__ C cmul(C a, C b) { return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ C cmul(C a, C b) {
+#pragma clang fp contract(off)
+    return mk<C>(fmx(a.x, b.x, -(a.y * b.y)), fmx(a.x, b.y, a.y * b.x));
+}
 
 // a b - c d: in FP64 each product and the difference rounded separately, as
 // the reference's pointwise Field operations do (src/linalg.hpp:756-816; no
-// contraction into a fused multiply-add); in FP32 contracted
+// contraction into a fused multiply-add); in FP32 one fused multiply-add
 __device__ __forceinline__ double cross_term(double a, double b, double c, double d) {
 #pragma clang fp contract(off)
     return a * b - c * d;
 }
-__device__ __forceinline__ float cross_term(float a, float b, float c, float d) { return a * b - c * d; }
+__device__ __forceinline__ float cross_term(float a, float b, float c, float d) {
+#pragma clang fp contract(off)
+    return fmx(a, b, -(c * d));
+}
 
 template <typename C, int R>
 __device__ __forceinline__ void dft(C (&x)[R], int sign) {
+#pragma clang fp contract(off)
     using T = decltype(C::x);
     if constexpr (R == 2) {
         C a = x[0], b = x[1];
@@ -67,10 +80,10 @@ __device__ __forceinline__ void dft(C (&x)[R], int sign) {
     } else if constexpr (R == 3) {
         const T sn = (T)sign * (T)0.86602540378443864676372317075294;
         C t = cadd(x[1], x[2]), d = csub(x[1], x[2]);
-        C m = mk<C>(x[0].x - (T)0.5 * t.x, x[0].y - (T)0.5 * t.y);
+        C m = mk<C>(fmx((T)-0.5, t.x, x[0].x), fmx((T)-0.5, t.y, x[0].y));
         x[0] = cadd(x[0], t);
-        x[1] = mk<C>(m.x - sn * d.y, m.y + sn * d.x);
-        x[2] = mk<C>(m.x + sn * d.y, m.y - sn * d.x);
+        x[1] = mk<C>(fmx(-sn, d.y, m.x), fmx(sn, d.x, m.y));
+        x[2] = mk<C>(fmx(sn, d.y, m.x), fmx(-sn, d.x, m.y));
     } else if constexpr (R == 4) {
         C a = cadd(x[0], x[2]), b = csub(x[0], x[2]), c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
         const T s = (T)sign;
@@ -86,10 +99,10 @@ __device__ __forceinline__ void dft(C (&x)[R], int sign) {
         const T s1 = (T)sign * (T)0.95105651629515357211643933337938214,
                 s2 = (T)sign * (T)0.58778525229247312916870595463907277;
         const C t1 = cadd(x[1], x[4]), t2 = cadd(x[2], x[3]), t3 = csub(x[1], x[4]), t4 = csub(x[2], x[3]);
-        const C a1 = mk<C>(x[0].x + c1 * t1.x + c2 * t2.x, x[0].y + c1 * t1.y + c2 * t2.y);
-        const C a2 = mk<C>(x[0].x + c2 * t1.x + c1 * t2.x, x[0].y + c2 * t1.y + c1 * t2.y);
-        const C b1 = mk<C>(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
-        const C b2 = mk<C>(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+        const C a1 = mk<C>(fmx(c2, t2.x, fmx(c1, t1.x, x[0].x)), fmx(c2, t2.y, fmx(c1, t1.y, x[0].y)));
+        const C a2 = mk<C>(fmx(c1, t2.x, fmx(c2, t1.x, x[0].x)), fmx(c1, t2.y, fmx(c2, t1.y, x[0].y)));
+        const C b1 = mk<C>(fmx(s2, t4.x, s1 * t3.x), fmx(s2, t4.y, s1 * t3.y));
+        const C b2 = mk<C>(fmx(-s1, t4.x, s2 * t3.x), fmx(-s1, t4.y, s2 * t3.y));
         x[0] = mk<C>(x[0].x + t1.x + t2.x, x[0].y + t1.y + t2.y);
         x[1] = mk<C>(a1.x - b1.y, a1.y + b1.x);
         x[4] = mk<C>(a1.x + b1.y, a1.y - b1.x);
@@ -138,10 +151,94 @@ __device__ __forceinline__ void stage(C *buf, const C *tw, int L, int Ns, unsign
     __syncthreads();
 }
 
+// The same stage with the length, span and twiddle stride known at compile
+// time: k = j mod Ns and every LDS address fold into shifts, adds and
+// immediate offsets, and rounds that are full need no guard. The arithmetic on
+// the data is the runtime stage's, so results are the same bit for bit.
+template <typename C, int R, int TJ, int TL, int L, int Ns>
+__device__ __forceinline__ void stage_fixed(C *buf, const C *tw, int sign, int jj, int tl) {
+    constexpr int Lr = L / R, tstep = L / (Ns * R), MAXB = (Lr + TL - 1) / TL;
+    C out[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tl + b * TL;
+        if ((b + 1) * TL <= Lr || j < Lr) {
+            const int k = j % Ns;
+            C x[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) x[q] = buf[(j + q * Lr) * TJ + jj];
+#pragma unroll
+            for (int q = 1; q < R; ++q) {
+                C w = tw[q * k * tstep];
+                if (sign > 0) w.y = -w.y;
+                x[q] = cmul(x[q], w);
+            }
+            dft<C, R>(x, sign);
+#pragma unroll
+            for (int q = 0; q < R; ++q) out[b][q] = x[q];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; ++b) {
+        const int j = tl + b * TL;
+        if ((b + 1) * TL <= Lr || j < Lr) {
+            const int k = j % Ns;
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[(base + q * Ns) * TJ + jj] = out[b][q];
+        }
+    }
+    __syncthreads();
+}
+
+// compile-time plan of a length: make_plan's radix order (4s, 3s, 2s, 5s)
+struct FixedPlan {
+    int nst = 0;
+    int radix[kMaxStages] = {0};
+    int ns[kMaxStages] = {0};
+};
+constexpr FixedPlan fixed_plan(int L) {
+    FixedPlan p{};
+    int n = L;
+    while (n % 4 == 0) p.radix[p.nst++] = 4, n /= 4;
+    while (n % 3 == 0) p.radix[p.nst++] = 3, n /= 3;
+    while (n % 2 == 0) p.radix[p.nst++] = 2, n /= 2;
+    while (n % 5 == 0) p.radix[p.nst++] = 5, n /= 5;
+    int ns = 1;
+    for (int s = 0; s < p.nst; ++s) {
+        p.ns[s] = ns;
+        ns *= p.radix[s];
+    }
+    return p;
+}
+
+template <typename C, int TJ, int TL, int L, int S>
+__device__ __forceinline__ void fft_fixed(C *buf, const C *tw, int sign, int jj, int tl) {
+    constexpr FixedPlan P = fixed_plan(L);
+    if constexpr (S < P.nst) {
+        stage_fixed<C, P.radix[S], TJ, TL, L, P.ns[S]>(buf, tw, sign, jj, tl);
+        fft_fixed<C, TJ, TL, L, S + 1>(buf, tw, sign, jj, tl);
+    }
+}
+
 // Full transform of the TJ lines in buf (natural order in and out).
 // Callers synchronise before the call; the call ends synchronised.
+// The lengths of the benchmark and test grids (the largest of each length
+// class, and 144, 480 and 960) run the compile-time plan; others the plan P.
 template <typename C, int TJ, int TL, int LMAX>
 __device__ void fft_lines(C *buf, const C *tw, const FFTPlan &P, int sign, int jj, int tl) {
+    if (P.L == LMAX) {
+        fft_fixed<C, TJ, TL, LMAX, 0>(buf, tw, sign, jj, tl);
+        return;
+    }
+    if constexpr (LMAX == 192 || LMAX == 576 || LMAX == 1152) {
+        constexpr int L2 = LMAX == 192 ? 144 : LMAX == 576 ? 480 : 960;
+        if (P.L == L2) {
+            fft_fixed<C, TJ, TL, L2, 0>(buf, tw, sign, jj, tl);
+            return;
+        }
+    }
     for (int s = 0; s < P.nst; ++s) {
         const int R = P.radix[s];
         if (R == 4)
