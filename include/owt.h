@@ -258,14 +258,19 @@ int owt_ensemble_launches(owt_ensemble *e, int *n);
  * the members with the same parameters take its choices */
 int owt_ensemble_autotune(owt_ensemble *e, int reps);
 
-/* timing: enqueue nsteps steps on the handle's stream between two HIP events
- * and return the elapsed milliseconds (inputs stay resident on the device) */
+/* timing: 3 untimed steps (one per ring phase: the graphs are built), then
+ * nsteps steps on the handle's stream between two HIP events; the elapsed
+ * milliseconds. The state advances by nsteps + 3 steps, as by owt_step (inputs
+ * stay resident on the device). Refused (OWT_E_ARG) with a tangent attached. */
 int owt_time_steps(owt_handle *h, int nsteps, double *ms);
-/* timing of one stage kernel launched reps times on the handle's stream:
- * 0 right-hand sides + implicit solves, 1 horizontal, 2 mean flow, 3 z inverse,
- * 4 x phase, 5 z forward, 6 nonlinear, 7 LU factorization (the plain method
- * refactors every step), 8 the batched method's omega_y solves (nothing in the
- * plain method). Returns the mean milliseconds per launch. */
+/* timing of one stage kernel launched reps times (after one untimed launch) on
+ * the handle's stream at the current ring phase: 0 right-hand sides + implicit
+ * solves, 1 horizontal, 2 mean flow, 3 z inverse, 4 x phase, 5 z forward,
+ * 6 nonlinear, 7 LU factorization (the plain method refactors every step),
+ * 8 the batched method's omega_y solves (nothing in the plain method). The
+ * state is saved before and restored after, so fields and friction series are
+ * unchanged. Returns the mean milliseconds per launch. Refused with a tangent
+ * attached. */
 int owt_time_kernel(owt_handle *h, int kernel, int reps, double *ms);
 
 #ifdef __cplusplus

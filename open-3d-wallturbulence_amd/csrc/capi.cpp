@@ -404,47 +404,13 @@ int owt_get_factor(owt_handle *h, int sys, int op, double *lu, int *perm) {
 
 int owt_time_steps(owt_handle *h, int nsteps, double *ms) {
     if (!ms || nsteps < 1) return OWT_E_ARG;
-    return guard(h, [&] {
-        owt::Channel &c = *h->ch;
-        hipEvent_t a, b;
-        OWT_HIP(hipEventCreate(&a));
-        OWT_HIP(hipEventCreate(&b));
-        // one step per ring phase builds and uploads the three graphs outside the timed region
-        for (int k = 0; k < 3; ++k) c.enqueue_step();
-        c.sync();
-        OWT_HIP(hipEventRecord(a, c.stream()));
-        for (int k = 0; k < nsteps; ++k) c.enqueue_step();
-        OWT_HIP(hipEventRecord(b, c.stream()));
-        OWT_HIP(hipEventSynchronize(b));
-        float t = 0;
-        OWT_HIP(hipEventElapsedTime(&t, a, b));
-        *ms = t;
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-    });
+    return guard(h, [&] { *ms = h->ch->time_steps(nsteps); });
 }
 
 int owt_time_kernel(owt_handle *h, int kernel, int reps, double *ms) {
     if (!ms || reps < 1) return OWT_E_ARG;
     if (kernel < 0 || kernel > 8) return OWT_E_ARG;
-    return guard(h, [&] {
-        owt::Channel &c = *h->ch;
-        hipStream_t s = c.stream();
-        c.enqueue_kernel(kernel);  // warm launch (and, batched, the one-time factorization)
-        c.sync();
-        hipEvent_t a, b;
-        OWT_HIP(hipEventCreate(&a));
-        OWT_HIP(hipEventCreate(&b));
-        OWT_HIP(hipEventRecord(a, s));
-        for (int r = 0; r < reps; ++r) c.enqueue_kernel(kernel);
-        OWT_HIP(hipEventRecord(b, s));
-        OWT_HIP(hipEventSynchronize(b));
-        float t = 0;
-        OWT_HIP(hipEventElapsedTime(&t, a, b));
-        *ms = t / reps;
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-    });
+    return guard(h, [&] { *ms = h->ch->time_kernel(kernel, reps); });
 }
 
 }  // extern "C"

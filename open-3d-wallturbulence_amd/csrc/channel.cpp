@@ -779,6 +779,67 @@ void Channel::enqueue_kernel(int id) {
 
 void Channel::sync() { OWT_HIP(hipStreamSynchronize(s_)); }
 
+static double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    OWT_HIP(hipEventSynchronize(b));
+    float t = 0;
+    OWT_HIP(hipEventElapsedTime(&t, a, b));
+    return t;
+}
+
+double Channel::time_steps(int n) {
+    if (n < 1) throw Error(2, "time_steps: step count must be positive");
+    if (tangent_) throw Error(2, "time_steps: not with a tangent attached (owt_step steps both)");
+    for (int k = 0; k < 3; ++k) enqueue_step();
+    sync();
+    hipEvent_t a = nullptr, b = nullptr;
+    OWT_HIP(hipEventCreate(&a));
+    OWT_HIP(hipEventCreate(&b));
+    double ms = 0;
+    try {
+        OWT_HIP(hipEventRecord(a, s_));
+        for (int k = 0; k < n; ++k) enqueue_step();
+        OWT_HIP(hipEventRecord(b, s_));
+        ms = elapsed_ms(a, b);
+    } catch (...) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        throw;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    finish();
+    return ms;
+}
+
+double Channel::time_kernel(int id, int reps) {
+    if (reps < 1) throw Error(2, "time_kernel: reps must be positive");
+    if (tangent_) throw Error(2, "time_kernel: not with a tangent attached");
+    prepare_step();
+    save_state(timing_);
+    enqueue_kernel(id);  // warm launch (and, batched, the one-time factorization)
+    sync();
+    hipEvent_t a = nullptr, b = nullptr;
+    OWT_HIP(hipEventCreate(&a));
+    OWT_HIP(hipEventCreate(&b));
+    double ms = 0;
+    try {
+        OWT_HIP(hipEventRecord(a, s_));
+        for (int r = 0; r < reps; ++r) enqueue_kernel(id);
+        OWT_HIP(hipEventRecord(b, s_));
+        ms = elapsed_ms(a, b) / reps;
+    } catch (...) {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        load_state(timing_);
+        throw;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    load_state(timing_);
+    finish();
+    return ms;
+}
+
 void Channel::prepare_step() {
     if (P_.method == 1) factor_operators();
     else ensure_plain_factors();

@@ -57,6 +57,16 @@ class Channel {
     void p2f(const double *phys, int nf, double *wave);
     void lamb(const double *six, double *three);
 
+    // Timing (HIP events on the channel's stream). time_steps runs 3 steps (one
+    // per ring phase, which builds the graphs) and then n timed steps: the state
+    // advances by n + 3 steps. time_kernel launches one stage of the step reps
+    // times at the current ring phase (after one untimed launch) on the state,
+    // which it saves before and restores after. Both refuse a channel with an
+    // attached tangent (whose step reads this channel's transform buffer) and
+    // report errors like step(). Milliseconds: total for time_steps, per launch
+    // for time_kernel.
+    double time_steps(int n);
+    double time_kernel(int id, int reps);
     // device-side timing helpers (bench): run one step on the stream without syncing
     void enqueue_step();
     // one launch of a single stage of the step at the current ring phase:
@@ -206,6 +216,7 @@ class Channel {
     std::unique_ptr<Channel> adjoint_;
     bool transposed_ = false;
     std::vector<State> ckpts_;
+    State timing_;  // the state saved around time_kernel
 };
 
 FFTPlan make_plan(int L);

@@ -374,3 +374,43 @@ def test_save_fields_writes_every_dataset_family(owt_mod, tmp_path):
         assert ph.shape == ch.phys_shape
         assert np.array_equal(ph, ch.f2p(f)), key
     ch.close()
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_timing_entry_points_keep_or_advance_the_state(owt_mod, method):
+    """owt_time_kernel launches a stage on the saved state and restores it: every
+    field, the friction series and the step count are unchanged, and the next
+    steps equal those of an untimed channel bit for bit. owt_time_steps advances
+    the state by nsteps + 3 steps, like owt_step. Both refuse a channel with a
+    tangent attached."""
+    p = params(16, 17, 16, method=method)
+    print(f"[gpu] timing entry points, method {method}", flush=True)
+    ch, ref = gpu(owt_mod, p), gpu(owt_mod, p)
+    for c in (ch, ref):
+        c.init_fields(2, seed=4, amp=0.15)
+        c.step(2)
+    before = {n: ch.field(n) for n in ("u", "v", "w", "omy", "Hv", "Hg", "lmbx")}
+    for name in owt_mod.KERNELS:
+        assert ch.time_kernel(name, 2) >= 0
+    for n, f in before.items():
+        assert np.array_equal(ch.field(n), f), n
+    assert ch.steps_done() == 2 and np.array_equal(ch.diag(), ref.diag())
+    ch.step(3)
+    ref.step(3)
+    for n in before:
+        assert np.array_equal(ch.field(n), ref.field(n)), n
+    assert ch.time_steps(2) > 0
+    ref.step(5)
+    assert ch.steps_done() == ref.steps_done() == 10
+    assert np.array_equal(ch.field("v"), ref.field("v"))
+    if method == 1:
+        # a tangent starts from an initial field
+        ref.init_fields(2, seed=4, amp=0.15)
+        z = np.zeros(ref.wave_shape, complex)
+        ref.tangent_init(z, z, z)
+        with pytest.raises(owt_mod.OwtError):
+            ref.time_steps(1)
+        with pytest.raises(owt_mod.OwtError):
+            ref.time_kernel("solve", 1)
+    ch.close()
+    ref.close()
