@@ -1,9 +1,9 @@
 """The headline workload at full size: Re_tau 550, 384x193x384 (576x576
 dealiased planes), tanh stretch 2.0, nu 1e-4, dt 1e-3, seeded 5 % perturbation.
 
-The oracle cannot run this size in a test, so the checks are the ones that do
-not depend on size (its FP64 parity at this wall-normal grid is
-test_gpu_floors.py's re550_wall_32):
+The oracle cannot run this size inside a test (its run of the 100 steps is a
+fixture, below); the other checks do not depend on size (FP64 parity at this
+wall-normal grid over 10 steps is test_gpu_floors.py's re550_wall_32):
   - mixed precision (FP32 transforms) against FP64 over 100 steps: the mean wall
     shear (fx_fric, the friction force) within 1e-6 relative at every step, and
     the per-field relative L2 errors of u, v, w, p;
@@ -13,8 +13,15 @@ test_gpu_floors.py's re550_wall_32):
     transforms are Hermitian to rounding only, and the v operator at this grid
     amplifies rounding to about 1e-9 of the field in 10 steps (its floor in
     tests/golden/floors.json, re550_wall_32): the defect is bounded by 1e-8 of
-    the field's largest value, where a sign or index error would be of order one.
+    the field's largest value, where a sign or index error would be of order one;
+  - both precisions against the FP64 oracle's own run of the same 100 steps
+    (tests/golden/re550_384_100.npz, made by tests/golden/make_headline.py: 64
+    stored pencils of u, v, w, omega_y, p and the friction series, with their
+    rounding floor over the perturbation seeds): FP64 within 1.5 times the floor,
+    mixed precision with the mean wall shear within 1e-6 relative at every step.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -40,12 +47,13 @@ def runs(owt_mod):
         with owt_mod.Channel(params=dict(RE550, precision=prec, method=1)) as ch:
             ch.init_fields(2, seed=0, amp=0.075)
             ch.step(STEPS)
-            out[prec] = ({n: ch.field(n) for n in ("u", "v", "w", "p")}, ch.diag()[:, 0], ch.matrix("DY1"))
+            out[prec] = ({n: ch.field(n) for n in ("u", "v", "w", "p", "omy")}, ch.diag(), ch.matrix("DY1"))
     return out
 
 
 def test_mixed_wall_shear_and_fields_against_fp64(runs):
-    (f64, fx64, _), (fmx, fxmx, _) = runs[0], runs[1]
+    (f64, d64, _), (fmx, dmx, _) = runs[0], runs[1]
+    fx64, fxmx = d64[:, 0], dmx[:, 0]
     assert len(fx64) == len(fxmx) == STEPS + 1
     ws = np.abs(fxmx - fx64).max() / np.abs(fx64).max()
     errs = {n: rel(fmx[n], f64[n]) for n in BARS}
@@ -74,3 +82,30 @@ def test_full_size_divergence_and_hermitian_plane(runs):
                for n in ("u", "v", "w"))
     print(f"[headline] divergence {worst:.2e}, Hermitian defect {herm:.2e}", flush=True)
     assert worst < 1e-11 and herm < 1e-8
+
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "re550_384_100.npz")
+
+
+def test_both_precisions_against_the_oracle_at_full_size(runs):
+    ref = np.load(FIXTURE)
+    if ref["seeds"].size == 0:
+        pytest.skip("the oracle fixture holds no rounding floor yet (tests/golden/make_headline.py)")
+    case = dict(zip(("nu", "dt", "dkx", "dkz", "alpha_mesh", "NX", "NY", "NZ", "amp"), ref["case"]))
+    assert all(case[k] == RE550[k] for k in RE550) and int(ref["steps"]) == STEPS
+    kz, kx = ref["kz"], ref["kx"]
+    names = ("u", "v", "w", "omy", "p")
+    for prec, label in ((0, "fp64"), (1, "mixed")):
+        f, d, _ = runs[prec]
+        err = {n: rel(f[n][kz][:, kx], ref[n]) for n in names}
+        ws = np.abs(d[:, 0] - ref["diag"][:, 0]).max() / np.abs(ref["diag"][:, 0]).max()
+        print(f"[headline] {label} against the oracle: wall shear max rel {ws:.3e}; " +
+              " ".join(f"{n}={err[n]:.2e}/{err[n] / float(ref['floor_' + n]):.2f}" for n in names), flush=True)
+        assert ws <= 1e-6
+        if prec == 0:
+            for n in names:
+                assert err[n] <= 1.5 * float(ref["floor_" + n]), (n, err[n])
+            np.testing.assert_allclose(d[:, 0], ref["diag"][:, 0], rtol=1.5 * float(ref["floor_diag"]) + 1e-12)
+        else:
+            for n in ("u", "v", "w", "p"):
+                assert err[n] <= BARS[n], (n, err[n])
