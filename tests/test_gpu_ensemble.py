@@ -84,3 +84,29 @@ def test_launches_do_not_grow_with_members(owt_mod, ensemble):
     print(f"[ensemble] launches per step: 32 members {n32}, 3 members {n3}, 16 equal members {n16}", flush=True)
     assert n32 == n3 and 8 <= n32 <= 24
     assert n16 == 8
+
+
+def test_uneven_ensemble_of_other_kernel_instances(owt_mod):
+    """Members that run the grouped kernels' other instances, each equal to the
+    same channel stepped alone, bit for bit: mixed precision (FP32 transforms),
+    NY = 288 (three row tiles in the matrix-core kernels, the panel LU above 256
+    rows), a 480-point transform (radix 5), and an FP64 member of the same shape
+    as the mixed one (two lanes of one shape). Also after one member was stepped
+    alone (members stepped one by one until the phases agree)."""
+    base = dict(owt_mod.ENSEMBLES["re180_32"][0])
+    ps = [dict(base, NX=16, NY=33, NZ=16, precision=1), dict(base, NX=16, NY=33, NZ=16, precision=0),
+          dict(base, NX=8, NY=288, NZ=8, precision=0), dict(base, NX=320, NY=17, NZ=16, precision=1)]
+    print("[gpu] uneven ensemble: mixed, NY 288, 480-point x transform", flush=True)
+    with owt_mod.Ensemble(ps, method=owt_mod.BATCHED) as ens:
+        ens.init_fields(2, seeds=[3, 4, 5, 6], amp=0.15)
+        ens.step(3)
+        assert ens.launches_per_step() > 0
+        ens[2].step(1)
+        ens.step(2)
+        for m, (p, seed) in enumerate(zip(ps, (3, 4, 5, 6))):
+            with owt_mod.Channel(params=dict(p, method=1)) as ch:
+                ch.init_fields(2, seed=seed, amp=0.15)
+                ch.step(6 if m == 2 else 5)
+                for name in ("u", "v", "w", "omy", "Hv", "Hg", "lmbx"):
+                    assert np.array_equal(ens[m].field(name), ch.field(name)), (m, name)
+                assert np.array_equal(ens[m].diag(), ch.diag()), m
