@@ -183,8 +183,10 @@ int owt_get_factor(owt_handle *h, int sys, int op, double *lu, int *perm);
  * the nonlinear kernels (one shape each, the matrix-core kernels, so these two
  * slots accept any index and are not timed), tile shape of the z inverse, x
  * and z forward transforms. Every candidate computes the same values in the
- * same order, so results do not depend on the choice. report (may be NULL)
- * receives a JSON summary of the timings. */
+ * same order, so results do not depend on the choice. The candidates run on
+ * the handle's state, which is saved before and restored after (fields,
+ * histories and counters are unchanged). Refused with a tangent attached.
+ * report (may be NULL) receives a JSON summary of the timings. */
 enum { OWT_TUNE_COUNT = 6 };
 int owt_autotune(owt_handle *h, int reps, char *report, int report_len);
 int owt_get_tune(owt_handle *h, int *knobs);

@@ -1384,7 +1384,12 @@ std::string Channel::autotune(int reps) {
         {"default", "alt1", "alt2", "alt3"}, {"default", "alt1", "alt2", "alt3"}};
     // stage kernel of each knob (enqueue_kernel ids)
     static const int kernel[T_COUNT] = {0, 1, 6, 3, 4, 5};
+    if (tangent_) throw Error(2, "autotune: not with a tangent attached");
     prepare_step();
+    // the candidates run on the state, which they change (a stage writes the
+    // new-level slots, which hold history level 2 between steps): saved here and
+    // restored at the end
+    save_state(timing_);
     sync();
     hipEvent_t a, b;
     OWT_HIP(hipEventCreate(&a));
@@ -1431,6 +1436,8 @@ std::string Channel::autotune(int reps) {
     rep += "}";
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
+    load_state(timing_);
+    sync();
     check_errors();
     drop_graph();
     return rep;

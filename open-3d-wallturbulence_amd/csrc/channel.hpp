@@ -74,8 +74,10 @@ class Channel {
     // 6 nonlinear, 7 factorization, 8 omega_y solves by the eigendecomposition
     void enqueue_kernel(int id);
     // Launch-shape autotuner: times every candidate of every knob (TuneKnob) on
-    // this channel's data, keeps the fastest, and recaptures the step graphs.
-    // Returns a JSON report; get/set_tune read or force the choices.
+    // this channel's data (saved before and restored after: fields, histories
+    // and counters are unchanged), keeps the fastest, and recaptures the step
+    // graphs. Returns a JSON report; get/set_tune read or force the choices.
+    // Refused with a tangent attached.
     std::string autotune(int reps);
     void get_tune(int *knobs) const;
     void set_tune(const int *knobs);
@@ -216,7 +218,7 @@ class Channel {
     std::unique_ptr<Channel> adjoint_;
     bool transposed_ = false;
     std::vector<State> ckpts_;
-    State timing_;  // the state saved around time_kernel
+    State timing_;  // the state saved around time_kernel and autotune
 };
 
 FFTPlan make_plan(int L);

@@ -414,3 +414,23 @@ def test_timing_entry_points_keep_or_advance_the_state(owt_mod, method):
             ref.time_kernel("solve", 1)
     ch.close()
     ref.close()
+
+
+def test_autotune_leaves_the_state_unchanged(owt_mod):
+    """The autotuner times its candidates on the channel's state between steps (the
+    new-level slots it writes hold history level 2 there) and restores it: steps
+    after tuning equal the same steps without it, bit for bit."""
+    p = params(24, 33, 24)
+    print("[gpu] autotune between steps", flush=True)
+    a, b = gpu(owt_mod, p), gpu(owt_mod, p)
+    for c in (a, b):
+        c.init_fields(2, seed=8, amp=0.15)
+        c.step(2)
+    a.autotune(2)
+    for c in (a, b):
+        c.step(3)
+    for n in ("u", "v", "w", "omy", "Hv", "Hg"):
+        assert np.array_equal(a.field(n), b.field(n)), n
+    assert np.array_equal(a.diag(), b.diag())
+    a.close()
+    b.close()
