@@ -167,11 +167,15 @@ __device__ __forceinline__ void stage_fixed(C *buf, const C *tw, int sign, int j
             C x[R];
 #pragma unroll
             for (int q = 0; q < R; ++q) x[q] = buf[(j + q * Lr) * TJ + jj];
+            // the first stage's twiddles are all tw[0] = 1: its products change at
+            // most the sign of a zero, and are left out
+            if constexpr (Ns > 1) {
 #pragma unroll
-            for (int q = 1; q < R; ++q) {
-                C w = tw[q * k * tstep];
-                if (sign > 0) w.y = -w.y;
-                x[q] = cmul(x[q], w);
+                for (int q = 1; q < R; ++q) {
+                    C w = tw[q * k * tstep];
+                    if (sign > 0) w.y = -w.y;
+                    x[q] = cmul(x[q], w);
+                }
             }
             dft<C, R>(x, sign);
 #pragma unroll
