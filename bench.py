@@ -325,8 +325,8 @@ def main():
                          "copy rate, printed as JSON")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
-              f"(multi-GPU sharding is a design in DESIGN.md, not implemented)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              f"(sharding across GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     if args.mode == "copy":
@@ -373,7 +373,7 @@ def main():
         tuned = ch.autotune(5)
     print("[bench] mixed batched", file=sys.stderr, flush=True)
     t_mixed = timed(K, Wm)
-    # the survey's protocol beside the contract's single run: 3 runs of 100 steps, median
+    # beside the single run of --steps steps: the median of 3 runs of 100 steps
     t_med = sorted(ch.time_steps(100) / 100 for _ in range(3))[1] if args.mode == "full" else None
     if args.mode == "profile":
         tune = ch.tune
