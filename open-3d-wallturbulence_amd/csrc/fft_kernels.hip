@@ -19,6 +19,8 @@
 
 #include <climits>
 
+#include <type_traits>
+
 #include "device.hpp"
 
 namespace owt {
@@ -366,6 +368,7 @@ __device__ __forceinline__ void z_forward_body(const Chan &c, const typename CT<
     __syncthreads();
     fft_lines<C, TJ, TL, LMAX>(buf, tw, c.pz, -1, jj, tl);
     OutC *dst = out + (size_t)f * c.W;
+    const T rdiv = (T)1 / div;
     if (valid)
         for (int k = tl; k < NZ; k += TL) {
             // mode k <- bin k (k < NZH); mode k >= NZH <- bin k + NZH (negative frequencies)
@@ -373,8 +376,15 @@ __device__ __forceinline__ void z_forward_body(const Chan &c, const typename CT<
             C v = buf[bin * TJ + jj];
             if (k == NZH && !nyq) v = mk<C>(0, 0);
             OutC o;
-            o.x = v.x / div;
-            o.y = v.y / div;
+            if constexpr (std::is_same<T, float>::value) {
+                // FP32 (the mixed transform phase): the product with the rounded
+                // reciprocal of div, within one FP32 rounding of the quotient
+                o.x = v.x * rdiv;
+                o.y = v.y * rdiv;
+            } else {
+                o.x = v.x / div;
+                o.y = v.y / div;
+            }
             dst[((size_t)k * NXH + i) * NY + j] = o;
         }
 }
@@ -484,6 +494,11 @@ __device__ __forceinline__ void x_lamb_body(const Chan &c, const typename CT<T>:
         a[1][e] = mk<C>(lz, (T)0);
     }
     const T sc = (T)L;
+    // the forward transform's 1/NX2 (src/FFT.cpp:161): divided in FP64; in FP32
+    // (the mixed transform phase) a product with the rounded reciprocal, within
+    // one FP32 rounding of the quotient and an eighth of the instructions
+    constexpr bool DIV = !std::is_same<T, float>::value;
+    const T rs = (T)1 / sc, hrs = (T)0.5 / sc;
 #pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
 #pragma unroll
@@ -499,12 +514,19 @@ __device__ __forceinline__ void x_lamb_body(const Chan &c, const typename CT<T>:
                 if (pass == 0) {
                     const C zm = buf[((L - n) % L) * TJ + jj];
                     // spectra of the two real sequences packed as lx + i ly
-                    C X = mk<C>((zn.x + zm.x) * (T)0.5 / sc, (zn.y - zm.y) * (T)0.5 / sc);
-                    C Y = mk<C>((zn.y + zm.y) * (T)0.5 / sc, (zm.x - zn.x) * (T)0.5 / sc);
+                    const T ax = zn.x + zm.x, ay = zn.y - zm.y, bx = zn.y + zm.y, by = zm.x - zn.x;
+                    C X, Y;
+                    if constexpr (DIV) {
+                        X = mk<C>(ax * (T)0.5 / sc, ay * (T)0.5 / sc);
+                        Y = mk<C>(bx * (T)0.5 / sc, by * (T)0.5 / sc);
+                    } else {
+                        X = mk<C>(ax * hrs, ay * hrs);
+                        Y = mk<C>(bx * hrs, by * hrs);
+                    }
                     out[0 * c.I + plane + (size_t)n * NY] = X;
                     out[1 * c.I + plane + (size_t)n * NY] = Y;
                 } else {
-                    out[2 * c.I + plane + (size_t)n * NY] = mk<C>(zn.x / sc, zn.y / sc);
+                    out[2 * c.I + plane + (size_t)n * NY] = DIV ? mk<C>(zn.x / sc, zn.y / sc) : mk<C>(zn.x * rs, zn.y * rs);
                 }
             }
         }
