@@ -137,6 +137,9 @@ void Channel::alloc() {
     c.hQi = (const double *)dmalloc((N - 2) * (N - 2) * sizeof(double));
     c.hlre = (const double *)dmalloc((N - 2) * sizeof(double));
     c.hlim = (const double *)dmalloc((N - 2) * sizeof(double));
+    const size_t h = (N + 1) / 2;
+    c.DY1m = (const double *)dmalloc(2 * h * h * sizeof(double));
+    c.DY2m = (const double *)dmalloc(2 * h * h * sizeof(double));
     c.kx = (const double *)dmalloc(c.NXH * sizeof(double));
     c.kz = (const double *)dmalloc(c.NZ * sizeof(double));
     c.means = (double2 *)dmalloc(12 * N * sizeof(double2));
@@ -164,6 +167,9 @@ void Channel::upload_operators() {
     OWT_HIP(hipMemcpyAsync((void *)c_.DY1, op_.DY1.data(), N * N * sizeof(double), hipMemcpyHostToDevice, s_));
     OWT_HIP(hipMemcpyAsync((void *)c_.DY2, op_.DY2.data(), N * N * sizeof(double), hipMemcpyHostToDevice, s_));
     OWT_HIP(hipMemcpyAsync((void *)c_.DY4, op_.DY4.data(), N * N * sizeof(double), hipMemcpyHostToDevice, s_));
+    const std::vector<double> m1 = mirror_halves(op_.DY1, (int)N, -1), m2 = mirror_halves(op_.DY2, (int)N, +1);
+    OWT_HIP(hipMemcpyAsync((void *)c_.DY1m, m1.data(), m1.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+    OWT_HIP(hipMemcpyAsync((void *)c_.DY2m, m2.data(), m2.size() * sizeof(double), hipMemcpyHostToDevice, s_));
     // trapezoid weights of the y grid (pressure gauge): (y[j+1] - y[j-1]) / 2,
     // one-sided at the walls
     std::vector<double> qw(N);

@@ -87,7 +87,13 @@ struct Chan {
     // imaginary part hlim (0 for a real one; rows 2m, 2m+1 of a pair hold a, b of
     // a +- i b and form the block [[a, b], [-b, a]])
     const double *hQ, *hQi, *hlre, *hlim;
+    // mirror halves of DY1 and DY2 (A then B, h x h column-major each,
+    // h = (NY+1)/2): the mixed-precision derivative products
+    const double *DY1m, *DY2m;
 };
+// the mixed-precision step takes the DY1 and DY2 products as mirror halves
+// (NY <= 256: one row tile of the half range per wave)
+__host__ __device__ inline bool use_mirror(const Chan &c) { return c.precision == 1 && c.NY <= 256; }
 
 // History slots of one step: level 0 is the current field, levels 1 and 2 the
 // two before it; the new level overwrites level 2 (slot sn = s[2]).

@@ -113,6 +113,8 @@ void Ensemble::build_tables(int ring) {
             c.DY1 = o.c.DY1;
             c.DY2 = o.c.DY2;
             c.DY4 = o.c.DY4;
+            c.DY1m = o.c.DY1m;
+            c.DY2m = o.c.DY2m;
             c.kx = o.c.kx;
             c.kz = o.c.kz;
             c.qw = o.c.qw;

@@ -252,4 +252,25 @@ Operators build_operators(int N, double alpha) {
     return op;
 }
 
+std::vector<double> mirror_halves(const std::vector<double> &M, int N, int s) {
+    const int h = (N + 1) / 2;
+    auto at = [&](int i, int j) { return (long double)M[(size_t)j * N + i]; };
+    // rows of the symmetrised operator
+    auto ms = [&](int i, int j) { return (at(i, j) + (long double)s * at(N - 1 - i, N - 1 - j)) / 2; };
+    std::vector<double> out(2 * (size_t)h * h);
+    double *A = out.data(), *B = out.data() + (size_t)h * h;
+    for (int j = 0; j < h; ++j)
+        for (int i = 0; i < h; ++i) {
+            const size_t e = (size_t)j * h + i;
+            if (2 * j == N - 1) {
+                A[e] = (double)(ms(i, j) / 2);
+                B[e] = 0.0;
+            } else {
+                A[e] = (double)((ms(i, j) + ms(i, N - 1 - j)) / 2);
+                B[e] = (double)((ms(i, j) - ms(i, N - 1 - j)) / 2);
+            }
+        }
+    return out;
+}
+
 }  // namespace owt

@@ -54,6 +54,18 @@ struct Operators {
 };
 Operators build_operators(int NY, double alpha_mesh);
 
+// Mirror halves of a wall-normal operator M (column-major N x N) that is even
+// (s = +1: DY2) or odd (s = -1: DY1) under the reflection j -> N-1-j, up to the
+// rounding of its construction (a few hundred ulps). With h = (N+1)/2 and, for
+// j < h, e_j = x_j + x_{N-1-j} and o_j = x_j - x_{N-1-j}, the symmetrised
+// operator Ms = (M + s J M J) / 2 gives on rows i < h
+//   (Ms x)_i = sum_j A_ij e_j + B_ij o_j,
+//   A_ij = (Ms_ij + Ms_i,N-1-j) / 2,   B_ij = (Ms_ij - Ms_i,N-1-j) / 2
+// (for odd N the middle column c: A_ic = Ms_ic / 2, B_ic = 0), and on the rows
+// below (Ms x)_{N-1-i} = s (sum_j A_ij e_j - B_ij o_j). Returns A then B,
+// h x h column-major each.
+std::vector<double> mirror_halves(const std::vector<double> &M, int N, int s);
+
 // D = DY2's interior block (rows and columns 1..N-2) as Q B Q^-1, B block
 // diagonal: rows (2m, 2m+1), m < npair, a 2 x 2 block [[a, b], [-b, a]] of the
 // pair a +- i b (lre = a, lim = b on both rows); the other rows real eigenvalues

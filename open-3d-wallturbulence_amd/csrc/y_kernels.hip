@@ -195,6 +195,176 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma_ens(const EnsEntry *__r
     horizontal_mfma_body<RT>(e.c, e.sp, bid);
 }
 
+// ---------------------------------------------------------------------------
+// Mixed precision: the products with DY1 (odd under the reflection
+// j -> N-1-j) and DY2 (even) as two half-size products each. With h = (N+1)/2,
+// e_j = x_j + x_{N-1-j}, o_j = x_j - x_{N-1-j} (j < h) and the operator's
+// mirror halves A, B (h x h, from the symmetrised operator; operators.cpp):
+//   P = A e, Q = B o on the rows i < h,
+//   y_i = P_i + Q_i,  y_{N-1-i} = s (P_i - Q_i)   (s = +1 for DY2, -1 for DY1),
+// half the matrix-core work of the full products. The operators' departure from
+// mirror symmetry is the rounding of their construction (a few hundred FP64
+// ulps), far below the FP32 rounding of the transform phase. NY <= 256
+// (h <= 128): wave w owns row tile w of the half range with both parities, so P
+// and Q of a row meet in registers. The pencils' whole columns are staged in
+// LDS, each chunk from both ends of the column (k step j reads rows j and
+// N-1-j), and folded as the B operands are read.
+// ---------------------------------------------------------------------------
+constexpr int kMirrorMaxH = 128;
+
+__device__ __forceinline__ void horizontal_mirror_body(const Chan &c, const StepPtrs &sp, const int bid) {
+    __shared__ double2 sv[kMP][kMaxNY + 1], sg[kMP][kMaxNY + 1];
+    const int N = c.NY, h = (N + 1) / 2, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int p0 = bid * kMP;
+    if (p0 >= c.npencil) return;
+    const bool busy = w < (h + 15) / 16;
+    // [parity: 0 = A e, 1 = B o][column tile][product: DY1 v, DY2 v, DY1 omega_y]
+    owt_d4 acc[2][2][3];
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) acc[par][ct][q] = owt_d4{0.0, 0.0, 0.0, 0.0};
+    const int sq = threadIdx.x / kKC, skk = threadIdx.x % kKC, sp_ = p0 + sq;
+    const int am = lane & 15, ak = lane >> 4, bq = (lane & 15) >> 1, bc = lane & 1;
+    // operands of k step k for this lane's row: A1, B1 (DY1), A2, B2 (DY2);
+    // clamped indices as in the full kernel (the B operands of k >= h are zero)
+    const int ra = w * 16 + am < h ? w * 16 + am : h - 1;
+    const size_t hh = (size_t)h * h;
+    auto loadA = [&](int k, double (&a)[4]) {
+        const size_t e = (size_t)(k < h ? k : h - 1) * h + ra;
+        a[0] = c.DY1m[e];
+        a[1] = c.DY1m[hh + e];
+        a[2] = c.DY2m[e];
+        a[3] = c.DY2m[hh + e];
+    };
+    const size_t so = (size_t)(sp_ < c.npencil ? sp_ : c.npencil - 1) * N;
+    double2 nvt, nvb, ngt, ngb;
+    auto fetch = [&](int k0) {
+        const int t = k0 + skk < h ? k0 + skk : h - 1;
+        nvt = sp.vn[so + t];
+        nvb = sp.vn[so + N - 1 - t];
+        ngt = sp.gn[so + t];
+        ngb = sp.gn[so + N - 1 - t];
+    };
+    constexpr int PF = 4;
+    static_assert((kKC / 4) % PF == 0, "chunks hold whole rounds of the operand ring");
+    double q[PF][4];
+    fetch(0);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int d = 0; d < PF; ++d) loadA(4 * d + ak, q[d]);
+#pragma unroll
+    for (int ci = 0; ci < kMirrorMaxH / kKC; ++ci) {
+        const int k0 = ci * kKC;
+        if (k0 >= h) break;
+        const int t = k0 + skk;
+        if (t < h) {
+            const bool live = sp_ < c.npencil;
+            sv[sq][t] = live ? nvt : c2(0, 0);
+            sv[sq][N - 1 - t] = live ? nvb : c2(0, 0);
+            sg[sq][t] = live ? ngt : c2(0, 0);
+            sg[sq][N - 1 - t] = live ? ngb : c2(0, 0);
+        }
+        __syncthreads();
+        fetch(k0 + kKC);
+        const int nst = ((h - k0 < kKC ? h - k0 : kKC) + 3) / 4;
+#pragma unroll
+        for (int st = 0; st < kKC / 4; ++st) {
+            const int d = st % PF, kk = 4 * st;
+            if (st < nst) {
+                // rows j and N-1-j, staged by this chunk (rows j >= h read row 0, a
+                // staged row, and are replaced by zeros)
+                const int j = k0 + kk + ak;
+                const bool in = j < h;
+                const int jt = in ? j : 0, jb = in ? N - 1 - j : 0;
+                double ev[2], ov[2], eg[2], og[2];
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    const double2 xt = sv[ct * 8 + bq][jt], xb = sv[ct * 8 + bq][jb];
+                    const double2 yt = sg[ct * 8 + bq][jt], yb = sg[ct * 8 + bq][jb];
+                    const double a = bc ? xt.y : xt.x, b = bc ? xb.y : xb.x;
+                    const double ga = bc ? yt.y : yt.x, gb = bc ? yb.y : yb.x;
+                    ev[ct] = in ? add(a, b) : 0.0;
+                    ov[ct] = in ? sub(a, b) : 0.0;
+                    eg[ct] = in ? add(ga, gb) : 0.0;
+                    og[ct] = in ? sub(ga, gb) : 0.0;
+                }
+                if (busy) {
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        acc[0][ct][0] = mfma4(q[d][0], ev[ct], acc[0][ct][0]);
+                        acc[0][ct][1] = mfma4(q[d][2], ev[ct], acc[0][ct][1]);
+                        acc[0][ct][2] = mfma4(q[d][0], eg[ct], acc[0][ct][2]);
+                        acc[1][ct][0] = mfma4(q[d][1], ov[ct], acc[1][ct][0]);
+                        acc[1][ct][1] = mfma4(q[d][3], ov[ct], acc[1][ct][1]);
+                        acc[1][ct][2] = mfma4(q[d][1], og[ct], acc[1][ct][2]);
+                    }
+                }
+            }
+            loadA(k0 + kk + 4 * PF + ak, q[d]);
+        }
+        // no barrier here: the next chunk's rows (t and N-1-t for the next t
+        // range) are disjoint from the rows this chunk's k steps read
+    }
+    if (!busy) return;
+    double2 *u = c.cur, *wv = c.cur + c.W, *ox = c.cur + 2 * c.W, *oz = c.cur + 3 * c.W;
+    double pkx[2], pkz[2], pk2[2], pix[2], piz[2], pir[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+        const int p = p0 + ct * 8 + bq, pq = p < c.npencil ? p : 0;
+        const int k = pq / c.NXH, i = pq % c.NXH;
+        pkx[ct] = c.kx[i];
+        pkz[ct] = c.kz[k];
+        pk2[ct] = k2_of(c, k, i);
+        const double d = pk2[ct] < 1.e-8 ? 1.0 : pk2[ct];
+        pix[ct] = __ddiv_rn(pkx[ct], d);
+        piz[ct] = __ddiv_rn(pkz[ct], d);
+        pir[ct] = __ddiv_rn(1.0, d);
+    }
+    auto store = [&](int ct, int row, double2 d1v, double2 d2v, double2 d1g) {
+        const int p = p0 + ct * 8 + bq;
+        const size_t o = (size_t)p * N + row;
+        sp.dvn[o] = d2v;
+        const double kx = pkx[ct], kz = pkz[ct], k2 = pk2[ct];
+        if (k2 < 1.e-8) return;  // the mean pencil is set by the mean-flow kernels
+        const double ix = pix[ct], iz = piz[ct], ir = pir[ct];
+        const double2 vv = sv[ct * 8 + bq][row], gg = sg[ct * 8 + bq][row];
+        u[o] = csubc(imul(ix, d1v), imul(iz, gg));
+        wv[o] = caddc(imul(ix, gg), imul(iz, d1v));
+        const double2 X = csubc(caddc(cscale(d1g, kx), cscale(d2v, kz)), cscale(vv, mul(kz, k2)));
+        const double2 Yv = caddc(csubc(cscale(d1g, kz), cscale(d2v, kx)), cscale(vv, mul(kx, k2)));
+        ox[o] = imul(ir, X);
+        oz[o] = imul(ir, Yv);
+    };
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double P1 = acc[0][ct][0][r], Q1 = acc[1][ct][0][r];
+            const double P2 = acc[0][ct][1][r], Q2 = acc[1][ct][1][r];
+            const double P3 = acc[0][ct][2][r], Q3 = acc[1][ct][2][r];
+            // row i: P + Q; row N-1-i: Q - P (DY1, odd), P - Q (DY2, even)
+            const double t1 = add(P1, Q1), t2 = add(P2, Q2), t3 = add(P3, Q3);
+            const double b1 = sub(Q1, P1), b2 = sub(P2, Q2), b3 = sub(Q3, P3);
+            const double ot1 = __shfl_xor(t1, 1), ot2 = __shfl_xor(t2, 1), ot3 = __shfl_xor(t3, 1);
+            const double ob1 = __shfl_xor(b1, 1), ob2 = __shfl_xor(b2, 1), ob3 = __shfl_xor(b3, 1);
+            const int row = w * 16 + 4 * r + (lane >> 4), p = p0 + ct * 8 + bq;
+            if (bc != 0 || row >= h || p >= c.npencil) continue;
+            store(ct, row, c2(t1, ot1), c2(t2, ot2), c2(t3, ot3));
+            if (N - 1 - row != row) store(ct, N - 1 - row, c2(b1, ob1), c2(b2, ob2), c2(b3, ob3));
+        }
+    }
+}
+__global__ __launch_bounds__(512) void k_horizontal_mirror(Chan c, StepPtrs sp) { horizontal_mirror_body(c, sp, blockIdx.x); }
+__global__ __launch_bounds__(512) void k_horizontal_mirror_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    horizontal_mirror_body(e.c, e.sp, bid);
+}
+
 // nonlinear source terms (src/solver.cpp:240-260) with DY1 (i kx Lx + i kz Lz)
 // on the matrix cores; Hg and the mean Lamb history (the k = 0 pencil of the
 // Lamb vector, src/solver.cpp:293-303) are written while staging.
@@ -1315,6 +1485,8 @@ static int blocks_tp(const Chan &c) { return (c.npencil + kTP - 1) / kTP; }
 void launch_horizontal(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s) {
     if (exact)
         hipLaunchKernelGGL(k_horizontal_exact<kTPH>, dim3((c.npencil + kTPH - 1) / kTPH), dim3(256), 0, s, c, sp);
+    else if (use_mirror(c))
+        hipLaunchKernelGGL(k_horizontal_mirror, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
     else if (c.NY <= 256)
         hipLaunchKernelGGL(k_horizontal_mfma<2>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
     else
@@ -1343,7 +1515,7 @@ int ens_key(const EnsEntry &e, int stage) {
         case ES_SOLVE: return ens_key_solve(e, &blocks);
         case ES_HELMHOLTZ: return (c.NY - 2 + 15) / 16 <= 16 ? 2 : 3;
         case ES_MEAN: return (c.NY + 63) / 64;
-        case ES_HORIZONTAL: return c.NY <= 256 ? 2 : 3;
+        case ES_HORIZONTAL: return use_mirror(c) ? 4 : c.NY <= 256 ? 2 : 3;
         case ES_NONLINEAR: return (c.NY <= 256 ? 2 : 3) * 2 + c.precision;
         default: return ens_key_fft(e, stage, &blocks);
     }
@@ -1376,7 +1548,8 @@ void launch_ens(int stage, const EnsEntry *ents, const int *first, int nm, int g
         }
         case ES_MEAN: OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL(k_mean_ens<NQ>, g, dim3(256), 0, s, ents, first, nm)); break;
         case ES_HORIZONTAL:
-            if (c.NY <= 256) hipLaunchKernelGGL(k_horizontal_mfma_ens<2>, g, dim3(512), 0, s, ents, first, nm);
+            if (use_mirror(c)) hipLaunchKernelGGL(k_horizontal_mirror_ens, g, dim3(512), 0, s, ents, first, nm);
+            else if (c.NY <= 256) hipLaunchKernelGGL(k_horizontal_mfma_ens<2>, g, dim3(512), 0, s, ents, first, nm);
             else hipLaunchKernelGGL(k_horizontal_mfma_ens<3>, g, dim3(512), 0, s, ents, first, nm);
             break;
         case ES_NONLINEAR:
