@@ -496,6 +496,146 @@ __global__ __launch_bounds__(512) void k_nonlinear_mfma_ens(const EnsEntry *__re
     nonlinear_mfma_body<RT, LC>(e.c, e.sp, bid);
 }
 
+// The same stage with DY1 as mirror halves (mixed precision, NY <= 256; see
+// k_horizontal_mirror): each chunk stages rows t and N-1-t of
+// i kx Lx + i kz Lz (writing Hg and the mean Lamb history of both), and
+// DY1 (.) = P + Q on row i, Q - P on row N-1-i, with P = A1 e and Q = B1 o.
+template <typename LC>
+__device__ __forceinline__ void nonlinear_mirror_body(const Chan &c, const StepPtrs &sp, const int bid) {
+    constexpr int PF = 4;
+    static_assert((kKC / 4) % PF == 0, "chunks hold whole rounds of the operand ring");
+    __shared__ double2 st[2][kMP][kKC + 1];  // rows t (top) and N-1-t (bottom) of a chunk
+    const int N = c.NY, h = (N + 1) / 2, ns = sp.sn, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const LC *lx = reinterpret_cast<const LC *>(c.lmb);
+    const LC *ly = lx + c.W, *lz = lx + 2 * c.W;
+    const int p0 = bid * kMP;
+    if (p0 >= c.npencil) return;
+    const bool busy = w < (h + 15) / 16;
+    const int sq = threadIdx.x / kKC, skk = threadIdx.x % kKC, spn = p0 + sq;
+    double skx = 0.0, skz = 0.0;
+    if (spn < c.npencil) {
+        skx = c.kx[spn % c.NXH];
+        skz = c.kz[spn / c.NXH];
+    }
+    const size_t so = (size_t)(spn < c.npencil ? spn : c.npencil - 1) * N;
+    LC nat, nbt, nab, nbb;
+    auto fetch = [&](int k0) {
+        const int t = k0 + skk < h ? k0 + skk : h - 1;
+        nat = lx[so + t];
+        nbt = lz[so + t];
+        nab = lx[so + N - 1 - t];
+        nbb = lz[so + N - 1 - t];
+    };
+    // one row's staged value, with Hg and the mean Lamb history written
+    auto row_term = [&](int k, LC la, LC lb) {
+        const size_t o = so + k;
+        const double2 a = c2(la.x, la.y), b = c2(lb.x, lb.y);
+        sp.hgn[o] = csubc(imul(skz, a), imul(skx, b));
+        if (spn == 0) {
+            c.means[(size_t)(M_LMBX * 3 + ns) * N + k] = a;
+            c.means[(size_t)(M_LMBZ * 3 + ns) * N + k] = b;
+        }
+        return caddc(imul(skx, a), imul(skz, b));
+    };
+    auto stage = [&](int k0) {
+        const int t = k0 + skk;
+        double2 tt = c2(0, 0), tb = c2(0, 0);
+        if (spn < c.npencil && t < h) {
+            tt = row_term(t, nat, nbt);
+            tb = row_term(N - 1 - t, nab, nbb);  // the middle row of odd N: tb = tt
+        }
+        st[0][sq][skk] = tt;
+        st[1][sq][skk] = tb;
+    };
+    owt_d4 acc[2][2];  // [parity: A1 e, B1 o][column tile]
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) acc[par][ct] = owt_d4{0.0, 0.0, 0.0, 0.0};
+    const int am = lane & 15, ak = lane >> 4, bq = (lane & 15) >> 1, bc = lane & 1;
+    const int ra = w * 16 + am < h ? w * 16 + am : h - 1;
+    const size_t hh = (size_t)h * h;
+    auto loadA = [&](int k, double (&a)[2]) {
+        const size_t e = (size_t)(k < h ? k : h - 1) * h + ra;
+        a[0] = c.DY1m[e];
+        a[1] = c.DY1m[hh + e];
+    };
+    double q[PF][2];
+    fetch(0);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int d = 0; d < PF; ++d) loadA(4 * d + ak, q[d]);
+#pragma unroll
+    for (int ci = 0; ci < kMirrorMaxH / kKC; ++ci) {
+        const int k0 = ci * kKC;
+        if (k0 >= h) break;
+        stage(k0);
+        __syncthreads();
+        fetch(k0 + kKC);
+        const int nst = ((h - k0 < kKC ? h - k0 : kKC) + 3) / 4;
+#pragma unroll
+        for (int stp = 0; stp < kKC / 4; ++stp) {
+            const int d = stp % PF, kk = 4 * stp;
+            if (stp < nst) {
+                // rows past h were staged as zeros
+                double e[2], o[2];
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    const double2 xt = st[0][ct * 8 + bq][kk + ak], xb = st[1][ct * 8 + bq][kk + ak];
+                    const double a = bc ? xt.y : xt.x, b = bc ? xb.y : xb.x;
+                    e[ct] = add(a, b);
+                    o[ct] = sub(a, b);
+                }
+                if (busy) {
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        acc[0][ct] = mfma4(q[d][0], e[ct], acc[0][ct]);
+                        acc[1][ct] = mfma4(q[d][1], o[ct], acc[1][ct]);
+                    }
+                }
+            }
+            loadA(k0 + kk + 4 * PF + ak, q[d]);
+        }
+        __syncthreads();
+    }
+    if (!busy) return;
+    double pk2[2];
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+        const int p = p0 + ct * 8 + bq, pq = p < c.npencil ? p : 0;
+        pk2[ct] = k2_of(c, pq / c.NXH, pq % c.NXH);
+    }
+#pragma unroll
+    for (int ct = 0; ct < 2; ++ct) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double P = acc[0][ct][r], Q = acc[1][ct][r];
+            const double top = add(P, Q), bot = sub(Q, P);
+            const double otop = __shfl_xor(top, 1), obot = __shfl_xor(bot, 1);
+            const int row = w * 16 + 4 * r + (lane >> 4), p = p0 + ct * 8 + bq;
+            if (bc != 0 || row >= h || p >= c.npencil) continue;
+            const double k2 = pk2[ct];
+            size_t o = (size_t)p * N + row;
+            sp.hvn[o] = csubc(cscale(ld2(ly + o), -k2), c2(top, otop));
+            if (N - 1 - row != row) {
+                o = (size_t)p * N + N - 1 - row;
+                sp.hvn[o] = csubc(cscale(ld2(ly + o), -k2), c2(bot, obot));
+            }
+        }
+    }
+}
+template <typename LC>
+__global__ __launch_bounds__(512) void k_nonlinear_mirror(Chan c, StepPtrs sp) {
+    nonlinear_mirror_body<LC>(c, sp, blockIdx.x);
+}
+template <typename LC>
+__global__ __launch_bounds__(512) void k_nonlinear_mirror_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    nonlinear_mirror_body<LC>(e.c, e.sp, bid);
+}
+
 // ---------------------------------------------------------------------------
 // omega_y of the new level, batched method (src/solver.cpp:574-603 for the
 // omega_y equation): rows 0 and N-1 are omega_y = 0, and on the interior
@@ -1516,7 +1656,7 @@ int ens_key(const EnsEntry &e, int stage) {
         case ES_HELMHOLTZ: return (c.NY - 2 + 15) / 16 <= 16 ? 2 : 3;
         case ES_MEAN: return (c.NY + 63) / 64;
         case ES_HORIZONTAL: return use_mirror(c) ? 4 : c.NY <= 256 ? 2 : 3;
-        case ES_NONLINEAR: return (c.NY <= 256 ? 2 : 3) * 2 + c.precision;
+        case ES_NONLINEAR: return use_mirror(c) ? 8 : (c.NY <= 256 ? 2 : 3) * 2 + c.precision;
         default: return ens_key_fft(e, stage, &blocks);
     }
 }
@@ -1553,7 +1693,9 @@ void launch_ens(int stage, const EnsEntry *ents, const int *first, int nm, int g
             else hipLaunchKernelGGL(k_horizontal_mfma_ens<3>, g, dim3(512), 0, s, ents, first, nm);
             break;
         case ES_NONLINEAR:
-            if (c.NY <= 256) {
+            if (use_mirror(c)) {
+                hipLaunchKernelGGL(k_nonlinear_mirror_ens<float2>, g, dim3(512), 0, s, ents, first, nm);
+            } else if (c.NY <= 256) {
                 if (c.precision == 0) hipLaunchKernelGGL((k_nonlinear_mfma_ens<2, double2>), g, dim3(512), 0, s, ents, first, nm);
                 else hipLaunchKernelGGL((k_nonlinear_mfma_ens<2, float2>), g, dim3(512), 0, s, ents, first, nm);
             } else {
@@ -1573,7 +1715,9 @@ void launch_nonlinear(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t
             hipLaunchKernelGGL((k_nonlinear_exact<kTP, float2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, sp);
     } else {
         const dim3 g((c.npencil + kMP - 1) / kMP);
-        if (c.NY <= 256) {
+        if (use_mirror(c)) {
+            hipLaunchKernelGGL(k_nonlinear_mirror<float2>, g, dim3(512), 0, s, c, sp);
+        } else if (c.NY <= 256) {
             if (c.precision == 0) hipLaunchKernelGGL((k_nonlinear_mfma<2, double2>), g, dim3(512), 0, s, c, sp);
             else hipLaunchKernelGGL((k_nonlinear_mfma<2, float2>), g, dim3(512), 0, s, c, sp);
         } else {
