@@ -195,6 +195,21 @@ void Channel::upload_operators() {
     OWT_HIP(hipMemcpyAsync((void *)c_.hQi, hb.Qinv.data(), M * M * sizeof(double), hipMemcpyHostToDevice, s_));
     OWT_HIP(hipMemcpyAsync((void *)c_.hlre, hb.lre.data(), M * sizeof(double), hipMemcpyHostToDevice, s_));
     OWT_HIP(hipMemcpyAsync((void *)c_.hlim, hb.lim.data(), M * sizeof(double), hipMemcpyHostToDevice, s_));
+    // its mirror-parity blocks (the mixed step at NY <= 256); the full
+    // decomposition serves where they are unavailable
+    c_.hmir = nullptr;
+    if (N <= 256) {
+        std::vector<double> mb;
+        try {
+            mb = helmholtz_mirror_basis(op_.DY2, (int)N);
+        } catch (const Error &) {
+            mb.clear();
+        }
+        if (!mb.empty()) {
+            c_.hmir = (const double *)dmalloc(mb.size() * sizeof(double));
+            OWT_HIP(hipMemcpyAsync((void *)c_.hmir, mb.data(), mb.size() * sizeof(double), hipMemcpyHostToDevice, s_));
+        }
+    }
     auto tx = twiddles(c_.NX2), tz = twiddles(c_.NZ2);
     std::vector<float2> txf(tx.size()), tzf(tz.size());
     for (size_t m = 0; m < tx.size(); ++m) txf[m] = make_float2((float)tx[m].x, (float)tx[m].y);

@@ -90,10 +90,14 @@ struct Chan {
     // mirror halves of DY1 and DY2 (A then B, h x h column-major each,
     // h = (NY+1)/2): the mixed-precision derivative products
     const double *DY1m, *DY2m;
+    // the omega_y solves' mirror-parity blocks (helmholtz_mirror_basis); null
+    // where their decomposition is unavailable
+    const double *hmir;
 };
 // the mixed-precision step takes the DY1 and DY2 products as mirror halves
 // (NY <= 256: one row tile of the half range per wave)
 __host__ __device__ inline bool use_mirror(const Chan &c) { return c.precision == 1 && c.NY <= 256; }
+__host__ __device__ inline bool use_mirror_g(const Chan &c) { return use_mirror(c) && c.hmir != nullptr; }
 
 // History slots of one step: level 0 is the current field, levels 1 and 2 the
 // two before it; the new level overwrites level 2 (slot sn = s[2]).

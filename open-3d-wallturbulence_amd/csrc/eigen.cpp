@@ -306,18 +306,17 @@ struct CLU {
 
 }  // namespace
 
-HelmholtzBasis helmholtz_basis(const std::vector<double> &DY2, int N) {
-    const int M = N - 2;
-    Mat D(M);
-    for (int i = 0; i < M; ++i)
-        for (int j = 0; j < M; ++j) D(i, j) = DY2[(size_t)(j + 1) * N + (i + 1)];
+// D = Q B Q^-1 of one M x M block (see the file comment); `what` names the block
+// in errors
+static HelmholtzBasis eigen_basis(const Mat &D, const char *what) {
+    const int M = D.n;
     Mat Bal = D;
     const std::vector<ld> sc = balance(Bal);
     Mat H = Bal, Z(M);
     hessenberg(H, Z);
     std::vector<cld> ev = francis(H);
     const Mat &H0 = H;
-    if ((int)ev.size() != M) throw Error(2, "eigendecomposition of DY2: eigenvalue count");
+    if ((int)ev.size() != M) throw Error(2, std::string("eigendecomposition of ") + what + ": eigenvalue count");
     // pairs first (positive imaginary part kept), then the real eigenvalues
     ld norm = 0;
     for (int i = 0; i < M; ++i) {
@@ -333,7 +332,8 @@ HelmholtzBasis helmholtz_basis(const std::vector<double> &DY2, int N) {
             reals.push_back(cld(e.real(), 0));
         }
     }
-    if ((int)(2 * pairs.size() + reals.size()) != M) throw Error(2, "eigendecomposition of DY2: unpaired complex eigenvalue");
+    if ((int)(2 * pairs.size() + reals.size()) != M)
+        throw Error(2, std::string("eigendecomposition of ") + what + ": unpaired complex eigenvalue");
     std::sort(reals.begin(), reals.end(), [](const cld &a, const cld &b) { return a.real() > b.real(); });
     HelmholtzBasis hb;
     hb.M = M;
@@ -419,8 +419,8 @@ HelmholtzBasis helmholtz_basis(const std::vector<double> &DY2, int N) {
     hb.residual = (double)(res / (norm * qn));
     hb.identity = (double)ide;
     if (!(hb.residual < 1e-15) || !(hb.identity < 1e-12))
-        throw Error(2, "eigendecomposition of DY2 failed its check (residual " + std::to_string(hb.residual) + ", identity " +
-                           std::to_string(hb.identity) + ")");
+        throw Error(2, std::string("eigendecomposition of ") + what + " failed its check (residual " +
+                           std::to_string(hb.residual) + ", identity " + std::to_string(hb.identity) + ")");
     hb.Q.resize((size_t)M * M);
     hb.Qinv.resize((size_t)M * M);
     for (size_t e = 0; e < (size_t)M * M; ++e) {
@@ -430,6 +430,55 @@ HelmholtzBasis helmholtz_basis(const std::vector<double> &DY2, int N) {
     hb.lre.assign(lre.begin(), lre.end());
     hb.lim.assign(lim.begin(), lim.end());
     return hb;
+}
+
+HelmholtzBasis helmholtz_basis(const std::vector<double> &DY2, int N) {
+    const int M = N - 2;
+    Mat D(M);
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j) D(i, j) = DY2[(size_t)(j + 1) * N + (i + 1)];
+    return eigen_basis(D, "DY2");
+}
+
+// The interior block D symmetrised, Ds = (D + J D J) / 2, maps even vectors
+// (x_j = x_{M-1-j}) to even and odd to odd. On the half coordinates j < hm
+// (even) and j < ho (odd; the middle of odd M is 0 there) it acts as
+//   DE_ij = Ds_ij + Ds_i,M-1-j  (j != middle),  DE_i,mid = Ds_i,mid,
+//   DO_ij = Ds_ij - Ds_i,M-1-j,
+// and each block is decomposed on its own.
+std::vector<double> helmholtz_mirror_basis(const std::vector<double> &DY2, int N) {
+    const int M = N - 2, hm = (M + 1) / 2, ho = M / 2;
+    auto d = [&](int i, int j) { return (ld)DY2[(size_t)(j + 1) * N + (i + 1)]; };
+    auto ds = [&](int i, int j) { return (d(i, j) + d(M - 1 - i, M - 1 - j)) / 2; };
+    Mat DE(hm), DO(ho);
+    for (int i = 0; i < hm; ++i)
+        for (int j = 0; j < hm; ++j) DE(i, j) = 2 * j == M - 1 ? ds(i, j) : ds(i, j) + ds(i, M - 1 - j);
+    for (int i = 0; i < ho; ++i)
+        for (int j = 0; j < ho; ++j) DO(i, j) = ds(i, j) - ds(i, M - 1 - j);
+    const HelmholtzBasis be = eigen_basis(DE, "DY2 (even block)"), bo = eigen_basis(DO, "DY2 (odd block)");
+    const size_t hh = (size_t)hm * hm;
+    std::vector<double> out(4 * hh + 4 * (size_t)hm, 0.0);
+    // the inverses take the fold's factor 1/2: e_j = r_j + r_{M-1-j}, o_j = r_j - r_{M-1-j}
+    for (int j = 0; j < hm; ++j)
+        for (int i = 0; i < hm; ++i) {
+            out[(size_t)j * hm + i] = 0.5 * be.Qinv[(size_t)j * hm + i];
+            out[hh + (size_t)j * hm + i] = be.Q[(size_t)j * hm + i];
+        }
+    for (int j = 0; j < ho; ++j)
+        for (int i = 0; i < ho; ++i) {
+            out[2 * hh + (size_t)j * hm + i] = 0.5 * bo.Qinv[(size_t)j * ho + i];
+            out[3 * hh + (size_t)j * hm + i] = bo.Q[(size_t)j * ho + i];
+        }
+    double *lr = out.data() + 4 * hh;
+    for (int i = 0; i < hm; ++i) {
+        lr[i] = be.lre[i];
+        lr[hm + i] = be.lim[i];
+    }
+    for (int i = 0; i < ho; ++i) {
+        lr[2 * hm + i] = bo.lre[i];
+        lr[3 * hm + i] = bo.lim[i];
+    }
+    return out;
 }
 
 }  // namespace owt

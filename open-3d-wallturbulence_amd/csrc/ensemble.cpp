@@ -115,6 +115,7 @@ void Ensemble::build_tables(int ring) {
             c.DY4 = o.c.DY4;
             c.DY1m = o.c.DY1m;
             c.DY2m = o.c.DY2m;
+            c.hmir = o.c.hmir;
             c.kx = o.c.kx;
             c.kz = o.c.kz;
             c.qw = o.c.qw;

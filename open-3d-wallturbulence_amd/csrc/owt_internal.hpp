@@ -78,6 +78,14 @@ struct HelmholtzBasis {
 };
 HelmholtzBasis helmholtz_basis(const std::vector<double> &DY2, int N);
 
+// The same decomposition split by mirror parity (the mixed-precision step):
+// the symmetrised interior block acts on even and odd vectors through two blocks
+// of the half sizes hm = (M+1)/2 and ho = M/2, decomposed each (eigen.cpp).
+// Returns, hm x hm column-major each (the odd block padded with a zero last
+// row and column when M is odd): Q_E^-1 / 2, Q_E, Q_O^-1 / 2, Q_O, then the
+// eigenvalue rows lre_E, lim_E, lre_O, lim_O (hm each, the pad 0).
+std::vector<double> helmholtz_mirror_basis(const std::vector<double> &DY2, int N);
+
 // host dense LU with partial pivoting (dgetrf semantics), column-major
 int host_lu_factor(int n, double *A, int *ipiv);
 void host_lu_solve(int n, const double *LU, const int *ipiv, double *b);

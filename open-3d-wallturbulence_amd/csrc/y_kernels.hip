@@ -825,6 +825,207 @@ __global__ __launch_bounds__(512) void k_helmholtz_g_ens(const EnsEntry *__restr
     helmholtz_g_body<RT>(e.c, e.sp, bid);
 }
 
+// omega_y of the mixed-precision step by the mirror-parity blocks of the
+// symmetrised interior DY2 (helmholtz_mirror_basis, eigen.cpp). M = N - 2,
+// hm = (M+1)/2; interior index j is row j + 1 of y. The right-hand side is
+// folded as e_j = r_j + r_{M-1-j}, o_j = r_j - r_{M-1-j} (j < hm), then
+//   Z_E = (Q_E^-1 / 2) e,  Z_O = (Q_O^-1 / 2) o,
+//   scaled by (c I - nu B_E)^-1 and (c I - nu B_O)^-1,
+//   X_E = Q_E Z_E,  X_O = Q_O Z_O,  x_j = X_E,j + X_O,j,  x_{M-1-j} = X_E,j - X_O,j:
+// four half-size products instead of two full ones. NY <= 256: wave w owns row
+// tile w of the half range with both parities; each chunk stages rows t and
+// M-1-t of the right-hand side and keeps their sum and difference.
+__device__ __forceinline__ void helmholtz_mirror_body(const Chan &c, const StepPtrs &sp, const int bid) {
+    constexpr int PF = 4;
+    static_assert((kKC / 4) % PF == 0, "chunks hold whole rounds of the operand ring");
+    __shared__ double2 st[2][kMP][kKC + 1];  // e, o of a chunk
+    extern __shared__ double szs[];          // Z_E, Z_O: 2 x 32 columns of ZS = 16 ntile + 1 rows
+    const int N = c.NY, M = N - 2, hm = (M + 1) / 2, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int p0 = bid * kMP, ntile = (hm + 15) / 16, ZS = 16 * ntile + 1;
+    if (p0 >= c.npencil) return;
+    const bool busy = w < ntile;
+    const double dt = c.dt, nu = c.nu;
+    const size_t hh = (size_t)hm * hm;
+    const double *QEi = c.hmir, *QE = QEi + hh, *QOi = QE + hh, *QO = QOi + hh, *LR = QO + hh;
+    const int sq = threadIdx.x / kKC, skk = threadIdx.x % kKC, spn = p0 + sq;
+    const size_t so = (size_t)(spn < c.npencil ? spn : c.npencil - 1) * N;
+    double2 ngt[3], nht[3], ngb[3], nhb[3];
+    auto fetch = [&](int k0) {
+        const int t = k0 + skk < hm ? k0 + skk : hm - 1;
+        const size_t jt = so + t + 1, jb = so + M - t;  // rows of interior t and M-1-t
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            ngt[l] = sp.g[l][jt];
+            nht[l] = sp.hg[l][jt];
+            ngb[l] = sp.g[l][jb];
+            nhb[l] = sp.hg[l][jb];
+        }
+    };
+    auto rhs = [&](const double2 (&g)[3], const double2 (&hg)[3]) {
+        const double2 a = caddc(caddc(cscale(g[0], AL0), cscale(g[1], AL1)), cscale(g[2], AL2));
+        const double2 h = caddc(caddc(cscale(hg[0], BE0), cscale(hg[1], BE1)), cscale(hg[2], BE2));
+        return caddc(h, cdiv(a, dt));
+    };
+    auto stage = [&](int k0) {
+        const int t = k0 + skk;
+        double2 e = c2(0, 0), o = c2(0, 0);
+        if (spn < c.npencil && t < hm) {
+            const double2 rt = rhs(ngt, nht), rb = rhs(ngb, nhb);  // the middle of odd M: o = 0
+            e = caddc(rt, rb);
+            o = csubc(rt, rb);
+        }
+        st[0][sq][skk] = e;
+        st[1][sq][skk] = o;
+    };
+    owt_d4 acc[2][2];  // [parity][column tile]
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) acc[par][ct] = owt_d4{0.0, 0.0, 0.0, 0.0};
+    const int am = lane & 15, ak = lane >> 4, bq = (lane & 15) >> 1, bc = lane & 1;
+    const int ra = w * 16 + am < hm ? w * 16 + am : hm - 1;
+    auto loadA = [&](const double *AE, const double *AO, int k, double (&a)[2]) {
+        const size_t e = (size_t)(k < hm ? k : hm - 1) * hm + ra;
+        a[0] = AE[e];
+        a[1] = AO[e];
+    };
+    double q[PF][2];
+    // ---- Z = Q^-1 (e, o) / 2 ----
+    fetch(0);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int d = 0; d < PF; ++d) loadA(QEi, QOi, 4 * d + ak, q[d]);
+#pragma unroll
+    for (int ci = 0; ci < kMirrorMaxH / kKC; ++ci) {
+        const int k0 = ci * kKC;
+        if (k0 >= hm) break;
+        stage(k0);
+        __syncthreads();
+        fetch(k0 + kKC);
+        const int nst = ((hm - k0 < kKC ? hm - k0 : kKC) + 3) / 4;
+#pragma unroll
+        for (int stp = 0; stp < kKC / 4; ++stp) {
+            const int d = stp % PF, kk = 4 * stp;
+            if (stp < nst) {
+                double be[2], bo[2];
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    const double2 xe = st[0][ct * 8 + bq][kk + ak], xo = st[1][ct * 8 + bq][kk + ak];
+                    be[ct] = bc ? xe.y : xe.x;
+                    bo[ct] = bc ? xo.y : xo.x;
+                }
+                if (busy) {
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        acc[0][ct] = mfma4(q[d][0], be[ct], acc[0][ct]);
+                        acc[1][ct] = mfma4(q[d][1], bo[ct], acc[1][ct]);
+                    }
+                }
+            }
+            loadA(QEi, QOi, k0 + kk + 4 * PF + ak, q[d]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int d = 0; d < PF; ++d) loadA(QE, QO, 4 * d + ak, q[d]);
+    // ---- Z <- (c I - nu B)^-1 Z per parity, into LDS ----
+    if (busy) {
+        double cp[2];
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            const int p = p0 + ct * 8 + bq, pq = p < c.npencil ? p : 0;
+            cp[ct] = add(mul(nu, k2_of(c, pq / c.NXH, pq % c.NXH)), __ddiv_rn(G0, dt));
+        }
+#pragma unroll
+        for (int par = 0; par < 2; ++par) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = w * 16 + 4 * r + (lane >> 4);
+                const int ic = i < hm ? i : hm - 1;
+                const double a = LR[2 * par * hm + ic], b = LR[(2 * par + 1) * hm + ic];
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    const double y = acc[par][ct][r];
+                    const double yo = __shfl_xor(y, 16);  // row i ^ 1 of the same column
+                    const double cc = sub(cp[ct], mul(nu, a));
+                    double z;
+                    if (b == 0.0) {
+                        z = __ddiv_rn(y, cc);
+                    } else {
+                        const double nb = mul(nu, b), det = add(mul(cc, cc), mul(nb, nb));
+                        z = (i & 1) ? __ddiv_rn(sub(mul(cc, y), mul(nb, yo)), det) : __ddiv_rn(add(mul(cc, y), mul(nb, yo)), det);
+                    }
+                    szs[(par * 32 + ct * 16 + am) * ZS + i] = i < hm ? z : 0.0;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) acc[par][ct] = owt_d4{0.0, 0.0, 0.0, 0.0};
+    __syncthreads();
+    // ---- X = Q Z ----
+    const int nk = (hm + 3) / 4;  // k steps; Z rows hm..4 nk - 1 are zero
+#pragma unroll
+    for (int ci = 0; ci < kMirrorMaxH / kKC; ++ci) {
+        const int k0 = ci * kKC;
+        if (k0 >= hm) break;
+#pragma unroll
+        for (int stp = 0; stp < kKC / 4; ++stp) {
+            const int d = stp % PF, kk = k0 + 4 * stp;
+            if (kk < 4 * nk) {
+                double be[2], bo[2];
+#pragma unroll
+                for (int ct = 0; ct < 2; ++ct) {
+                    be[ct] = szs[(ct * 16 + am) * ZS + kk + ak];
+                    bo[ct] = szs[(32 + ct * 16 + am) * ZS + kk + ak];
+                }
+                if (busy) {
+#pragma unroll
+                    for (int ct = 0; ct < 2; ++ct) {
+                        acc[0][ct] = mfma4(q[d][0], be[ct], acc[0][ct]);
+                        acc[1][ct] = mfma4(q[d][1], bo[ct], acc[1][ct]);
+                    }
+                }
+            }
+            loadA(QE, QO, kk + 4 * PF + ak, q[d]);
+        }
+    }
+    // store omega_y: interior rows j + 1 and M - j from X, wall rows 0
+    if (busy) {
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double XE = acc[0][ct][r], XO = acc[1][ct][r];
+                const double top = add(XE, XO), bot = sub(XE, XO);
+                const double otop = __shfl_xor(top, 1), obot = __shfl_xor(bot, 1);
+                const int i = w * 16 + 4 * r + (lane >> 4), p = p0 + ct * 8 + bq;
+                if (bc != 0 || i >= hm || p >= c.npencil) continue;
+                sp.gn[(size_t)p * N + i + 1] = c2(top, otop);
+                if (M - 1 - i != i) sp.gn[(size_t)p * N + M - i] = c2(bot, obot);
+            }
+        }
+    }
+    if (threadIdx.x < 2 * kMP) {
+        const int p = p0 + (threadIdx.x >> 1);
+        if (p < c.npencil) sp.gn[(size_t)p * N + ((threadIdx.x & 1) ? N - 1 : 0)] = c2(0, 0);
+    }
+}
+__global__ __launch_bounds__(512) void k_helmholtz_mirror(Chan c, StepPtrs sp) { helmholtz_mirror_body(c, sp, blockIdx.x); }
+__global__ __launch_bounds__(512) void k_helmholtz_mirror_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    helmholtz_mirror_body(e.c, e.sp, bid);
+}
+// Z of the mirror kernel: 2 x 32 columns of 16 ntile + 1 rows
+static size_t helmholtz_mirror_lds(int ny) {
+    const int hm = (ny - 2 + 1) / 2, ntile = (hm + 15) / 16;
+    return sizeof(double) * 64 * (size_t)(16 * ntile + 1);
+}
+
 // ---------------------------------------------------------------------------
 // The plain method's derivative stages in the reference's arithmetic: each
 // mat-vec in column order (src/linalg.hpp:434-453) with every product and sum
@@ -1634,6 +1835,11 @@ void launch_horizontal(const Chan &c, const StepPtrs &sp, bool exact, hipStream_
 }
 
 void launch_helmholtz_g(const Chan &c, const StepPtrs &sp, hipStream_t s) {
+    if (use_mirror_g(c)) {
+        hipLaunchKernelGGL(k_helmholtz_mirror, dim3((c.npencil + kMP - 1) / kMP), dim3(512), helmholtz_mirror_lds(c.NY), s, c,
+                           sp);
+        return;
+    }
     const int M = c.NY - 2, ntile = (M + 15) / 16;
     const size_t lds = sizeof(double) * 2 * kMP * (16 * ntile + 1);
     const dim3 g((c.npencil + kMP - 1) / kMP);
@@ -1653,7 +1859,7 @@ int ens_key(const EnsEntry &e, int stage) {
     const Chan &c = e.c;
     switch (stage) {
         case ES_SOLVE: return ens_key_solve(e, &blocks);
-        case ES_HELMHOLTZ: return (c.NY - 2 + 15) / 16 <= 16 ? 2 : 3;
+        case ES_HELMHOLTZ: return use_mirror_g(c) ? 4 : (c.NY - 2 + 15) / 16 <= 16 ? 2 : 3;
         case ES_MEAN: return (c.NY + 63) / 64;
         case ES_HORIZONTAL: return use_mirror(c) ? 4 : c.NY <= 256 ? 2 : 3;
         case ES_NONLINEAR: return use_mirror(c) ? 8 : (c.NY <= 256 ? 2 : 3) * 2 + c.precision;
@@ -1680,6 +1886,10 @@ void launch_ens(int stage, const EnsEntry *ents, const int *first, int nm, int g
     switch (stage) {
         case ES_SOLVE: launch_ens_solve(ents, first, nm, grid, rep, s); break;
         case ES_HELMHOLTZ: {
+            if (use_mirror_g(c)) {
+                hipLaunchKernelGGL(k_helmholtz_mirror_ens, g, dim3(512), helmholtz_mirror_lds(ny_max), s, ents, first, nm);
+                break;
+            }
             const int ntile = (ny_max - 2 + 15) / 16;
             const size_t lds = sizeof(double) * 2 * kMP * (16 * ntile + 1);
             if ((c.NY - 2 + 15) / 16 <= 16) hipLaunchKernelGGL(k_helmholtz_g_ens<2>, g, dim3(512), lds, s, ents, first, nm);
