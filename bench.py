@@ -63,19 +63,22 @@ def stage_bytes(W, I, es):
     }
 
 
-# the kernel of each stage in the mixed-precision batched step
-STAGE_KERNEL = {"solve": "k_solve_inv", "helmholtz": "k_helmholtz_g", "horizontal": "k_horizontal_mfma",
+# the kernel of each stage in the mixed-precision batched step at NY <= 256
+# (the derivative products and both solves in mirror form)
+STAGE_KERNEL = {"solve": "k_solve_mirror", "helmholtz": "k_helmholtz_mirror", "horizontal": "k_horizontal_mirror",
                 "mean": "k_mean", "z_inverse": "k_z_inverse", "x_lamb": "k_x_lamb", "z_forward": "k_z_forward",
-                "nonlinear": "k_nonlinear_mfma"}
+                "nonlinear": "k_nonlinear_mirror"}
 
 
 def operator_bytes(N, nfac):
-    """Bytes of the stored explicit inverses of Av one solve launch streams at
-    least once: nfac distinct k^2 x N^2 doubles (operator data, reported beside
-    the roofline; a job streams its inverse once for up to two +-kz systems, so
-    an inverse shared by more systems is read more than once; the omega_y solves
-    read two shared (N-2)^2 matrices)."""
-    return nfac * N * N * 8
+    """Bytes of the stored v-solve inverses one mixed-precision solve launch
+    streams at least once: per distinct k^2 the mirror form (two h x h halves,
+    h = (N+1)/2, a rank-2 correction of 2N + 4h doubles; operator data, reported
+    beside the roofline; a job streams its operator once for up to two +-kz
+    systems, so one shared by more systems is read more than once; the omega_y
+    solves read their shared eigenbases)."""
+    h = (N + 1) // 2
+    return nfac * (2 * h * h + 2 * N + 4 * h) * 8
 
 
 def cpu_baseline(threads, budget_s=20.0):

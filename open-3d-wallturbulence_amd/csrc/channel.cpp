@@ -57,6 +57,8 @@ void *Channel::dmalloc(size_t bytes) {
 Channel::Channel(const Params &p, int device, bool verbose, hipStream_t stream) : P_(p), device_(device), verbose_(verbose) {
     const char *fp = std::getenv("OWT_FACTOR_PANEL");
     force_panel_ = fp && fp[0] == '1';
+    const char *fm = std::getenv("OWT_MIRROR_SOLVE");
+    mirror_solve_ = !(fm && fm[0] == '0');
     int ndev = 0;
     OWT_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) throw Error(3, "no HIP device " + std::to_string(device));
@@ -366,6 +368,22 @@ void Channel::invert_operators() {
         xscratch_ = (double *)dmalloc((size_t)xchunk_ * per);
     }
     launch_invert(c_, Fb_, Pb_, nfac_, Xb_, xscratch_, xchunk_, s_);
+    // mirror form (k_mirror_inverse): half the bytes per step for the mixed
+    // step's v solves; the full inverses stay for FP64. The environment variable
+    // OWT_MIRROR_SOLVE=0, read when the channel is created, keeps the mixed step
+    // on the full inverses.
+    if (mirror_solve_ && c_.NY <= 256 && !UV_) {
+        std::vector<double> uv;
+        if (mirror_lowrank(op_.DY4, c_.NY, c_.nu, uv)) {
+            UV_ = (double *)dmalloc(uv.size() * sizeof(double));
+            OWT_HIP(hipMemcpy(UV_, uv.data(), uv.size() * sizeof(double), hipMemcpyHostToDevice));
+            Xm_ = (double *)dmalloc((size_t)nfac_ * mirror_inv_stride(c_.NY) * sizeof(double));
+        }
+    }
+    if (Xm_) {
+        launch_mirror_inverse(c_, Xb_, nfac_, UV_, Xm_, s_);
+        c_.Xm = Xm_;
+    }
 }
 
 // the plain method's per-pencil factor set (refilled every step)

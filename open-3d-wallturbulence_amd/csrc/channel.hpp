@@ -190,6 +190,9 @@ class Channel {
     // and the scratch of their computation (xchunk_ matrices)
     double *Xb_ = nullptr, *xscratch_ = nullptr;
     int xchunk_ = 0;
+    // the same inverses in mirror form for the mixed-precision step (NY <= 256),
+    // and the rank-2 factors U, V of the odd part of Av they are built from
+    double *Xm_ = nullptr, *UV_ = nullptr;
     // all Ag factors, made only when requested (get_factor) with bops_ == 1
     double *Fag_ = nullptr;
     int *Pag_ = nullptr;
@@ -197,6 +200,7 @@ class Channel {
     double *fscratch_ = nullptr;
     int fchunk_ = 0;
     bool force_panel_ = false;
+    bool mirror_solve_ = true;
     // batched factor tables: one factor per distinct k^2; jobs ({factor, system,
     // system or -1}: systems sharing a factor, two at a time), frep (factor -> a
     // system); fmap_host_ (system -> factor)

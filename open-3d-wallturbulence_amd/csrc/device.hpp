@@ -93,11 +93,21 @@ struct Chan {
     // the omega_y solves' mirror-parity blocks (helmholtz_mirror_basis); null
     // where their decomposition is unavailable
     const double *hmir;
+    // the v-solve inverses in mirror form (k_mirror_inverse), per distinct k^2
+    // mirror_inv_stride(NY) doubles; null where unavailable
+    const double *Xm;
 };
+// per k^2: A and B (h x h column-major each, h = (NY+1)/2), P (NY x 2
+// column-major), then Qe_0, Qo_0, Qe_1, Qo_1 (h each)
+__host__ __device__ inline size_t mirror_inv_stride(int N) {
+    const size_t h = (N + 1) / 2;
+    return 2 * h * h + 2 * (size_t)N + 4 * h;
+}
 // the mixed-precision step takes the DY1 and DY2 products as mirror halves
 // (NY <= 256: one row tile of the half range per wave)
 __host__ __device__ inline bool use_mirror(const Chan &c) { return c.precision == 1 && c.NY <= 256; }
 __host__ __device__ inline bool use_mirror_g(const Chan &c) { return use_mirror(c) && c.hmir != nullptr; }
+__host__ __device__ inline bool use_mirror_v(const Chan &c) { return use_mirror(c) && c.Xm != nullptr; }
 
 // History slots of one step: level 0 is the current field, levels 1 and 2 the
 // two before it; the new level overwrites level 2 (slot sn = s[2]).
@@ -200,7 +210,11 @@ void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int 
 // (bops 1), refined once; scratch holds chunk matrices
 void launch_invert(const Chan &c, const double *F, const int *P, int nfac, double *X, double *scratch,
                    int chunk, hipStream_t s);
-// the batched method's v solves with the inverses X of Av (one per factor index)
+// mirror form of nfac inverses X (mirror_inv_stride doubles each into Xm) from
+// the rank-2 factors UV of the odd part of Av (mirror_lowrank); NY <= 256
+void launch_mirror_inverse(const Chan &c, const double *X, int nfac, const double *UV, double *Xm, hipStream_t s);
+// the batched method's v solves with the inverses X of Av (one per factor index);
+// in mirror form from c.Xm where use_mirror_v(c)
 void launch_solve_inv(const Chan &c, const StepPtrs &sp, const double *X, const int4 *jobs, int njobs, hipStream_t s);
 // omega_y of the new level by the eigendecomposition of DY2 (batched method)
 void launch_helmholtz_g(const Chan &c, const StepPtrs &sp, hipStream_t s);

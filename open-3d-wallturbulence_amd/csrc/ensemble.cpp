@@ -116,6 +116,7 @@ void Ensemble::build_tables(int ring) {
             c.DY1m = o.c.DY1m;
             c.DY2m = o.c.DY2m;
             c.hmir = o.c.hmir;
+            c.Xm = o.c.Xm;
             c.kx = o.c.kx;
             c.kz = o.c.kz;
             c.qw = o.c.qw;

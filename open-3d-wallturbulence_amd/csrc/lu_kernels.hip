@@ -863,6 +863,289 @@ __global__ __launch_bounds__(64 * JPB) void k_solve_inv_ens(const EnsEntry *__re
 }
 
 // ---------------------------------------------------------------------------
+// v solves in mirror form (the mixed-precision step, NY <= 256).
+// With R = diag(1, .., 1, -1, 1) (row N-2 negated), R Av = S + U V^T where S
+// commutes with the reflection J (j -> N-1-j) to the rounding of DY1, DY2 and
+// U V^T is the rank-2 odd part that the reference's DY4 carries (host
+// mirror_lowrank). With X' = X R (X = Av^-1) the Woodbury identity gives
+//   S^-1 = X' + P Q,  P = X' U,  Q = (I - V^T X' U)^-1 V^T X',
+// so X' = S^-1 - P Q. A right-hand side has b_{N-2} = 0, so X b = X' b =
+// Ss b - P (Q b) with Ss = (S^-1 + J S^-1 J) / 2: the symmetrised inverse in
+// mirror halves (two h x h blocks, h = (N+1)/2, as mirror_halves builds them)
+// and a rank-2 correction, 2 h^2 + 2N + 4h doubles per k^2 instead of N^2.
+// Symmetrising drops the odd part of S^-1, which is the rounding of X
+// amplified by the operator's conditioning (about 1e-9 of the largest entry
+// at the Re_tau 550 grid); x then differs from the LU substitution by an
+// amount of the order of the substitution's own rounding.
+// ---------------------------------------------------------------------------
+// One block per k^2. Xm per k^2: A, B (h x h column-major), P (N x 2
+// column-major), then the folded rows of Q: Qe_m[j] = (Q_mj + Q_m,N-1-j) / 2,
+// Qo_m[j] = (Q_mj - Q_m,N-1-j) / 2 (middle j of odd N: Q_mj / 2 and 0), so that
+// (Q b)_m = sum_{j<h} Qe_m[j] e_j + Qo_m[j] o_j with e_j = b_j + b_{N-1-j},
+// o_j = b_j - b_{N-1-j}.
+__global__ __launch_bounds__(256) void k_mirror_inverse(const double *X, int N, int nf, const double *UV, double *Xm) {
+    __shared__ double sU[2][256], sV[2][256], sP[2][256], sG[2][256], sQ[2][256];
+    __shared__ double sC[4];
+    const int f = blockIdx.x, t = threadIdx.x;
+    if (f >= nf) return;
+    const int h = (N + 1) / 2;
+    const size_t hh = (size_t)h * h;
+    const double *Xo = X + (size_t)f * N * N;
+    double *Out = Xm + (size_t)f * mirror_inv_stride(N);
+    auto xp = [&](int a, int b) -> double {
+        const double x = Xo[(size_t)b * N + a];
+        return b == N - 2 ? -x : x;
+    };
+    for (int i = t; i < N; i += 256) {
+        sU[0][i] = UV[i];
+        sU[1][i] = UV[N + i];
+        sV[0][i] = UV[2 * N + i];
+        sV[1][i] = UV[3 * N + i];
+    }
+    __syncthreads();
+    // P = X' U (row i per thread, columns of X read coalesced across threads)
+    for (int i = t; i < N; i += 256) {
+        double p0 = 0.0, p1 = 0.0;
+        for (int j = 0; j < N; ++j) {
+            const double x = xp(i, j);
+            p0 = fma(x, sU[0][j], p0);
+            p1 = fma(x, sU[1][j], p1);
+        }
+        sP[0][i] = p0;
+        sP[1][i] = p1;
+    }
+    // G = V^T X' (column j per wave, rows across the lanes, then a wave sum)
+    const int lane = t & 63, wv = t >> 6;
+    for (int j = wv; j < N; j += 4) {
+        double g0 = 0.0, g1 = 0.0;
+        for (int i = lane; i < N; i += 64) {
+            const double x = xp(i, j);
+            g0 = fma(sV[0][i], x, g0);
+            g1 = fma(sV[1][i], x, g1);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            g0 += __shfl_xor(g0, o, 64);
+            g1 += __shfl_xor(g1, o, 64);
+        }
+        if (lane == 0) {
+            sG[0][j] = g0;
+            sG[1][j] = g1;
+        }
+    }
+    __syncthreads();
+    // C = I - G U (2 x 2)
+    if (t < 4) {
+        const int m = t >> 1, n = t & 1;
+        double acc = 0.0;
+        for (int j = 0; j < N; ++j) acc = fma(sG[m][j], sU[n][j], acc);
+        sC[t] = (m == n ? 1.0 : 0.0) - acc;
+    }
+    __syncthreads();
+    const double det = sC[0] * sC[3] - sC[1] * sC[2];
+    const double m00 = sC[3] / det, m01 = -sC[1] / det, m10 = -sC[2] / det, m11 = sC[0] / det;
+    for (int j = t; j < N; j += 256) {
+        sQ[0][j] = m00 * sG[0][j] + m01 * sG[1][j];
+        sQ[1][j] = m10 * sG[0][j] + m11 * sG[1][j];
+    }
+    __syncthreads();
+    // Ss = (S^-1 + J S^-1 J) / 2 in mirror halves, S^-1 = X' + P Q
+    auto sinv = [&](int a, int b) { return fma(sP[1][a], sQ[1][b], fma(sP[0][a], sQ[0][b], xp(a, b))); };
+    for (size_t e = t; e < hh; e += 256) {
+        const int j = (int)(e / h), i = (int)(e % h);
+        const double ms = (sinv(i, j) + sinv(N - 1 - i, N - 1 - j)) * 0.5;
+        double a, b;
+        if (2 * j == N - 1) {
+            a = ms * 0.5;
+            b = 0.0;
+        } else {
+            const double mr = (sinv(i, N - 1 - j) + sinv(N - 1 - i, j)) * 0.5;
+            a = (ms + mr) * 0.5;
+            b = (ms - mr) * 0.5;
+        }
+        Out[e] = a;
+        Out[hh + e] = b;
+    }
+    double *Po = Out + 2 * hh, *Qf = Po + 2 * N;
+    for (int i = t; i < N; i += 256) {
+        Po[i] = sP[0][i];
+        Po[N + i] = sP[1][i];
+    }
+    for (int j = t; j < h; j += 256) {
+        const int r = N - 1 - j;
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const bool mid = r == j;
+            Qf[2 * m * h + j] = mid ? sQ[m][j] * 0.5 : (sQ[m][j] + sQ[m][r]) * 0.5;
+            Qf[(2 * m + 1) * h + j] = mid ? 0.0 : (sQ[m][j] - sQ[m][r]) * 0.5;
+        }
+    }
+}
+
+// One wave per batched job (as solve_inv_body), with the inverse in mirror
+// form: the right-hand sides of rows i and N-1-i are folded into e and o as
+// they are formed (LDS, h rows x 8 real columns each) and into the rank-2 sums
+// t = Q b; lane l then accumulates P_i = sum_j A_ij e_j and Q_i = sum_j B_ij o_j
+// for rows i = l + 64q < h, the columns of A and B read coalesced eight ahead,
+// and writes x_i = P_i + Q_i - (P U-part)_i, x_{N-1-i} = P_i - Q_i - (...)_{N-1-i}.
+template <int NQH, int JPB>
+__device__ __forceinline__ void solve_mirror_body(const Chan &c, const StepPtrs &sp, const double *Xm, const int4 *jobs,
+                                                  int njobs, const int bid) {
+    __shared__ double se[JPB][64 * NQH][8], so[JPB][64 * NQH][8], st[JPB][2][8];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nb = (njobs + JPB - 1) / JPB, per = (nb + 7) / 8;
+    const int blk = (bid % 8) * per + bid / 8;  // XCD-contiguous jobs
+    if (blk >= nb) return;
+    const int job = blk * JPB + w;
+    const int N = c.NY, h = (N + 1) / 2;
+    if (job >= njobs) return;
+    const int4 jb = jobs[job];
+    int pc[4];
+    job_pencils(c, jb, pc);
+    const int kz = jb.y / c.NXH, ix = jb.y % c.NXH;
+    const double dt = c.dt, k2dt = __ddiv_rn(k2_of(c, kz, ix), dt);
+    const size_t hh = (size_t)h * h;
+    const double *Am = Xm + (size_t)jb.x * mirror_inv_stride(N), *Bm = Am + hh, *Pm = Bm + hh, *Qf = Pm + 2 * N;
+    auto rhs_at = [&](int r, int pp) -> double2 {
+        const size_t o = (size_t)(pc[pp] >= 0 ? pc[pp] : pc[0]) * N + r;
+        const double2 a = caddc(caddc(cscale(sp.v[0][o], AL0), cscale(sp.v[1][o], AL1)), cscale(sp.v[2][o], AL2));
+        const double2 hv = caddc(caddc(cscale(sp.hv[0][o], BE0), cscale(sp.hv[1][o], BE1)), cscale(sp.hv[2][o], BE2));
+        const double2 d = caddc(caddc(cscale(sp.dv[0][o], AL0), cscale(sp.dv[1][o], AL1)), cscale(sp.dv[2][o], AL2));
+        double2 rhs = caddc(csubc(hv, cscale(a, k2dt)), cdiv(d, dt));
+        if (r <= 1 || r >= N - 2 || pc[pp] < 0) rhs = c2(0, 0);
+        return rhs;
+    };
+    double(*be)[8] = se[w];
+    double(*bo)[8] = so[w];
+    // rank-2 sums over the lane's rows, t[m][k]
+    double ts[2][8];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ts[m][k] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQH; ++q) {
+        const int i = lane + 64 * q;
+        if (i >= h) continue;
+        const int r = N - 1 - i;
+        const double qe0 = Qf[i], qo0 = Qf[h + i], qe1 = Qf[2 * h + i], qo1 = Qf[3 * h + i];
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) {
+            const double2 bi = rhs_at(i, pp), br = rhs_at(r, pp);
+            const double er = bi.x + br.x, ei = bi.y + br.y;
+            const double orr = r == i ? 0.0 : bi.x - br.x, oi = r == i ? 0.0 : bi.y - br.y;
+            be[i][2 * pp] = er;
+            be[i][2 * pp + 1] = ei;
+            bo[i][2 * pp] = orr;
+            bo[i][2 * pp + 1] = oi;
+            ts[0][2 * pp] = fma(qo0, orr, fma(qe0, er, ts[0][2 * pp]));
+            ts[0][2 * pp + 1] = fma(qo0, oi, fma(qe0, ei, ts[0][2 * pp + 1]));
+            ts[1][2 * pp] = fma(qo1, orr, fma(qe1, er, ts[1][2 * pp]));
+            ts[1][2 * pp + 1] = fma(qo1, oi, fma(qe1, ei, ts[1][2 * pp + 1]));
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) ts[m][k] += __shfl_xor(ts[m][k], o, 64);
+    // parked in LDS until the epilogue (registers go to the product loop)
+    if (lane == 0)
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) st[w][m][k] = ts[m][k];
+    wave_sync();
+    double pa[8][NQH], qa[8][NQH];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+        for (int q = 0; q < NQH; ++q) pa[k][q] = qa[k][q] = 0.0;
+    int rc[NQH];
+#pragma unroll
+    for (int q = 0; q < NQH; ++q) rc[q] = lane + 64 * q < h ? lane + 64 * q : h - 1;
+    // columns in chunks of CJ, the next chunk's loads issued before this one's use
+    constexpr int CJ = 4;
+    double xa[CJ][2][NQH], xb[CJ][2][NQH];
+    auto load = [&](int j0, double (&xc)[CJ][2][NQH]) {
+#pragma unroll
+        for (int cc = 0; cc < CJ; ++cc) {
+            const int j = j0 + cc < h ? j0 + cc : h - 1;
+#pragma unroll
+            for (int q = 0; q < NQH; ++q) {
+                xc[cc][0][q] = Am[(size_t)j * h + rc[q]];
+                xc[cc][1][q] = Bm[(size_t)j * h + rc[q]];
+            }
+        }
+    };
+    auto apply = [&](int j0, double (&xc)[CJ][2][NQH]) {
+#pragma unroll
+        for (int cc = 0; cc < CJ; ++cc) {
+            const int j = j0 + cc;
+            if (j >= h) break;
+            double ej[8], oj[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                ej[k] = be[j][k];
+                oj[k] = bo[j][k];
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+#pragma unroll
+                for (int q = 0; q < NQH; ++q) {
+                    pa[k][q] = fma(xc[cc][0][q], ej[k], pa[k][q]);
+                    qa[k][q] = fma(xc[cc][1][q], oj[k], qa[k][q]);
+                }
+        }
+    };
+    load(0, xa);
+    for (int j0 = 0; j0 < h; j0 += 2 * CJ) {
+        if (j0 + CJ < h) load(j0 + CJ, xb);
+        apply(j0, xa);
+        if (j0 + CJ >= h) break;
+        if (j0 + 2 * CJ < h) load(j0 + 2 * CJ, xa);
+        apply(j0 + CJ, xb);
+    }
+    double2 *XN = sp.vn;
+#pragma unroll
+    for (int q = 0; q < NQH; ++q) {
+        const int i = lane + 64 * q;
+        if (i >= h) continue;
+        const int r = N - 1 - i;
+        const double p0i = Pm[i], p1i = Pm[N + i], p0r = Pm[r], p1r = Pm[N + r];
+#pragma unroll
+        for (int pp = 0; pp < 4; ++pp) {
+            if (pc[pp] < 0) continue;
+            double xi[2], xr[2];
+#pragma unroll
+            for (int ri = 0; ri < 2; ++ri) {
+                const int k = 2 * pp + ri;
+                const double t0 = st[w][0][k], t1 = st[w][1][k];
+                const double ci = fma(p1i, t1, p0i * t0);
+                const double cr = fma(p1r, t1, p0r * t0);
+                xi[ri] = (pa[k][q] + qa[k][q]) - ci;
+                xr[ri] = (pa[k][q] - qa[k][q]) - cr;
+            }
+            XN[(size_t)pc[pp] * N + i] = c2(xi[0], xi[1]);
+            if (r != i) XN[(size_t)pc[pp] * N + r] = c2(xr[0], xr[1]);
+        }
+    }
+}
+template <int NQH, int JPB>
+__global__ __launch_bounds__(64 * JPB) void k_solve_mirror(Chan c, StepPtrs sp, const double *Xm, const int4 *jobs, int njobs) {
+    solve_mirror_body<NQH, JPB>(c, sp, Xm, jobs, njobs, blockIdx.x);
+}
+template <int NQH, int JPB>
+__global__ __launch_bounds__(64 * JPB) void k_solve_mirror_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first,
+                                                               int nm) {
+    const int m = ens_member(first, nm, blockIdx.x);
+    const EnsEntry &e = ents[m];
+    const int bid = blockIdx.x - first[m];
+    solve_mirror_body<NQH, JPB>(e.c, e.sp, e.c.Xm, e.jobs, e.njobs, bid);
+}
+
+// ---------------------------------------------------------------------------
 // Dynamic pressure (no reference counterpart): the right-hand sides R of the
 // current level (y_kernels.hip) solved with the pressure operator
 //   (DY2 - k^2) P = R on rows 1..N-2,  DY1 P = R on rows 0 and N-1,
@@ -1099,10 +1382,28 @@ static int solve_jobs_per_block(const Chan &c) {
     return sel == 1 ? 1 : sel == 2 ? 4 : 2;
 }
 
+void launch_mirror_inverse(const Chan &c, const double *X, int nfac, const double *UV, double *Xm, hipStream_t s) {
+    // the kernel's LDS holds 256 rows; the channel builds the mirror form only below that
+    if (c.NY <= 256 && nfac > 0) hipLaunchKernelGGL(k_mirror_inverse, dim3(nfac), dim3(256), 0, s, X, c.NY, nfac, UV, Xm);
+}
+
+// rows per lane of the half range (h = (NY+1)/2 <= 128)
+#define OWT_NQH_DISPATCH(N, ...)                                      \
+    do {                                                               \
+        if ((N) <= 127) { constexpr int NQH = 1; __VA_ARGS__; }        \
+        else { constexpr int NQH = 2; __VA_ARGS__; }                   \
+    } while (0)
+
 void launch_solve_inv(const Chan &c, const StepPtrs &sp, const double *X, const int4 *jobs, int njobs, hipStream_t s) {
     const int jpb = solve_jobs_per_block(c);
     const int nb = (njobs + jpb - 1) / jpb;
     const dim3 g(8 * ((nb + 7) / 8)), b(64 * jpb);
+    if (use_mirror_v(c)) {
+        if (jpb == 1) OWT_NQH_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_mirror<NQH, 1>), g, b, 0, s, c, sp, c.Xm, jobs, njobs));
+        else if (jpb == 4) OWT_NQH_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_mirror<NQH, 4>), g, b, 0, s, c, sp, c.Xm, jobs, njobs));
+        else OWT_NQH_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_mirror<NQH, 2>), g, b, 0, s, c, sp, c.Xm, jobs, njobs));
+        return;
+    }
     if (jpb == 1) OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_inv<NQ, 1>), g, b, 0, s, c, sp, X, jobs, njobs));
     else if (jpb == 4) OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_inv<NQ, 4>), g, b, 0, s, c, sp, X, jobs, njobs));
     else OWT_NQ_DISPATCH(c.NY, hipLaunchKernelGGL((k_solve_inv<NQ, 2>), g, b, 0, s, c, sp, X, jobs, njobs));
@@ -1113,12 +1414,18 @@ void launch_solve_inv(const Chan &c, const StepPtrs &sp, const double *X, const 
 int ens_key_solve(const EnsEntry &e, int *blocks) {
     const int jpb = solve_jobs_per_block(e.c), nb = (e.njobs + jpb - 1) / jpb;
     *blocks = 8 * ((nb + 7) / 8);
-    return ((e.c.NY + 63) / 64) * 16 + jpb;
+    return ((e.c.NY + 63) / 64) * 16 + jpb + (use_mirror_v(e.c) ? 1000 : 0);
 }
 
 void launch_ens_solve(const EnsEntry *ents, const int *first, int nm, int grid, const EnsEntry &rep, hipStream_t s) {
     const int jpb = solve_jobs_per_block(rep.c);
     const dim3 g(grid), b(64 * jpb);
+    if (use_mirror_v(rep.c)) {
+        if (jpb == 1) OWT_NQH_DISPATCH(rep.c.NY, hipLaunchKernelGGL((k_solve_mirror_ens<NQH, 1>), g, b, 0, s, ents, first, nm));
+        else if (jpb == 4) OWT_NQH_DISPATCH(rep.c.NY, hipLaunchKernelGGL((k_solve_mirror_ens<NQH, 4>), g, b, 0, s, ents, first, nm));
+        else OWT_NQH_DISPATCH(rep.c.NY, hipLaunchKernelGGL((k_solve_mirror_ens<NQH, 2>), g, b, 0, s, ents, first, nm));
+        return;
+    }
     if (jpb == 1) OWT_NQ_DISPATCH(rep.c.NY, hipLaunchKernelGGL((k_solve_inv_ens<NQ, 1>), g, b, 0, s, ents, first, nm));
     else if (jpb == 4) OWT_NQ_DISPATCH(rep.c.NY, hipLaunchKernelGGL((k_solve_inv_ens<NQ, 4>), g, b, 0, s, ents, first, nm));
     else OWT_NQ_DISPATCH(rep.c.NY, hipLaunchKernelGGL((k_solve_inv_ens<NQ, 2>), g, b, 0, s, ents, first, nm));

@@ -274,3 +274,58 @@ std::vector<double> mirror_halves(const std::vector<double> &M, int N, int s) {
 }
 
 }  // namespace owt
+
+namespace owt {
+
+// E = -nu (DY4 - J DY4 J) / 2 with rows 0, 1, N-2, N-1 zero (those rows of Av
+// are the boundary rows, mirror images of each other once row N-2 changes
+// sign). The reference's DY4 top-row overwrite makes E rank 2; its factors
+// come from two steps of Gram-Schmidt with column pivoting in long double.
+bool mirror_lowrank(const std::vector<double> &DY4, int N, double nu, std::vector<double> &UV) {
+    if (N < 8) return false;
+    auto at = [&](int i, int j) { return (long double)DY4[(size_t)j * N + i]; };
+    std::vector<long double> E((size_t)N * N, 0.0L);
+    long double emax = 0;
+    for (int j = 0; j < N; ++j)
+        for (int i = 2; i < N - 2; ++i) {
+            const long double e = -(long double)nu * (at(i, j) - at(N - 1 - i, N - 1 - j)) / 2;
+            E[(size_t)j * N + i] = e;
+            emax = std::max(emax, std::fabs(e));
+        }
+    if (emax == 0) return false;
+    std::vector<long double> R = E, Q(2 * (size_t)N);
+    for (int m = 0; m < 2; ++m) {
+        int best = -1;
+        long double bn = 0;
+        for (int j = 0; j < N; ++j) {
+            long double s = 0;
+            for (int i = 0; i < N; ++i) s += R[(size_t)j * N + i] * R[(size_t)j * N + i];
+            if (s > bn) bn = s, best = j;
+        }
+        if (best < 0) return false;
+        const long double nrm = std::sqrt(bn);
+        for (int i = 0; i < N; ++i) Q[(size_t)m * N + i] = R[(size_t)best * N + i] / nrm;
+        for (int j = 0; j < N; ++j) {
+            long double d = 0;
+            for (int i = 0; i < N; ++i) d += Q[(size_t)m * N + i] * R[(size_t)j * N + i];
+            for (int i = 0; i < N; ++i) R[(size_t)j * N + i] -= d * Q[(size_t)m * N + i];
+        }
+    }
+    // what the two directions leave must be rounding
+    long double rmax = 0;
+    for (long double r : R) rmax = std::max(rmax, std::fabs(r));
+    if (rmax > 1e-12L * emax) return false;
+    // U = Q (orthonormal columns), V^T = U^T E
+    UV.assign(4 * (size_t)N, 0.0);
+    for (int m = 0; m < 2; ++m) {
+        for (int i = 0; i < N; ++i) UV[(size_t)m * N + i] = (double)Q[(size_t)m * N + i];
+        for (int j = 0; j < N; ++j) {
+            long double d = 0;
+            for (int i = 0; i < N; ++i) d += Q[(size_t)m * N + i] * E[(size_t)j * N + i];
+            UV[(size_t)(2 + m) * N + j] = (double)d;
+        }
+    }
+    return true;
+}
+
+}  // namespace owt

@@ -66,6 +66,14 @@ Operators build_operators(int NY, double alpha_mesh);
 // h x h column-major each.
 std::vector<double> mirror_halves(const std::vector<double> &M, int N, int s);
 
+// The part of the v operator Av (row N-2 negated) that is odd under the
+// reflection: E = -nu (DY4 - J DY4 J) / 2 on rows 2..N-3, the same for every
+// k^2 (DY1 and DY2 are mirror symmetric to rounding, the boundary rows map onto
+// each other). The reference's DY4 makes E rank 2: E = U V^T. UV receives U
+// then V (N x 2 column-major each). False if E is not rank 2 to 1e-12 of its
+// largest entry.
+bool mirror_lowrank(const std::vector<double> &DY4, int N, double nu, std::vector<double> &UV);
+
 // D = DY2's interior block (rows and columns 1..N-2) as Q B Q^-1, B block
 // diagonal: rows (2m, 2m+1), m < npair, a 2 x 2 block [[a, b], [-b, a]] of the
 // pair a +- i b (lre = a, lim = b on both rows); the other rows real eigenvalues

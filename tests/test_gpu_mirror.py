@@ -1,0 +1,102 @@
+"""The mixed-precision step's v solves with the inverses in mirror form
+(k_mirror_inverse + k_solve_mirror) against the same step with the full
+explicit inverses (OWT_MIRROR_SOLVE=0 at channel creation), and against the
+FP64 oracle.
+
+After one step from the same initial field the two mixed channels differ only
+in the v solve (everything before it is the same arithmetic on the same data),
+so the first new v level compares the two solves directly. The full inverse X
+is refined against the LU factors, so x = X b carries the LU substitution's
+rounding, which the v operator's conditioning (3e10 at the Re_tau 550 grid)
+amplifies for right-hand sides that reach its ill-conditioned directions, as
+the FP32 rounding of the Lamb vector does. The mirror form starts from X and
+drops the part of its rounding that is odd under the reflection. A CPU model of
+both forms at NY 193 (X the inverse of the LU factors, right-hand sides of a
+smooth v plus 1e-7 noise) puts each within 2e-9 of a long-double-refined
+solution, as dgetrs itself (4e-9), and the two 9e-10 apart. Bar: 1.5 times
+the grid's one-step FP64 rounding floor (tests/golden/floors.json,
+"floor_1step": how far the oracle moves after one step under one-ulp noise on
+its field, Lamb vector and solves), the convention of the parity tests. Printed
+beside it: what mixed precision itself changes at the same step (the
+full-inverse mixed channel against an FP64 channel).
+Over 10 steps the FP32 transform phase turns any difference into FP32-level
+rounding flips, so the 10-step check is against the FP64 oracle with the mixed
+path's bars: u 1e-6, v and w 2e-5 (the mixed path's distance from FP64 at 64^3
+after 20 steps is about 1e-6).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FLOORS = json.load(open(os.path.join(HERE, "golden", "floors.json")))
+KEYS = ("nu", "dt", "dkx", "dkz", "alpha_mesh", "NX", "NY", "NZ")
+# the floor grids: NY 65 (one row block of the half range), NY 160 (even, no
+# middle row), NY 193 (the Re_tau 550 wall-normal grid), NY 256 (a full second
+# row block)
+GRIDS = {name: ({k: FLOORS[name]["case"][k] for k in KEYS}, FLOORS[name]["case"]["amp"], FLOORS[name]["floor_1step"])
+         for name in ("re180_64", "case180_wall_16", "re550_wall_32", "re550_ny256_16")}
+SLACK = 1.5
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def mixed_channel(owt, p, mirror):
+    old = os.environ.get("OWT_MIRROR_SOLVE")
+    os.environ["OWT_MIRROR_SOLVE"] = "1" if mirror else "0"
+    try:
+        return owt.Channel(params=dict(p, precision=1, method=1))
+    finally:
+        if old is None:
+            del os.environ["OWT_MIRROR_SOLVE"]
+        else:
+            os.environ["OWT_MIRROR_SOLVE"] = old
+
+
+@pytest.mark.parametrize("name", list(GRIDS))
+def test_mirror_v_solve_equals_full_inverse(owt_mod, name):
+    p, amp, floor = GRIDS[name]
+    print(f"[gpu] mirror v solve {name} {p['NX']}x{p['NY']}x{p['NZ']}", flush=True)
+    a, b = mixed_channel(owt_mod, p, True), mixed_channel(owt_mod, p, False)
+    c = owt_mod.Channel(params=dict(p, precision=0, method=1))
+    try:
+        for ch in (a, b, c):
+            ch.init_fields(2, seed=0, amp=amp)
+            ch.step(1)
+        errs = {n: rel(a.field(n), b.field(n)) for n in ("v", "u", "w")}
+        prec = {n: rel(b.field(n), c.field(n)) for n in ("v", "u", "w")}
+    finally:
+        a.close()
+        b.close()
+        c.close()
+    print(f"[mirror] {name} one step, mirror vs full: " + " ".join(f"{n}={e:.2e}" for n, e in errs.items())
+          + "; mixed vs FP64: " + " ".join(f"{n}={e:.2e}" for n, e in prec.items()), flush=True)
+    for n, e in errs.items():
+        assert e <= SLACK * floor[n], (name, n, e, floor[n])
+
+
+@pytest.mark.parametrize("name", ["re180_64", "re550_wall_32"])
+def test_mirror_mixed_step_against_the_oracle(owt_mod, oracle_mod, name):
+    p, amp, _ = GRIDS[name]
+    o = oracle_mod.Oracle(p["nu"], p["dt"], p["dkx"], p["dkz"], p["alpha_mesh"], p["NX"], p["NY"], p["NZ"])
+    o.init(2, seed=0, amp=amp)
+    o.step(10)
+    print(f"[gpu] mirror mixed 10 steps {name}", flush=True)
+    a = mixed_channel(owt_mod, p, True)
+    try:
+        a.init_fields(2, seed=0, amp=amp)
+        a.step(10)
+        errs = {n: rel(a.field(n), o.field(n)) for n in ("u", "v", "w")}
+        da, do = a.diag(), o.diag()
+    finally:
+        a.close()
+    print(f"[mirror] {name} 10 steps vs oracle: " + " ".join(f"{n}={e:.2e}" for n, e in errs.items()), flush=True)
+    assert errs["u"] <= 1e-6 and errs["v"] <= 2e-5 and errs["w"] <= 2e-5, errs
+    assert np.abs(da[:, 0] - do[:, 0]).max() <= 1e-6 * np.abs(do[:, 0]).max()

@@ -171,6 +171,56 @@ def test_lu_on_step_operators_matches_lapack(oracle_mod):
             assert np.linalg.norm(x - ref) <= 1e-9 * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("NY", [65, 160, 193])
+def test_v_operator_mirror_form(oracle_mod, NY):
+    """The premise of the mixed step's mirror-form v solves: with row N-2
+    negated, Av's part that is odd under the reflection j -> N-1-j is rank 2
+    and the same for every k^2 (it comes from the reference's DY4 top-row
+    overwrite; DY1 and DY2 are mirror symmetric on the rows Av keeps), so
+    X b = Ss b - P (Q b) with Ss the symmetrised inverse of S = R Av - U V^T
+    (Woodbury). Checked against LAPACK's solve on smooth right-hand sides, and
+    against the full inverse's own error on right-hand sides carrying 1e-7
+    noise (the FP32 Lamb vector's rounding)."""
+    o = make(oracle_mod, 8, NY, 8, nu=1e-4, alpha_mesh=2.0)
+    N = NY
+    J = np.eye(N)[::-1]
+    R = np.eye(N)
+    R[N - 2, N - 2] = -1
+    y = o.vector("y")
+    rng = np.random.default_rng(NY)
+    for k2 in (0.25, 17.0, 400.0, 1e4):
+        Av, _ = assemble_Av_Ag(o, k2)
+        Ap = R @ Av
+        E = (Ap - J @ Ap @ J) / 2
+        sv = np.linalg.svd(E, compute_uv=False)
+        # two directions of about 1e-3 |Av|, the rest at Av's rounding
+        assert sv[1] >= 1e-4 * np.linalg.norm(Ap, 2) and sv[2] <= 1e-14 * np.linalg.norm(Ap, 2), sv[:4]
+        U_, s_, Vt = np.linalg.svd(E)
+        U, V = U_[:, :2], (s_[:2, None] * Vt[:2]).T
+        X = np.linalg.inv(Av)
+        Xp = X @ R
+        P = Xp @ U
+        G = V.T @ Xp
+        Q = np.linalg.solve(np.eye(2) - G @ U, G)
+        Sinv = Xp + P @ Q
+        Ss = (Sinv + J @ Sinv @ J) / 2
+        for noise in (0.0, 1e-7):
+            c = rng.standard_normal(5)
+            v = (1 - y ** 2) ** 2 * sum(c[m] * np.cos(m * np.pi * y / 2 + c[m]) for m in range(5))
+            b = Av @ v
+            b = b + noise * np.abs(b).max() * rng.standard_normal(N)
+            b[[0, 1, N - 2, N - 1]] = 0
+            # the solution to well below FP64 rounding: LU refined with long-double residuals
+            ref = np.linalg.solve(Av, b)
+            for _ in range(3):
+                r = (b.astype(np.longdouble) - Av.astype(np.longdouble) @ ref.astype(np.longdouble)).astype(np.float64)
+                ref = ref + np.linalg.solve(Av, r)
+            e = lambda x: np.linalg.norm(x - ref) / np.linalg.norm(ref)
+            xm = Ss @ b - P @ (Q @ b)
+            # as accurate as the full inverse and LAPACK's solve, to a factor
+            assert e(xm) <= max(1e-12, 10 * max(e(X @ b), e(np.linalg.solve(Av, b)))), (k2, noise, e(xm), e(X @ b))
+
+
 # ---------------------------------------------------------------------------
 # compact schemes and metrics (src/compdiff.hpp, src/solver.cpp:57-216)
 # ---------------------------------------------------------------------------
