@@ -375,14 +375,18 @@ void Channel::invert_operators() {
     if (mirror_solve_ && c_.NY <= 256 && !UV_) {
         std::vector<double> uv;
         if (mirror_lowrank(op_.DY4, c_.NY, c_.nu, uv)) {
-            UV_ = (double *)dmalloc(uv.size() * sizeof(double));
+            UV_ = (double *)dmalloc(uv.size() * sizeof(double) + sizeof(int));
             OWT_HIP(hipMemcpy(UV_, uv.data(), uv.size() * sizeof(double), hipMemcpyHostToDevice));
             Xm_ = (double *)dmalloc((size_t)nfac_ * mirror_inv_stride(c_.NY) * sizeof(double));
         }
     }
     if (Xm_) {
-        launch_mirror_inverse(c_, Xb_, nfac_, UV_, Xm_, s_);
-        c_.Xm = Xm_;
+        int *bad = (int *)(UV_ + 4 * (size_t)c_.NY), flag = 0;
+        OWT_HIP(hipMemsetAsync(bad, 0, sizeof(int), s_));
+        launch_mirror_inverse(c_, Xb_, nfac_, UV_, Xm_, bad, s_);
+        OWT_HIP(hipMemcpyAsync(&flag, bad, sizeof(int), hipMemcpyDeviceToHost, s_));
+        OWT_HIP(hipStreamSynchronize(s_));
+        c_.Xm = flag ? nullptr : Xm_;
     }
 }
 

@@ -211,8 +211,10 @@ void launch_solve(const Chan &c, const StepPtrs &sp, const double *F, const int 
 void launch_invert(const Chan &c, const double *F, const int *P, int nfac, double *X, double *scratch,
                    int chunk, hipStream_t s);
 // mirror form of nfac inverses X (mirror_inv_stride doubles each into Xm) from
-// the rank-2 factors UV of the odd part of Av (mirror_lowrank); NY <= 256
-void launch_mirror_inverse(const Chan &c, const double *X, int nfac, const double *UV, double *Xm, hipStream_t s);
+// the rank-2 factors UV of the odd part of Av (mirror_lowrank); NY <= 256.
+// *bad (zeroed by the caller) becomes nonzero if a k^2's Woodbury matrix is
+// far from the identity
+void launch_mirror_inverse(const Chan &c, const double *X, int nfac, const double *UV, double *Xm, int *bad, hipStream_t s);
 // the batched method's v solves with the inverses X of Av (one per factor index);
 // in mirror form from c.Xm where use_mirror_v(c)
 void launch_solve_inv(const Chan &c, const StepPtrs &sp, const double *X, const int4 *jobs, int njobs, hipStream_t s);
