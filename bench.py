@@ -200,6 +200,29 @@ def ensemble_throughput(owt, name, steps):
             "members_one_by_one_ms_per_step": round(seq, 4), "speedup_vs_one_by_one": round(seq / t, 2)}
 
 
+def re180_128_throughput(owt, steps, plain_steps):
+    """The Re_tau 180 128x129x128 channel in FP64: the plain method (the
+    reference's: every pencil's operators assembled and factored each step) and
+    the batched method, each timed over its own run of steps from the seeded
+    field, with the algorithmic HBM rate of the FP64 step."""
+    cfg = dict(owt.CONFIGS["re180_128"])
+    out = {"workload": "re180_128: 128x129x128 channel, Re_tau 180, box 4pi x 2 x 4pi/3", "precision": "fp64"}
+    with owt.Channel(params=cfg, precision=owt.FP64, method=owt.BATCHED) as ch:
+        ch.init_fields(2, seed=0, amp=0.15)
+        G = ch.gridpoints
+        W = ch.NZ * ch.NXH * ch.NY
+        I = ch.NZ2 * ch.NXH * ch.NY
+        ch.autotune(5)
+        t_b = ch.time_steps(steps) / steps
+        ch.set_method(owt.PLAIN)
+        t_p = ch.time_steps(plain_steps) / plain_steps
+    step_bytes = sum(stage_bytes(W, I, 16).values())
+    for name, t, n in (("batched", t_b, steps), ("plain", t_p, plain_steps)):
+        out[name] = {"steps": n, "ms_per_step": round(t, 4), "ns_per_gridpoint_step": round(t * 1e6 / G, 5),
+                     "hbm_gbps_step": round(step_bytes / (t * 1e-3) / 1e9, 1)}
+    return out
+
+
 def adjoint_check(owt, K=20, ckpt=5, first=11, last=20, seeds=range(1000, 1016), eps=1e-6):
     """Discrete adjoint of K steps at Re_tau 180, 128x129x128, FP64: J = mean friction
     force over steps first..last, gradient with respect to the initial velocity.
@@ -319,6 +342,10 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=100,
                     help="timed steps of the 32-member ensemble workload (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient check")
+    ap.add_argument("--re180-128-steps", type=int, default=1000,
+                    help="timed steps of the 128x129x128 FP64 batched run (0 skips the 128^3 runs)")
+    ap.add_argument("--re180-128-plain-steps", type=int, default=100,
+                    help="timed steps of the 128x129x128 FP64 plain-method run")
     ap.add_argument("--no-autotune", action="store_true", help="time the default launch shapes")
     ap.add_argument("--accuracy-steps", type=int, default=100,
                     help="steps of the mixed-vs-FP64 accuracy comparison (0 skips it)")
@@ -466,6 +493,9 @@ def main():
     if args.ensemble_steps > 0:
         print("[bench] ensemble", file=sys.stderr, flush=True)
         out["ensemble"] = ensemble_throughput(owt, "re180_32", args.ensemble_steps)
+    if args.re180_128_steps > 0:
+        print("[bench] re180_128", file=sys.stderr, flush=True)
+        out["re180_128"] = re180_128_throughput(owt, args.re180_128_steps, max(1, args.re180_128_plain_steps))
     if args.accuracy_steps > 0:
         print("[bench] accuracy mixed vs fp64", file=sys.stderr, flush=True)
         out["accuracy_mixed_vs_fp64"] = accuracy_mixed_vs_fp64(owt, cfg, amp, args.accuracy_steps)
