@@ -49,8 +49,9 @@ def stage_bytes(W, I, es):
     field; es: bytes per complex element in the transform phase (16 FP64, 8 FP32)."""
     c = 16
     return {
-        # the v solves: 3 levels of v, Hv and DY2 v in, v out
-        "solve": (9 * W + W) * c,
+        # the v solves: 3 levels of v, Hv and DY2 v in, v out (FP64); in mixed
+        # precision the stored levels hold DY2 v - k^2 v, so v is not read
+        "solve": ((9 if es == 16 else 6) * W + W) * c,
         # the omega_y solves: 3 levels of omega_y and Hg in, omega_y out
         "helmholtz": (6 * W + W) * c,
         # reads v, omega_y; writes u, w, omega_x, omega_z and DY2 v

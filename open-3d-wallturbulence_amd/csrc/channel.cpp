@@ -307,7 +307,7 @@ void Channel::pressure(double *out) {
         check_errors();
     }
     double2 *R = (double2 *)c_.zbuf, *X = R + W;
-    launch_pressure_rhs(c_, c_.d2v + (size_t)ring_ * W, R, s_);
+    launch_pressure_rhs(c_, c_.d2v + (size_t)ring_ * W, c_.hist + (size_t)(H_V * 3 + ring_) * W, R, s_);
     launch_pressure_solve(c_, Fq_, Pq_, jobs_, njobs_, R, X, s_);
     OWT_HIP(hipGetLastError());
     OWT_HIP(hipMemcpyAsync(out, X, W * sizeof(double2), hipMemcpyDeviceToHost, s_));
@@ -442,6 +442,10 @@ void Channel::set_precision(int precision) {
     }
     P_.precision = precision;
     c_.precision = precision;
+    // the stored wall-normal terms of the history take the new precision's form
+    launch_d2v_convert(c_, precision, s_);
+    OWT_HIP(hipGetLastError());
+    OWT_HIP(hipStreamSynchronize(s_));
     drop_graph();
 }
 
@@ -1142,7 +1146,7 @@ void Channel::checkpoint(const std::string &path) {
     if (!f) throw Error(5, "cannot open " + path + " for writing");
     CkptHeader h{};
     std::memcpy(h.magic, "OWTCKPT2", 8);
-    h.version = 1;
+    h.version = 2;  // 2: the stored wall-normal terms in their precision's form (d2v_store)
     h.NX = c_.NX;
     h.NY = c_.NY;
     h.NZ = c_.NZ;
@@ -1194,6 +1198,12 @@ void Channel::restore(const std::string &path) {
     const size_t W = c_.W, N = c_.NY;
     load(c_.hist, 12 * W * sizeof(double2));
     load(c_.d2v, 3 * W * sizeof(double2));
+    if (h.version < 2 && h.precision == 1) {
+        // written when every precision stored DY2 v
+        launch_d2v_convert(c_, 1, s_);
+        OWT_HIP(hipGetLastError());
+        OWT_HIP(hipStreamSynchronize(s_));
+    }
     load(c_.cur, 4 * W * sizeof(double2));
     load(c_.lmb, 3 * W * (c_.precision == 0 ? sizeof(double2) : sizeof(float2)));
     load(c_.means, 12 * N * sizeof(double2));

@@ -73,7 +73,7 @@ struct Chan {
     double2 *means;  // 4 x 3 x NY
     double *fric;    // fx[3], fz[3]
     double2 *msol;   // 2 x NY mean-flow solutions
-    double2 *d2v;    // 3 W: DY2 v at the 3 history levels (slots as in hist)
+    double2 *d2v;    // 3 W: DY2 v at the 3 history levels (slots as in hist); DY2 v - k^2 v in mixed precision
     const double *qw;  // NY trapezoid weights of y (pressure gauge)
     int *ctrl;
     double *diag;    // diag_cap x 3: fx_fric, bottom, upper per friction evaluation
@@ -198,7 +198,9 @@ void launch_factor(const Chan &c, double *F, int *P, int nfac, const int *frep, 
 // dynamic pressure of the current level: right-hand sides into R (W) from the
 // Lamb vector (in the transform precision) and dv = DY2 v of the level; then the
 // solves with the pressure factors (op 2 of the batched factor tables) into out
-void launch_pressure_rhs(const Chan &c, const double2 *dv, double2 *R, hipStream_t s);
+void launch_pressure_rhs(const Chan &c, const double2 *dv, const double2 *v, double2 *R, hipStream_t s);
+// Chan::d2v's three levels into the form of the given precision (1 mixed, 0 FP64)
+void launch_d2v_convert(const Chan &c, int mixed, hipStream_t s);
 void launch_pressure_solve(const Chan &c, const double *F, const int *P, const int4 *jobs, int njobs, const double2 *R,
                            double2 *out, hipStream_t s);
 // right-hand sides + solves; plain: njobs = pencils, each with its own factor;

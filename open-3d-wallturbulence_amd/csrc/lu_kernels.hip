@@ -564,7 +564,8 @@ __global__ __launch_bounds__(64 * NOPS * JPB) void k_solve(Chan c, StepPtrs sp, 
                 double2 d;
                 if (PLAIN) d = d2s[PLAIN ? q : 0];
                 else d = caddc(caddc(cscale(D0[o], AL0), cscale(D1[o], AL1)), cscale(D2[o], AL2));
-                rhs = caddc(csubc(h, cscale(a, k2dt)), cdiv(d, dt));
+                // stored levels in mixed precision already hold DY2 v - k^2 v (d2v_store)
+                rhs = (!PLAIN && c.precision == 1) ? caddc(h, cdiv(d, dt)) : caddc(csubc(h, cscale(a, k2dt)), cdiv(d, dt));
             } else {
                 rhs = caddc(h, cdiv(a, dt));
             }
@@ -777,6 +778,7 @@ __device__ __forceinline__ void solve_inv_body(const Chan &c, const StepPtrs &sp
     job_pencils(c, jb, pc);
     const int kz = jb.y / c.NXH, ix = jb.y % c.NXH;
     const double dt = c.dt, k2dt = __ddiv_rn(k2_of(c, kz, ix), dt);
+    const bool mixed = c.precision == 1;
     double(*b)[8] = sb[w];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -786,10 +788,16 @@ __device__ __forceinline__ void solve_inv_body(const Chan &c, const StepPtrs &sp
 #pragma unroll
         for (int pp = 0; pp < 4; ++pp) {
             const size_t o = (size_t)(pc[pp] >= 0 ? pc[pp] : pc[0]) * N + rr;
-            const double2 a = caddc(caddc(cscale(sp.v[0][o], AL0), cscale(sp.v[1][o], AL1)), cscale(sp.v[2][o], AL2));
             const double2 h = caddc(caddc(cscale(sp.hv[0][o], BE0), cscale(sp.hv[1][o], BE1)), cscale(sp.hv[2][o], BE2));
             const double2 d = caddc(caddc(cscale(sp.dv[0][o], AL0), cscale(sp.dv[1][o], AL1)), cscale(sp.dv[2][o], AL2));
-            double2 rhs = caddc(csubc(h, cscale(a, k2dt)), cdiv(d, dt));
+            double2 rhs;
+            if (mixed) {
+                // the stored levels hold DY2 v - k^2 v (d2v_store)
+                rhs = caddc(h, cdiv(d, dt));
+            } else {
+                const double2 a = caddc(caddc(cscale(sp.v[0][o], AL0), cscale(sp.v[1][o], AL1)), cscale(sp.v[2][o], AL2));
+                rhs = caddc(csubc(h, cscale(a, k2dt)), cdiv(d, dt));
+            }
             if (bc || r >= N || pc[pp] < 0) rhs = c2(0, 0);
             b[r][2 * pp] = rhs.x;
             b[r][2 * pp + 1] = rhs.y;
@@ -1005,16 +1013,15 @@ __device__ __forceinline__ void solve_mirror_body(const Chan &c, const StepPtrs 
     const int4 jb = jobs[job];
     int pc[4];
     job_pencils(c, jb, pc);
-    const int kz = jb.y / c.NXH, ix = jb.y % c.NXH;
-    const double dt = c.dt, k2dt = __ddiv_rn(k2_of(c, kz, ix), dt);
+    const double dt = c.dt;
     const size_t hh = (size_t)h * h;
     const double *Am = Xm + (size_t)jb.x * mirror_inv_stride(N), *Bm = Am + hh, *Pm = Bm + hh, *Qf = Pm + 2 * N;
+    // mixed precision only: the stored levels hold DY2 v - k^2 v (d2v_store)
     auto rhs_at = [&](int r, int pp) -> double2 {
         const size_t o = (size_t)(pc[pp] >= 0 ? pc[pp] : pc[0]) * N + r;
-        const double2 a = caddc(caddc(cscale(sp.v[0][o], AL0), cscale(sp.v[1][o], AL1)), cscale(sp.v[2][o], AL2));
         const double2 hv = caddc(caddc(cscale(sp.hv[0][o], BE0), cscale(sp.hv[1][o], BE1)), cscale(sp.hv[2][o], BE2));
         const double2 d = caddc(caddc(cscale(sp.dv[0][o], AL0), cscale(sp.dv[1][o], AL1)), cscale(sp.dv[2][o], AL2));
-        double2 rhs = caddc(csubc(hv, cscale(a, k2dt)), cdiv(d, dt));
+        double2 rhs = caddc(hv, cdiv(d, dt));
         if (r <= 1 || r >= N - 2 || pc[pp] < 0) rhs = c2(0, 0);
         return rhs;
     };

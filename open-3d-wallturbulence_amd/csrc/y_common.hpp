@@ -41,6 +41,13 @@ __device__ __forceinline__ double k2_of(const Chan &c, int k, int i) {
     return add(mul(a, a), mul(b, b));
 }
 
+// The stored wall-normal term of a history level (Chan::d2v) from DY2 v and v
+// of one row: DY2 v in FP64, DY2 v - k^2 v in mixed precision (the combination
+// the v right-hand side takes, so that its levels need no reads of v)
+__device__ __forceinline__ double2 d2v_store(const Chan &c, double2 d2v, double2 v, double k2) {
+    return c.precision == 1 ? csubc(d2v, cscale(v, k2)) : d2v;
+}
+
 template <typename T>
 __device__ __forceinline__ double2 ld2(const T *p) {
     T v = *p;

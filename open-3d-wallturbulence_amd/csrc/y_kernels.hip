@@ -168,7 +168,7 @@ __device__ __forceinline__ void horizontal_mfma_body(const Chan &c, const StepPt
                 if (bc != 0 || row >= N || p >= c.npencil) continue;
                 const double2 d1v = c2(m1, o1), d2v = c2(m2, o2), d1g = c2(m3, o3);
                 const size_t o = (size_t)p * N + row;
-                sp.dvn[o] = d2v;
+                sp.dvn[o] = d2v_store(c, d2v, sv[ct * 8 + bq][row], pk2[ct]);
                 const double kx = pkx[ct], kz = pkz[ct], k2 = pk2[ct];
                 if (k2 < 1.e-8) continue;  // the mean pencil is set by the mean-flow kernels
                 const double ix = pix[ct], iz = piz[ct], ir = pir[ct];
@@ -326,7 +326,7 @@ __device__ __forceinline__ void horizontal_mirror_body(const Chan &c, const Step
     auto store = [&](int ct, int row, double2 d1v, double2 d2v, double2 d1g) {
         const int p = p0 + ct * 8 + bq;
         const size_t o = (size_t)p * N + row;
-        sp.dvn[o] = d2v;
+        sp.dvn[o] = d2v_store(c, d2v, sv[ct * 8 + bq][row], pk2[ct]);
         const double kx = pkx[ct], kz = pkz[ct], k2 = pk2[ct];
         if (k2 < 1.e-8) return;  // the mean pencil is set by the mean-flow kernels
         const double ix = pix[ct], iz = piz[ct], ir = pir[ct];
@@ -1064,9 +1064,9 @@ __global__ __launch_bounds__(256) void k_horizontal_exact(Chan c, StepPtrs sp) {
             const int p = p0 + pp;
             if (p >= c.npencil) break;
             const size_t o = (size_t)p * N + row;
-            sp.dvn[o] = d2v[pp];
             const int k = p / c.NXH, i = p % c.NXH;
             const double kx = c.kx[i], kz = c.kz[k], k2 = k2_of(c, k, i);
+            sp.dvn[o] = d2v_store(c, d2v[pp], sv[pp][row], k2);
             if (k2 < 1.e-8) continue;  // the mean pencil is set by the mean-flow kernel
             const double ix = __ddiv_rn(kx, k2), iz = __ddiv_rn(kz, k2), ir = __ddiv_rn(1.0, k2);
             const double2 vv = sv[pp][row], gg = sg[pp][row];
@@ -1261,10 +1261,11 @@ __global__ __launch_bounds__(256) void k_mean_ens(const EnsEntry *__restrict__ e
 // and its y component at the walls, where v = 0:
 //   rows 0, N-1:  DY1 P = Ly + nu DY2 v
 // The k = 0 pencil's row N-1 is its gauge row (right-hand side 0). This kernel
-// writes the right-hand sides; lu_kernels.hip solves.
+// writes the right-hand sides; lu_kernels.hip solves. dv is the level's stored
+// wall-normal term (d2v_store), v its v: DY2 v = dv + k^2 v in mixed precision.
 // ---------------------------------------------------------------------------
 template <int TP, typename LC>
-__global__ __launch_bounds__(256) void k_pressure_rhs(Chan c, const double2 *dv, double2 *R) {
+__global__ __launch_bounds__(256) void k_pressure_rhs(Chan c, const double2 *dv, const double2 *v, double2 *R) {
     __shared__ double2 st[TP][kMaxNY];
     const int N = c.NY;
     const LC *lx = reinterpret_cast<const LC *>(c.lmb);
@@ -1299,7 +1300,8 @@ __global__ __launch_bounds__(256) void k_pressure_rhs(Chan c, const double2 *dv,
             const size_t o = (size_t)p * N + row;
             double2 r;
             if (row == 0 || row == N - 1) {
-                r = caddc(st[pp][row], cscale(dv[o], c.nu));
+                const double2 d2 = c.precision == 1 ? caddc(dv[o], cscale(v[o], k2_of(c, k, i))) : dv[o];
+                r = caddc(st[pp][row], cscale(d2, c.nu));
                 if (row == N - 1 && k2_of(c, k, i) < 1.e-8) r = c2(0, 0);
             } else {
                 r = caddc(caddc(imul(c.kx[i], ld2(lx + o)), imul(c.kz[k], ld2(lz + o))), acc[pp]);
@@ -1359,6 +1361,7 @@ __global__ __launch_bounds__(256) void k_vor_from_vel(Chan c) {
 }
 
 // DY2 v of the initial field (slot 0), with the mat-vec of the horizontal kernel
+// (stored as d2v_store does: minus k^2 v in mixed precision)
 template <int TP>
 __global__ __launch_bounds__(256) void k_init_d2v(Chan c) {
     __shared__ double2 sv[TP][kMaxNY];
@@ -1381,7 +1384,9 @@ __global__ __launch_bounds__(256) void k_init_d2v(Chan c) {
         }
     #pragma unroll
         for (int pp = 0; pp < TP; ++pp)
-            if (p0 + pp < c.npencil) c.d2v[(size_t)(p0 + pp) * N + row] = acc[pp];
+            if (p0 + pp < c.npencil)
+                c.d2v[(size_t)(p0 + pp) * N + row] =
+                    d2v_store(c, acc[pp], sv[pp][row], k2_of(c, (p0 + pp) / c.NXH, (p0 + pp) % c.NXH));
     }
 }
 
@@ -1937,11 +1942,27 @@ void launch_nonlinear(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t
     }
 }
 
-void launch_pressure_rhs(const Chan &c, const double2 *dv, double2 *R, hipStream_t s) {
+void launch_pressure_rhs(const Chan &c, const double2 *dv, const double2 *v, double2 *R, hipStream_t s) {
     if (c.precision == 0)
-        hipLaunchKernelGGL((k_pressure_rhs<kTP, double2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, dv, R);
+        hipLaunchKernelGGL((k_pressure_rhs<kTP, double2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, dv, v, R);
     else
-        hipLaunchKernelGGL((k_pressure_rhs<kTP, float2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, dv, R);
+        hipLaunchKernelGGL((k_pressure_rhs<kTP, float2>), dim3(blocks_tp(c)), dim3(256), 0, s, c, dv, v, R);
+}
+
+// the stored wall-normal terms of the three levels between the two precisions'
+// forms (d2v_store): mixed = 1 takes DY2 v to DY2 v - k^2 v, mixed = 0 back
+__global__ void k_d2v_convert(Chan c, int mixed) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < 3 * c.W; e += (size_t)gridDim.x * blockDim.x) {
+        const int s = (int)(e / c.W);
+        const size_t r = e % c.W;
+        const int p = (int)(r / c.NY);
+        const double k2 = k2_of(c, p / c.NXH, p % c.NXH);
+        const double2 kv = cscale(hist(c, H_V, s)[r], k2);
+        c.d2v[e] = mixed ? csubc(c.d2v[e], kv) : caddc(c.d2v[e], kv);
+    }
+}
+void launch_d2v_convert(const Chan &c, int mixed, hipStream_t s) {
+    hipLaunchKernelGGL(k_d2v_convert, dim3(2048), dim3(256), 0, s, c, mixed);
 }
 
 void launch_horizontal_adj(const Chan &c, const StepPtrs &sp, hipStream_t s) {

@@ -100,3 +100,74 @@ def test_mirror_mixed_step_against_the_oracle(owt_mod, oracle_mod, name):
     print(f"[mirror] {name} 10 steps vs oracle: " + " ".join(f"{n}={e:.2e}" for n, e in errs.items()), flush=True)
     assert errs["u"] <= 1e-6 and errs["v"] <= 2e-5 and errs["w"] <= 2e-5, errs
     assert np.abs(da[:, 0] - do[:, 0]).max() <= 1e-6 * np.abs(do[:, 0]).max()
+
+
+def test_precision_switch_converts_the_stored_wall_normal_terms(owt_mod):
+    """The stored wall-normal term of each history level is DY2 v in FP64 and
+    DY2 v - k^2 v in mixed precision (the v right-hand side's combination);
+    owt_set_precision converts all three levels. A mixed channel switched to
+    FP64 and back steps on like one never switched, within the one-step
+    rounding floor (the round trip rounds the stored terms once); an FP64 channel
+    switched to mixed steps like an FP64 channel up to the FP32 Lamb vector of
+    that one step (a missing or doubled k^2 v term would move v by O(1))."""
+    p, amp, floor = GRIDS["re180_64"]
+    print("[gpu] precision switch re180_64", flush=True)
+    a, b = mixed_channel(owt_mod, p, True), mixed_channel(owt_mod, p, True)
+    try:
+        for ch in (a, b):
+            ch.init_fields(2, seed=0, amp=amp)
+            ch.step(3)
+        a.set_precision(owt_mod.FP64)
+        a.set_precision(owt_mod.MIXED)
+        for ch in (a, b):
+            ch.step(1)
+        errs = {n: rel(a.field(n), b.field(n)) for n in ("v", "u", "w")}
+    finally:
+        a.close()
+        b.close()
+    print("[mirror] mixed -> FP64 -> mixed, one step: " + " ".join(f"{n}={e:.2e}" for n, e in errs.items()), flush=True)
+    for n, e in errs.items():
+        assert e <= SLACK * floor[n], (n, e, floor[n])
+    c = owt_mod.Channel(params=dict(p, precision=0, method=1))
+    d = owt_mod.Channel(params=dict(p, precision=0, method=1))
+    try:
+        for ch in (c, d):
+            ch.init_fields(2, seed=0, amp=amp)
+            ch.step(3)
+        c.set_precision(owt_mod.MIXED)
+        for ch in (c, d):
+            ch.step(1)
+        errs = {n: rel(c.field(n), d.field(n)) for n in ("v", "u", "w")}
+    finally:
+        c.close()
+        d.close()
+    print("[mirror] FP64 -> mixed, one step against FP64: " + " ".join(f"{n}={e:.2e}" for n, e in errs.items()), flush=True)
+    for n, e in errs.items():
+        assert e <= 1e-6, (n, e)
+
+
+def test_mixed_checkpoint_restores_exactly(owt_mod, tmp_path):
+    """A mixed-precision checkpoint (stored wall-normal terms in the mixed form)
+    restored into a fresh mixed channel and into an FP64 channel (which the
+    restore switches to mixed) continues bit for bit."""
+    p, amp, _ = GRIDS["re180_64"]
+    path = str(tmp_path / "mixed.ckpt")
+    print("[gpu] mixed checkpoint re180_64", flush=True)
+    a = mixed_channel(owt_mod, p, True)
+    try:
+        a.init_fields(2, seed=0, amp=amp)
+        a.step(3)
+        a.checkpoint(path)
+        a.step(2)
+        ref = {n: a.field(n) for n in ("v", "u", "w", "omy")}
+    finally:
+        a.close()
+    for prec in (owt_mod.MIXED, owt_mod.FP64):
+        b = owt_mod.Channel(params=dict(p, precision=prec, method=1))
+        try:
+            b.restore(path)
+            b.step(2)
+            for n, r in ref.items():
+                assert np.array_equal(b.field(n), r), (prec, n)
+        finally:
+            b.close()
