@@ -125,6 +125,7 @@ void Channel::alloc() {
     const size_t es = P_.precision == 0 ? sizeof(double2) : sizeof(float2);
     c.hist = (double2 *)dmalloc(12 * c.W * sizeof(double2));
     c.cur = (double2 *)dmalloc(4 * c.W * sizeof(double2));
+    six32_ = (float2 *)dmalloc(6 * c.W * sizeof(float2));
     // transform buffers sized for FP64 so the precision can be switched in place
     c.lmb = dmalloc(3 * c.W * sizeof(double2));
     c.zbuf = dmalloc(6 * c.I * sizeof(double2));
@@ -428,6 +429,7 @@ void Channel::set_precision(int precision) {
     if (precision != 0 && precision != 1) throw Error(2, "precision must be 0 (fp64) or 1 (mixed)");
     if (precision == P_.precision) return;
     sync();
+    refresh_cur();  // while the mirror kernels still apply
     // the Lamb vector is kept in the transform precision: convert it in place
     std::vector<double> l(3 * c_.W * 2);
     if (P_.precision == 0) {
@@ -660,6 +662,7 @@ void Channel::set_velocity(const double *u, const double *v, const double *w) {
 
 // src/solver.cpp:379-425
 void Channel::init_post() {
+    cur_stale_ = false;  // the initial u, w, omega_x, omega_z are computed into cur
     const size_t W = c_.W;
     launch_vor_from_vel(c_, s_);
     launch_init_d2v(c_, s_);
@@ -726,7 +729,7 @@ void Channel::enqueue_step_body(int ring) {
         launch_solve_inv(c_, sp, Xb_, jobs_, njobs_, s_);
         launch_helmholtz_g(c_, sp, s_);
         OWT_HIP(hipStreamWaitEvent(s_, ev_join_, 0));
-        launch_horizontal(c_, sp, plain, s_);
+        launch_horizontal(c_, sp, plain, s_, f32_six() ? six32_ : nullptr);
     } else {
         launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, plain ? 2 : bops_, s_);
         launch_horizontal(c_, sp, plain, s_);
@@ -736,7 +739,8 @@ void Channel::enqueue_step_body(int ring) {
     // the six transformed fields: u, v, w, omega_x, omega_y, omega_z of the new level
     const size_t W = c_.W;
     const double2 *six[6] = {c_.cur, sp.vn, c_.cur + W, c_.cur + 2 * W, sp.gn, c_.cur + 3 * W};
-    launch_z_inverse(c_, six, 6, c_.zbuf, s_);
+    if (f32_six()) launch_z_inverse_f32(c_, six32_, s_);
+    else launch_z_inverse(c_, six, 6, c_.zbuf, s_);
     if (tangent_base_) launch_x_tangent(c_, (const double2 *)tangent_base_->c_.zbuf, s_);
     else launch_x_lamb(c_, s_);
     launch_z_forward(c_, s_);
@@ -772,7 +776,27 @@ void Channel::build_graph(int ring) {
     OWT_HIP(hipGraphInstantiate(&gexec_[ring], graph_[ring], nullptr, nullptr, 0));
 }
 
+// The mixed batched step (mirror kernels, NY <= 256, no tangent attached): the
+// horizontal stage writes the z inverse's inputs in FP32 instead of cur.
+bool Channel::f32_six() const { return P_.method == 1 && bops_ == 1 && use_mirror(c_) && !tangent_base_; }
+
+// u, w, omega_x, omega_z of the current level in FP64 (cur), after steps that
+// left them to the FP32 copies: the same mirror-half kernel, writing cur; the
+// stored wall-normal term it also writes goes to the free transform buffer
+void Channel::refresh_cur() {
+    if (!cur_stale_) return;
+    StepPtrs sp = step_ptrs(ring_);
+    sp.vn = const_cast<double2 *>(sp.v[0]);
+    sp.gn = const_cast<double2 *>(sp.g[0]);
+    sp.dvn = (double2 *)c_.zbuf;
+    launch_horizontal(c_, sp, false, s_, nullptr);
+    OWT_HIP(hipGetLastError());
+    OWT_HIP(hipStreamSynchronize(s_));
+    cur_stale_ = false;
+}
+
 void Channel::enqueue_step() {
+    if (f32_six()) cur_stale_ = true;
     if (P_.method == 1) {
         factor_operators();
         if (!gexec_[ring_]) build_graph(ring_);
@@ -800,11 +824,14 @@ void Channel::enqueue_kernel(int id) {
             if (!plain && bops_ == 1) launch_solve_inv(c_, sp, Xb_, jobs_, njobs_, s_);
             else launch_solve(c_, sp, F, P, jobs_, plain ? c_.npencil : njobs_, plain, plain ? 2 : bops_, s_);
             break;
-        case 1: launch_horizontal(c_, sp, plain, s_); break;
+        case 1: launch_horizontal(c_, sp, plain, s_, f32_six() ? six32_ : nullptr); break;
         case 2:
             launch_mean(c_, sp, plain ? Fp_ + (size_t)c_.NY * c_.NY : Fmean_, plain ? Pp_ + c_.NY : Pmean_, s_);
             break;
-        case 3: launch_z_inverse(c_, six, 6, c_.zbuf, s_); break;
+        case 3:
+            if (f32_six()) launch_z_inverse_f32(c_, six32_, s_);
+            else launch_z_inverse(c_, six, 6, c_.zbuf, s_);
+            break;
         case 4: launch_x_lamb(c_, s_); break;
         case 5: launch_z_forward(c_, s_); break;
         case 6: launch_nonlinear(c_, sp, plain, s_); break;
@@ -926,6 +953,7 @@ long Channel::steps_done() {
 void Channel::get_field(int f, double *out) {
     if (f < 0 || f >= F_COUNT) throw Error(2, "unknown field id " + std::to_string(f));
     sync();
+    refresh_cur();
     if (f == F_P) {
         pressure(out);
         return;
@@ -1129,6 +1157,8 @@ void Channel::save(const std::string &dir) { save_fields(dir, 1); }
 // force balances the shear of both walls)
 void Channel::statistics(double *prof, double *re_tau) {
     sync();
+    refresh_cur();
+    sync();
     double *d = (double *)c_.zbuf;
     launch_statistics(c_, c_.hist + (size_t)(H_V * 3 + ring_) * c_.W, d, s_);
     OWT_HIP(hipGetLastError());
@@ -1141,6 +1171,8 @@ void Channel::statistics(double *prof, double *re_tau) {
 }
 
 void Channel::checkpoint(const std::string &path) {
+    sync();
+    refresh_cur();
     sync();
     std::ofstream f(path, std::ios::binary);
     if (!f) throw Error(5, "cannot open " + path + " for writing");
@@ -1205,6 +1237,7 @@ void Channel::restore(const std::string &path) {
         OWT_HIP(hipStreamSynchronize(s_));
     }
     load(c_.cur, 4 * W * sizeof(double2));
+    cur_stale_ = false;
     load(c_.lmb, 3 * W * (c_.precision == 0 ? sizeof(double2) : sizeof(float2)));
     load(c_.means, 12 * N * sizeof(double2));
     load(c_.fric, 8 * sizeof(double));
@@ -1241,6 +1274,7 @@ void Channel::save_state(State &st) {
     OWT_HIP(hipMemcpyAsync(st.ctrl, c_.ctrl, 16 * sizeof(int), d2d, s_));
     st.ring = ring_;
     st.host_steps = host_steps_;
+    st.cur_stale = cur_stale_;
 }
 
 void Channel::load_state(const State &st) {
@@ -1255,10 +1289,12 @@ void Channel::load_state(const State &st) {
     OWT_HIP(hipMemcpyAsync(c_.ctrl, st.ctrl, 16 * sizeof(int), d2d, s_));
     ring_ = st.ring;
     host_steps_ = st.host_steps;
+    cur_stale_ = st.cur_stale;
 }
 
 void Channel::zero_state() {
     const size_t W = c_.W, N = c_.NY;
+    cur_stale_ = false;
     OWT_HIP(hipMemsetAsync(c_.hist, 0, 12 * W * sizeof(double2), s_));
     OWT_HIP(hipMemsetAsync(c_.d2v, 0, 3 * W * sizeof(double2), s_));
     OWT_HIP(hipMemsetAsync(c_.cur, 0, 4 * W * sizeof(double2), s_));

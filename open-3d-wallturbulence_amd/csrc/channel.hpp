@@ -132,11 +132,17 @@ class Channel {
         int *ctrl = nullptr;
         int ring = 0;
         long host_steps = 0;
+        bool cur_stale = false;
     };
     void save_state(State &st);
     void load_state(const State &st);
     void zero_state();
     void six_to_zbuf(int slot);
+    // the mixed batched step at NY <= 256 hands the z inverse its six fields in
+    // FP32 (six32_) and leaves u, w, omega_x, omega_z of cur as they were
+    // (cur_stale_); refresh_cur recomputes them in FP64 for the calls that read them
+    bool f32_six() const;
+    void refresh_cur();
     void adjoint_prepare();
     void adjoint_lamb(const double2 *base_zbuf, int slot);
     void adjoint_step(int ring, const double2 *base_zbuf, double seed, const double *Xv);
@@ -210,6 +216,8 @@ class Channel {
     std::vector<int> fmap_host_;
     // history ring phase: slot of the current level; one captured graph per phase
     int ring_ = 0;
+    float2 *six32_ = nullptr;
+    bool cur_stale_ = false;
     hipGraph_t graph_[3] = {nullptr, nullptr, nullptr};
     hipGraphExec_t gexec_[3] = {nullptr, nullptr, nullptr};
     long host_steps_ = 0;

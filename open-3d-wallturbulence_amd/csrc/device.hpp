@@ -171,6 +171,9 @@ size_t noalloc_scratch_bytes();
 // ---- FFT phase launches (fft_kernels.hip) ----
 // z inverse: nf wave fields -> nf z-padded fields; fields given as pointer table
 void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out, hipStream_t s);
+// the step's z inverse (FP32 transforms) from the six fields in FP32 (6 W float2,
+// written by launch_horizontal's out32) into Chan::zbuf
+void launch_z_inverse_f32(const Chan &c, const float2 *six, hipStream_t s);
 // fused x phase: 6 z-padded fields -> Lamb vector (3 z-padded fields)
 void launch_x_lamb(const Chan &c, hipStream_t s);
 // z forward: 3 z-padded fields -> 3 wave fields (c.lmb)
@@ -224,7 +227,9 @@ void launch_solve_inv(const Chan &c, const StepPtrs &sp, const double *X, const 
 void launch_helmholtz_g(const Chan &c, const StepPtrs &sp, hipStream_t s);
 // exact: the plain method's kernels in the reference's arithmetic (column-order
 // mat-vecs, separate roundings); otherwise the matrix-core kernels
-void launch_horizontal(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s);
+// out32 (the mirror kernel only, else ignored): the z inverse's six input fields
+// in FP32 go there instead of u, w, omega_x, omega_z into Chan::cur
+void launch_horizontal(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s, float2 *out32 = nullptr);
 // mean flow, friction force and the k = 0 pencil (Fg, Pg: the LU factor of Ag(0))
 void launch_mean(const Chan &c, const StepPtrs &sp, const double *Fg, const int *Pg, hipStream_t s);
 void launch_nonlinear(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s);

@@ -263,6 +263,11 @@ __device__ __forceinline__ C load_c(const double2 *p) {
     double2 v = *p;
     return mk<C>((decltype(C::x))v.x, (decltype(C::x))v.y);
 }
+template <typename C>
+__device__ __forceinline__ C load_c(const float2 *p) {
+    float2 v = *p;
+    return mk<C>((decltype(C::x))v.x, (decltype(C::x))v.y);
+}
 
 template <typename C>
 __device__ __forceinline__ const C *tw_of(const Chan &c, bool xdir);
@@ -296,9 +301,11 @@ struct PtrTable {
 // div divides the input (1 in the step); nyq = 1 also feeds the mode at index
 // NZH into the Nyquist slot NZ (0 in the step, which drops it): with div = NZ2
 // and nyq = 1 the kernel is the transpose of k_z_forward.
-template <typename T, int TJ, int TL, int LMAX>
-__device__ __forceinline__ void z_inverse_body(const Chan &c, const double2 *const *in, typename CT<T>::t *out, int nf, T div, int nyq,
-                                               const int bid) {
+// src_of(f): field f's input, double2 or float2 (the FP32 copies of the step's
+// six fields hold the rounding the load would apply)
+template <typename T, int TJ, int TL, int LMAX, typename SrcOf>
+__device__ __forceinline__ void z_inverse_src(const Chan &c, SrcOf src_of, typename CT<T>::t *out, int nf, T div, int nyq,
+                                              const int bid) {
     using C = typename CT<T>::t;
     extern __shared__ unsigned char smem[];
     C *tw = reinterpret_cast<C *>(smem);
@@ -313,7 +320,7 @@ __device__ __forceinline__ void z_inverse_body(const Chan &c, const double2 *con
     const bool valid = j < NY;
     const C *twg = tw_of<C>(c, false);
     for (int n = threadIdx.x; n < L; n += TJ * TL) tw[n] = twg[n];
-    const double2 *src = in[f];
+    const auto *src = src_of(f);
     const bool scale = div != (T)1;  // x / 1 == x: the step's call skips the divisions
     for (int n = tl; n < L; n += TL) {
         // slot n <- mode n (n < NZH); slots NZH..NZ are padding and the dropped z-Nyquist
@@ -329,8 +336,20 @@ __device__ __forceinline__ void z_inverse_body(const Chan &c, const double2 *con
         for (int n = tl; n < L; n += TL) dst[((size_t)n * NXH + i) * NY + j] = buf[n * TJ + jj];
 }
 template <typename T, int TJ, int TL, int LMAX>
+__device__ __forceinline__ void z_inverse_body(const Chan &c, const double2 *const *in, typename CT<T>::t *out, int nf, T div, int nyq,
+                                               const int bid) {
+    z_inverse_src<T, TJ, TL, LMAX>(c, [in](int f) { return in[f]; }, out, nf, div, nyq, bid);
+}
+template <typename T, int TJ, int TL, int LMAX>
 __global__ __launch_bounds__(TJ *TL) void k_z_inverse(Chan c, PtrTable in, typename CT<T>::t *out, int nf, T div, int nyq) {
     z_inverse_body<T, TJ, TL, LMAX>(c, in.p, out, nf, div, nyq, blockIdx.x);
+}
+// the step's z inverse in mixed precision from the six fields' FP32 copies
+// (6 W float2, field f at six + f W)
+template <int TJ, int TL, int LMAX>
+__global__ __launch_bounds__(TJ *TL) void k_z_inverse_f32in(Chan c, const float2 *six, float2 *out) {
+    const size_t W = c.W;
+    z_inverse_src<float, TJ, TL, LMAX>(c, [six, W](int f) { return six + (size_t)f * W; }, out, 6, 1.0f, 0, blockIdx.x);
 }
 template <typename T, int TJ, int TL, int LMAX>
 __global__ __launch_bounds__(TJ *TL) void k_z_inverse_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
@@ -935,6 +954,14 @@ void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out
             hipLaunchKernelGGL((k_z_inverse<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c, t,
                                (float2 *)out, nf, 1.0f, 0);
         });
+}
+
+void launch_z_inverse_f32(const Chan &c, const float2 *six, hipStream_t s) {
+    OWT_TUNED_DISPATCH_F32(c.NZ2, c.tune[T_Z_INVERSE_TILE], {
+        dim3 g(xcd_grid(c.NXH * ((c.NY + TJ - 1) / TJ) * 6));
+        hipLaunchKernelGGL((k_z_inverse_f32in<TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c, six,
+                           (float2 *)c.zbuf);
+    });
 }
 
 void launch_x_lamb(const Chan &c, hipStream_t s) {

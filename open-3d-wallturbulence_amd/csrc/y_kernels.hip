@@ -212,7 +212,12 @@ __global__ __launch_bounds__(512) void k_horizontal_mfma_ens(const EnsEntry *__r
 // ---------------------------------------------------------------------------
 constexpr int kMirrorMaxH = 128;
 
-__device__ __forceinline__ void horizontal_mirror_body(const Chan &c, const StepPtrs &sp, const int bid) {
+// out32 null: u, w, omega_x, omega_z into Chan::cur (FP64). Otherwise the six
+// fields the z inverse transforms (u, v, w, omega_x, omega_y, omega_z of the new
+// level, W each) go to out32 in FP32 instead, each value the FP32 rounding the z
+// inverse applies to the FP64 one as it loads it, and cur is left as it was
+// (the mean pencil's values, set by the mean-flow kernel, are copied over).
+__device__ __forceinline__ void horizontal_mirror_body(const Chan &c, const StepPtrs &sp, const int bid, float2 *out32) {
     __shared__ double2 sv[kMP][kMaxNY + 1], sg[kMP][kMaxNY + 1];
     const int N = c.NY, h = (N + 1) / 2, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int p0 = bid * kMP;
@@ -323,20 +328,44 @@ __device__ __forceinline__ void horizontal_mirror_body(const Chan &c, const Step
         piz[ct] = __ddiv_rn(pkz[ct], d);
         pir[ct] = __ddiv_rn(1.0, d);
     }
+    auto f32 = [](double2 a) { return make_float2((float)a.x, (float)a.y); };
     auto store = [&](int ct, int row, double2 d1v, double2 d2v, double2 d1g) {
         const int p = p0 + ct * 8 + bq;
         const size_t o = (size_t)p * N + row;
         sp.dvn[o] = d2v_store(c, d2v, sv[ct * 8 + bq][row], pk2[ct]);
         const double kx = pkx[ct], kz = pkz[ct], k2 = pk2[ct];
-        if (k2 < 1.e-8) return;  // the mean pencil is set by the mean-flow kernels
-        const double ix = pix[ct], iz = piz[ct], ir = pir[ct];
         const double2 vv = sv[ct * 8 + bq][row], gg = sg[ct * 8 + bq][row];
-        u[o] = csubc(imul(ix, d1v), imul(iz, gg));
-        wv[o] = caddc(imul(ix, gg), imul(iz, d1v));
+        if (k2 < 1.e-8) {
+            // the mean pencil is set by the mean-flow kernels
+            if (out32) {
+                out32[o] = f32(u[o]);
+                out32[c.W + o] = f32(vv);
+                out32[2 * c.W + o] = f32(wv[o]);
+                out32[3 * c.W + o] = f32(ox[o]);
+                out32[4 * c.W + o] = f32(gg);
+                out32[5 * c.W + o] = f32(oz[o]);
+            }
+            return;
+        }
+        const double ix = pix[ct], iz = piz[ct], ir = pir[ct];
+        const double2 uu = csubc(imul(ix, d1v), imul(iz, gg));
+        const double2 ww = caddc(imul(ix, gg), imul(iz, d1v));
         const double2 X = csubc(caddc(cscale(d1g, kx), cscale(d2v, kz)), cscale(vv, mul(kz, k2)));
         const double2 Yv = caddc(csubc(cscale(d1g, kz), cscale(d2v, kx)), cscale(vv, mul(kx, k2)));
-        ox[o] = imul(ir, X);
-        oz[o] = imul(ir, Yv);
+        const double2 xx = imul(ir, X), zz = imul(ir, Yv);
+        if (out32) {
+            out32[o] = f32(uu);
+            out32[c.W + o] = f32(vv);
+            out32[2 * c.W + o] = f32(ww);
+            out32[3 * c.W + o] = f32(xx);
+            out32[4 * c.W + o] = f32(gg);
+            out32[5 * c.W + o] = f32(zz);
+        } else {
+            u[o] = uu;
+            wv[o] = ww;
+            ox[o] = xx;
+            oz[o] = zz;
+        }
     };
 #pragma unroll
     for (int ct = 0; ct < 2; ++ct) {
@@ -357,12 +386,14 @@ __device__ __forceinline__ void horizontal_mirror_body(const Chan &c, const Step
         }
     }
 }
-__global__ __launch_bounds__(512) void k_horizontal_mirror(Chan c, StepPtrs sp) { horizontal_mirror_body(c, sp, blockIdx.x); }
+__global__ __launch_bounds__(512) void k_horizontal_mirror(Chan c, StepPtrs sp, float2 *out32) {
+    horizontal_mirror_body(c, sp, blockIdx.x, out32);
+}
 __global__ __launch_bounds__(512) void k_horizontal_mirror_ens(const EnsEntry *__restrict__ ents, const int *__restrict__ first, int nm) {
     const int m = ens_member(first, nm, blockIdx.x);
     const EnsEntry &e = ents[m];
     const int bid = blockIdx.x - first[m];
-    horizontal_mirror_body(e.c, e.sp, bid);
+    horizontal_mirror_body(e.c, e.sp, bid, nullptr);
 }
 
 // nonlinear source terms (src/solver.cpp:240-260) with DY1 (i kx Lx + i kz Lz)
@@ -1828,11 +1859,11 @@ static int blocks_tp(const Chan &c) { return (c.npencil + kTP - 1) / kTP; }
 
 // the batched method's derivative stages run on the matrix-core kernels at every
 // NY (NY <= kMaxNY); the plain method's in the reference's arithmetic
-void launch_horizontal(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s) {
+void launch_horizontal(const Chan &c, const StepPtrs &sp, bool exact, hipStream_t s, float2 *out32) {
     if (exact)
         hipLaunchKernelGGL(k_horizontal_exact<kTPH>, dim3((c.npencil + kTPH - 1) / kTPH), dim3(256), 0, s, c, sp);
     else if (use_mirror(c))
-        hipLaunchKernelGGL(k_horizontal_mirror, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
+        hipLaunchKernelGGL(k_horizontal_mirror, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp, out32);
     else if (c.NY <= 256)
         hipLaunchKernelGGL(k_horizontal_mfma<2>, dim3((c.npencil + kMP - 1) / kMP), dim3(512), 0, s, c, sp);
     else
