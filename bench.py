@@ -54,10 +54,11 @@ def stage_bytes(W, I, es):
         "solve": ((9 if es == 16 else 6) * W + W) * c,
         # the omega_y solves: 3 levels of omega_y and Hg in, omega_y out
         "helmholtz": (6 * W + W) * c,
-        # reads v, omega_y; writes u, w, omega_x, omega_z and DY2 v
-        "horizontal": (2 * W + 5 * W) * c,
+        # reads v, omega_y; writes DY2 v and u, w, omega_x, omega_z (FP64), or
+        # in mixed precision the z inverse's six inputs in FP32
+        "horizontal": (2 * W + 5 * W) * c if es == 16 else (3 * W) * c + 6 * W * es,
         "mean": 0,
-        "z_inverse": 6 * W * c + 6 * I * es,
+        "z_inverse": 6 * W * (c if es == 16 else es) + 6 * I * es,
         "x_lamb": 6 * I * es + 3 * I * es,
         "z_forward": 3 * I * es + 3 * W * es,
         "nonlinear": 3 * W * es + 2 * W * c,
@@ -65,9 +66,10 @@ def stage_bytes(W, I, es):
 
 
 # the kernel of each stage in the mixed-precision batched step at NY <= 256
-# (the derivative products and both solves in mirror form)
+# (the derivative products and both solves in mirror form, the z inverse from
+# the six fields' FP32 copies)
 STAGE_KERNEL = {"solve": "k_solve_mirror", "helmholtz": "k_helmholtz_mirror", "horizontal": "k_horizontal_mirror",
-                "mean": "k_mean", "z_inverse": "k_z_inverse", "x_lamb": "k_x_lamb", "z_forward": "k_z_forward",
+                "mean": "k_mean", "z_inverse": "k_z_inverse_f32in", "x_lamb": "k_x_lamb", "z_forward": "k_z_forward",
                 "nonlinear": "k_nonlinear_mirror"}
 
 

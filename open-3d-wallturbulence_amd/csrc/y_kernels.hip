@@ -271,6 +271,13 @@ __device__ __forceinline__ void horizontal_mirror_body(const Chan &c, const Step
             sv[sq][N - 1 - t] = live ? nvb : c2(0, 0);
             sg[sq][t] = live ? ngt : c2(0, 0);
             sg[sq][N - 1 - t] = live ? ngb : c2(0, 0);
+            if (out32 && live) {
+                // v and omega_y as staged: runs of consecutive rows
+                out32[c.W + so + t] = make_float2((float)nvt.x, (float)nvt.y);
+                out32[c.W + so + N - 1 - t] = make_float2((float)nvb.x, (float)nvb.y);
+                out32[4 * c.W + so + t] = make_float2((float)ngt.x, (float)ngt.y);
+                out32[4 * c.W + so + N - 1 - t] = make_float2((float)ngb.x, (float)ngb.y);
+            }
         }
         __syncthreads();
         fetch(k0 + kKC);
@@ -339,10 +346,8 @@ __device__ __forceinline__ void horizontal_mirror_body(const Chan &c, const Step
             // the mean pencil is set by the mean-flow kernels
             if (out32) {
                 out32[o] = f32(u[o]);
-                out32[c.W + o] = f32(vv);
                 out32[2 * c.W + o] = f32(wv[o]);
                 out32[3 * c.W + o] = f32(ox[o]);
-                out32[4 * c.W + o] = f32(gg);
                 out32[5 * c.W + o] = f32(oz[o]);
             }
             return;
@@ -354,11 +359,10 @@ __device__ __forceinline__ void horizontal_mirror_body(const Chan &c, const Step
         const double2 Yv = caddc(csubc(cscale(d1g, kz), cscale(d2v, kx)), cscale(vv, mul(kx, k2)));
         const double2 xx = imul(ir, X), zz = imul(ir, Yv);
         if (out32) {
+            // (v and omega_y went out as they were staged)
             out32[o] = f32(uu);
-            out32[c.W + o] = f32(vv);
             out32[2 * c.W + o] = f32(ww);
             out32[3 * c.W + o] = f32(xx);
-            out32[4 * c.W + o] = f32(gg);
             out32[5 * c.W + o] = f32(zz);
         } else {
             u[o] = uu;
