@@ -714,19 +714,19 @@ void Channel::enqueue_step_body(int ring) {
     const int *P = plain ? Pp_ : Pb_;
     if (plain) factor_set(Fp_, Pp_, c_.npencil, nullptr, 0, 2);
     // The mean flow (one block: pencil 0's u, w, omega_x, omega_z, the mean
-    // histories and the friction) reads and writes nothing the v and omega_y
-    // solves or the horizontal stage touch: it runs on a side stream forked from
-    // the step's stream (a parallel branch of the captured graph), so that its
-    // serial chain hides behind the v solves. The omega_y solves stay on the
-    // step's stream: beside the v solves, whose one-wave blocks take every CU
-    // slot that frees, their 512-thread blocks were starved (0.57 ms alone,
-    // 2.4 ms beside them).
+    // histories and the friction) and the v solves read and write nothing of the
+    // omega_y solves: they run on a side stream forked from the step's stream (a
+    // parallel branch of the captured graph), the mean first, and join before the
+    // horizontal stage. The omega_y solves go first on the step's stream: their
+    // 512-thread blocks take the CUs as the step starts and the v solves' one-wave
+    // blocks fill the slots they free (the other way round, the omega_y blocks
+    // were starved by the v solves: 0.57 ms alone, 2.4 ms beside them).
     if (!plain && bops_ == 1) {
         OWT_HIP(hipEventRecord(ev_fork_, s_));
         OWT_HIP(hipStreamWaitEvent(side_, ev_fork_, 0));
         launch_mean(c_, sp, Fmean_, Pmean_, side_);
+        launch_solve_inv(c_, sp, Xb_, jobs_, njobs_, side_);
         OWT_HIP(hipEventRecord(ev_join_, side_));
-        launch_solve_inv(c_, sp, Xb_, jobs_, njobs_, s_);
         launch_helmholtz_g(c_, sp, s_);
         OWT_HIP(hipStreamWaitEvent(s_, ev_join_, 0));
         launch_horizontal(c_, sp, plain, s_, f32_six() ? six32_ : nullptr);
