@@ -518,8 +518,14 @@ __device__ __forceinline__ void x_lamb_body(const Chan &c, const typename CT<T>:
     // one FP32 rounding of the quotient and an eighth of the instructions
     constexpr bool DIV = !std::is_same<T, float>::value;
     const T rs = (T)1 / sc, hrs = (T)0.5 / sc;
+    // FP32 with an even number of lines: the real z component of lines 2m and
+    // 2m+1 goes through one complex transform (lz_2m + i lz_2m+1) of the TJ/2
+    // line pairs, all threads taking part, and the two spectra are separated
+    // like pass 0's; half the butterflies of a transform per line
+    constexpr bool PAIR_LZ = !DIV && TJ % 2 == 0;
+    constexpr int NPASS = PAIR_LZ ? 1 : 2;
 #pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = 0; pass < NPASS; ++pass) {
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
             const int n = tl + e * TL;
@@ -550,6 +556,25 @@ __device__ __forceinline__ void x_lamb_body(const Chan &c, const typename CT<T>:
             }
         }
         __syncthreads();
+    }
+    if constexpr (PAIR_LZ) {
+        constexpr int TJ2 = TJ / 2, TL2 = 2 * TL;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+            const int n = tl + e * TL;
+            if (n < L) reinterpret_cast<T *>(&buf[n * TJ2 + jj / 2])[jj & 1] = a[1][e].x;
+        }
+        __syncthreads();
+        const int jp = threadIdx.x % TJ2, tp = threadIdx.x / TJ2;
+        fft_lines<C, TJ2, TL2, LMAX>(buf, tw, c.px, -1, jp, tp);
+        const int j0 = (tile % nyt) * TJ + 2 * jp;
+        const size_t p0 = (size_t)k2 * NXH * NY + j0;
+        for (int n = tp; n < NXH; n += TL2) {
+            const C zn = buf[n * TJ2 + jp], zm = buf[((L - n) % L) * TJ2 + jp];
+            const T ax = zn.x + zm.x, ay = zn.y - zm.y, bx = zn.y + zm.y, by = zm.x - zn.x;
+            if (j0 < NY) out[2 * c.I + p0 + (size_t)n * NY] = mk<C>(ax * hrs, ay * hrs);
+            if (j0 + 1 < NY) out[2 * c.I + p0 + 1 + (size_t)n * NY] = mk<C>(bx * hrs, by * hrs);
+        }
     }
 }
 template <typename T, int TJ, int TL, int LMAX>
@@ -937,6 +962,14 @@ size_t smem_bytes() {
         else { constexpr int TJ = 2, TL = 128, LMAX = 1152; BODY; }                  \
     } while (0)
 
+// The FP32 x phase pairs its lines (x_lamb_body): its one-line candidates run
+// the two-line shape of their length class instead
+int x_sel_f32(int L, int sel) {
+    if (L > 576) return sel == 1 ? 0 : sel;
+    if (L > 192) return sel == 2 ? 1 : sel;
+    return sel;
+}
+
 }  // namespace
 
 void launch_z_inverse(const Chan &c, const double2 *const *in, int nf, void *out, hipStream_t s) {
@@ -972,7 +1005,7 @@ void launch_x_lamb(const Chan &c, hipStream_t s) {
                                (const double2 *)c.zbuf, (double2 *)c.xbuf);
         });
     else
-        OWT_TUNED_DISPATCH_F32(c.NX2, c.tune[T_X_TILE], {
+        OWT_TUNED_DISPATCH_F32(c.NX2, x_sel_f32(c.NX2, c.tune[T_X_TILE]), {
             dim3 g(xcd_grid(c.NZ2 * ((c.NY + TJ - 1) / TJ)));
             hipLaunchKernelGGL((k_x_lamb<float, TJ, TL, LMAX>), g, dim3(TJ * TL), (smem_bytes<float, TJ, TL, LMAX>()), s, c,
                                (const float2 *)c.zbuf, (float2 *)c.xbuf);
@@ -1065,7 +1098,8 @@ void launch_peak3(const double *phys, size_t n, unsigned long long *out, hipStre
 int ens_key_fft(const EnsEntry &e, int stage, int *blocks) {
     const Chan &c = e.c;
     const int L = stage == ES_X_LAMB ? c.NX2 : c.NZ2;
-    const int sel = c.tune[stage == ES_Z_INVERSE ? T_Z_INVERSE_TILE : stage == ES_X_LAMB ? T_X_TILE : T_Z_FORWARD_TILE];
+    int sel = c.tune[stage == ES_Z_INVERSE ? T_Z_INVERSE_TILE : stage == ES_X_LAMB ? T_X_TILE : T_Z_FORWARD_TILE];
+    if (stage == ES_X_LAMB && c.precision == 1) sel = x_sel_f32(L, sel);
     int key = 0, tj = 1;
     if (c.precision == 0) OWT_TUNED_DISPATCH(L, sel, { key = LMAX * 64 + TJ; tj = TJ; });
     else OWT_TUNED_DISPATCH_F32(L, sel, { key = (1 << 20) + LMAX * 64 + TJ; tj = TJ; });
@@ -1077,7 +1111,8 @@ int ens_key_fft(const EnsEntry &e, int stage, int *blocks) {
 void launch_ens_fft(int stage, const EnsEntry *ents, const int *first, int nm, int grid, const EnsEntry &rep, hipStream_t s) {
     const Chan &c = rep.c;
     const int L = stage == ES_X_LAMB ? c.NX2 : c.NZ2;
-    const int sel = c.tune[stage == ES_Z_INVERSE ? T_Z_INVERSE_TILE : stage == ES_X_LAMB ? T_X_TILE : T_Z_FORWARD_TILE];
+    int sel = c.tune[stage == ES_Z_INVERSE ? T_Z_INVERSE_TILE : stage == ES_X_LAMB ? T_X_TILE : T_Z_FORWARD_TILE];
+    if (stage == ES_X_LAMB && c.precision == 1) sel = x_sel_f32(L, sel);
     const dim3 g(grid);
     if (c.precision == 0)
         OWT_TUNED_DISPATCH(L, sel, {
