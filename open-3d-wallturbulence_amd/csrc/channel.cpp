@@ -1496,14 +1496,21 @@ std::string Channel::autotune(int reps) {
         if (k == T_SOLVE_JOBS && P_.method == 0) ncand = 1;  // the plain solve has one shape
         // the derivative stages run on the matrix-core kernels, which have one shape
         if (k == T_HORIZONTAL_TP || k == T_NONLINEAR_TP) ncand = 1;
+        // the batched v solves share the step's start with the omega_y solves
+        // (parallel branches): their candidates are timed by whole steps
+        const bool whole_step = k == T_SOLVE_JOBS && P_.method == 1 && bops_ == 1;
+        auto run = [&]() {
+            if (whole_step) enqueue_step_body(ring_);
+            else enqueue_kernel(kernel[k]);
+        };
         double best = 1e300;
         int arg = 0;
         std::string times;
         for (int cand = 0; cand < ncand; ++cand) {
             c_.tune[k] = cand;
-            enqueue_kernel(kernel[k]);
+            run();
             OWT_HIP(hipEventRecord(a, s_));
-            for (int r = 0; r < reps; ++r) enqueue_kernel(kernel[k]);
+            for (int r = 0; r < reps; ++r) run();
             OWT_HIP(hipEventRecord(b, s_));
             OWT_HIP(hipEventSynchronize(b));
             float t = 0;
