@@ -796,7 +796,7 @@ void Channel::refresh_cur() {
 }
 
 void Channel::enqueue_step() {
-    if (f32_six()) cur_stale_ = true;
+    cur_stale_ = f32_six();  // a step that writes cur leaves it current
     if (P_.method == 1) {
         factor_operators();
         if (!gexec_[ring_]) build_graph(ring_);

@@ -97,6 +97,7 @@ class Channel {
     void advance_ring() {
         ring_ = (ring_ + 1) % 3;
         ++host_steps_;
+        cur_stale_ = false;  // the grouped launches write u, w, omega_x, omega_z into cur
     }
     unsigned generation() const { return generation_; }
     void finish();
