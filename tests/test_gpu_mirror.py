@@ -171,3 +171,25 @@ def test_mixed_checkpoint_restores_exactly(owt_mod, tmp_path):
                 assert np.array_equal(b.field(n), r), (prec, n)
         finally:
             b.close()
+
+
+def test_mixed_step_above_ny256_against_the_oracle(owt_mod, oracle_mod):
+    """Above NY 256 the mixed step runs the full-product kernels and the full
+    inverses, with the stored wall-normal terms in the mixed form: the Retau-550
+    case file's wall-normal grid (NY 288) on 16 x 16, 10 steps against the FP64
+    oracle with the mixed path's bars."""
+    c = FLOORS["case550_wall_16"]["case"]
+    p = {k: c[k] for k in KEYS}
+    o = oracle_mod.Oracle(p["nu"], p["dt"], p["dkx"], p["dkz"], p["alpha_mesh"], p["NX"], p["NY"], p["NZ"])
+    o.init(2, seed=0, amp=c["amp"])
+    o.step(10)
+    print("[gpu] mixed 10 steps case550_wall_16 (NY 288)", flush=True)
+    with owt_mod.Channel(params=dict(p, precision=1, method=1)) as ch:
+        ch.init_fields(2, seed=0, amp=c["amp"])
+        ch.step(10)
+        errs = {n: rel(ch.field(n), o.field(n)) for n in ("u", "v", "w")}
+        d = ch.diag()
+    print("[mirror] NY 288 mixed, 10 steps vs oracle: " + " ".join(f"{n}={e:.2e}" for n, e in errs.items()), flush=True)
+    assert errs["u"] <= 1e-6 and errs["v"] <= 2e-5 and errs["w"] <= 2e-5, errs
+    do = o.diag()
+    assert np.abs(d[:, 0] - do[:, 0]).max() <= 1e-6 * np.abs(do[:, 0]).max()
